@@ -1,0 +1,64 @@
+"""Builds libblazert_amd.so in-tree with hipcc for gfx950.
+
+Sources: csrc/kernels.hip (device code), csrc/scene.cpp (host builder + C ABI). Both are compiled with
+-ffp-contract=off so that no multiply-add is fused (the results must round exactly like the reference's
+separate multiply and add), and device code with packed-FP32 instructions disabled.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libblazert_amd.so")
+ARCH = "gfx950"
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-fno-gpu-flush-denormals-to-zero"]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[-1]}")
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles every source (always) and links libblazert_amd.so next to this file."""
+    cc = hipcc()
+    tmp = tempfile.mkdtemp(prefix="brt_build_")
+    try:
+        objs = []
+        k_obj = os.path.join(tmp, "kernels.o")
+        _run([cc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "kernels.hip"), "-o", k_obj])
+        objs.append(k_obj)
+        s_obj = os.path.join(tmp, "scene.o")
+        _run([cc, *COMMON, "-x", "c++", "-pthread", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-c", os.path.join(CSRC, "scene.cpp"), "-o", s_obj])
+        objs.append(s_obj)
+        out_tmp = LIB + ".tmp"
+        _run([cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-pthread", *objs, "-o", out_tmp])
+        os.replace(out_tmp, LIB)
+        if verbose:
+            print("built", LIB)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

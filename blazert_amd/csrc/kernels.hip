@@ -1,0 +1,701 @@
+// HIP kernels for gfx950: closest-hit traversal of one geometry-type BVH per launch.
+//
+// FP64 path (the drop-in): bit-identical to blazeRT's traverse() (blazert/bvh/accel.h:79-120). The reference
+// pops a node, slab-tests it against [tmin, current hit] and pushes far then near. Here an internal record holds
+// both child boxes: the near child is tested at the parent (the reference tests it at the very next pop, with the
+// same hit distance), and the far child goes on the stack with the h-independent half of its slab test already
+// applied and its entry distance as the key, so the pop-time check `!(h < key)` reproduces the reference's
+// decision exactly, NaN propagation of std::max/std::min included (blazert/bvh/intersect.h:32-51).
+//
+// FP32 path (split precision): the same DFS with slab tests on outward-rounded FP32 child boxes and a per-ray
+// absolute error bound, so it visits a superset of the FP64 path's nodes; primitive tests stay FP64.
+//
+// All arithmetic is compiled with -ffp-contract=off so that products and sums round exactly as the reference's.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "engine.h"
+
+namespace brt {
+
+namespace {
+
+constexpr double L1 = 1.0 + 4.0 * DBL_EPSILON;  // far-plane widening of blazert/bvh/intersect.h:40
+constexpr uint32_t PREC_FP64 = 0;
+constexpr uint32_t FLAG_INOUT = 1u;
+constexpr uint32_t FLAG_ANY_HIT = 2u;
+constexpr int STACK_CAP = 64;
+
+__device__ __forceinline__ double std_min(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double std_max(double a, double b) { return (a < b) ? b : a; }
+
+// blazert/datatypes.h:83-85
+__device__ __forceinline__ double product_sign(double x, double y) {
+  const unsigned long long ux = (unsigned long long)__double_as_longlong(x);
+  const unsigned long long uy = (unsigned long long)__double_as_longlong(y);
+  return __longlong_as_double((long long)(ux ^ (uy & 0x8000000000000000ull)));
+}
+
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+
+struct Ray {
+  double o[3];
+  double d[3];    // normalize(raw direction)
+  double inv[3];  // 1 / raw direction
+  uint32_t sign;  // bit i: raw direction[i] < 0
+  double tmin, tmax;
+};
+
+// blazert/ray.h:56-65 with normalize(v) = v * (len != 0 ? 1/len : 0)
+__device__ __forceinline__ void make_ray(const RaysView &R, uint64_t i, Ray &r) {
+  const double raw[3] = {R.dx[i], R.dy[i], R.dz[i]};
+  r.o[0] = R.ox[i];
+  r.o[1] = R.oy[i];
+  r.o[2] = R.oz[i];
+  const double len = dsqrt(raw[0] * raw[0] + raw[1] * raw[1] + raw[2] * raw[2]);
+  const double il = (len != 0.0) ? 1.0 / len : 0.0;
+  r.sign = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r.d[k] = raw[k] * il;
+    r.inv[k] = 1.0 / raw[k];
+    r.sign |= (raw[k] < 0.0 ? 1u : 0u) << k;
+  }
+  r.tmin = R.tmin ? R.tmin[i] : 0.0;
+  r.tmax = R.tmax ? R.tmax[i] : DBL_MAX;
+}
+
+// ------------------------------------------------------------------------------------------------ slab tests
+// Exact FP64 slab test of one child box, split into its hit-distance independent part. Returns false when the
+// box fails for every hit distance; otherwise key is the value the current hit distance h must not be below
+// (pass iff !(h < key)); key = -inf when a NaN far-plane term removes h from the reference's min chain.
+__device__ __forceinline__ bool slab64(const double *b, const Ray &r, double &key) {
+  double acc = r.tmin;
+  double m[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const bool s = (r.sign >> i) & 1u;
+    const double nearp = s ? b[3 + i] : b[i];
+    const double farp = s ? b[i] : b[3 + i];
+    const double mn = (nearp - r.o[i]) * r.inv[i];
+    m[i] = ((farp - r.o[i]) * r.inv[i]) * L1;
+    acc = std_max(mn, acc);
+  }
+  const bool n0 = m[0] != m[0], n1 = m[1] != m[1], n2 = m[2] != m[2];
+  double M;
+  bool hdep = false;
+  if (n2) {
+    return false;
+  } else if (n1) {
+    M = m[2];
+  } else if (n0) {
+    M = std_min(m[2], m[1]);
+  } else {
+    M = fmin(fmin(m[0], m[1]), m[2]);
+    hdep = true;
+  }
+  if (!(acc <= M)) return false;
+  key = hdep ? acc : -__builtin_inf();
+  return true;
+}
+
+// Conservative FP32 slab test: per axis lower/upper bounds of the FP64 entry/exit distances.
+struct Ray32 {
+  float inv[3];
+  float offn[3];  // rounded-down (-o*inv - E)
+  float offf[3];  // rounded-up   (-o*inv + E)
+  float tmin;     // rounded down
+  uint32_t sign;
+};
+
+// Adjacent float toward -inf / +inf (finite or infinite input, never NaN here).
+__device__ __forceinline__ float float_below(float f) {
+  if (f == 0.0f) return __int_as_float((int)0x80000001);
+  const int b = __float_as_int(f);
+  return __int_as_float(f > 0.0f ? b - 1 : b + 1);
+}
+__device__ __forceinline__ float float_above(float f) {
+  if (f == 0.0f) return __int_as_float(1);
+  const int b = __float_as_int(f);
+  return __int_as_float(f > 0.0f ? b + 1 : b - 1);
+}
+// FP64 -> FP32 rounded toward -inf / +inf.
+__device__ __forceinline__ float f32_down(double x) {
+  const float f = (float)x;
+  return ((double)f > x) ? float_below(f) : f;
+}
+__device__ __forceinline__ float f32_up(double x) {
+  const float f = (float)x;
+  return ((double)f < x) ? float_above(f) : f;
+}
+
+// E_i = 5 * 2^-24 * |inv_i| * (S + |o_i|) bounds |fma(b32, inv32, off32) - (b - o) inv| for every |b| <= S,
+// including the FP64 reference's own rounding and its 1 + 4 eps far-plane factor.
+__device__ __forceinline__ void make_ray32(const Ray &r, double S, Ray32 &q) {
+  q.sign = r.sign;
+  q.tmin = f32_down(r.tmin);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const double inv = r.inv[i];
+    const double ainv = fabs(inv);
+    if (!(ainv <= 1e30)) {  // zero / tiny direction component: this axis never culls
+      q.inv[i] = 0.0f;
+      q.offn[i] = -__builtin_inff();
+      q.offf[i] = __builtin_inff();
+      continue;
+    }
+    const double off = -(r.o[i] * inv);
+    const double E = 5.0 * 5.9604644775390625e-08 * ainv * (S + fabs(r.o[i]));
+    q.inv[i] = (float)inv;
+    q.offn[i] = f32_down(off - E);
+    q.offf[i] = f32_up(off + E);
+  }
+}
+
+__device__ __forceinline__ bool slab32(const float *b, const Ray32 &q, float &key) {
+  float A = q.tmin;
+  float M = __builtin_inff();
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const bool s = (q.sign >> i) & 1u;
+    const float nearp = s ? b[3 + i] : b[i];
+    const float farp = s ? b[i] : b[3 + i];
+    A = fmaxf(A, __builtin_fmaf(nearp, q.inv[i], q.offn[i]));
+    M = fminf(M, __builtin_fmaf(farp, q.inv[i], q.offf[i]));
+  }
+  key = A;
+  return A <= M;
+}
+
+// ------------------------------------------------------------------------------------------------ intersectors
+// Closest-hit state of one traversal (prepare_traversal: hit = tmax, uv = 0, prim invalid).
+struct Hit {
+  double h;
+  double u, v;
+  uint32_t leaf_idx;  // leaf-order index of the current hit
+};
+
+// blazert/primitives/trimesh.h:153-181
+__device__ __forceinline__ bool isect(const TriRec &T, const Ray &r, Hit &s, uint32_t idx) {
+  const double tol = 4.0 * DBL_EPSILON;
+  const double c0 = T.v0[0] - r.o[0], c1 = T.v0[1] - r.o[1], c2 = T.v0[2] - r.o[2];
+  const double rx = r.d[1] * c2 - r.d[2] * c1;
+  const double ry = r.d[2] * c0 - r.d[0] * c2;
+  const double rz = r.d[0] * c1 - r.d[1] * c0;
+  const double *e1 = T.e1, *e2 = T.e2;
+  const double nx = e1[1] * e2[2] - e1[2] * e2[1];
+  const double ny = e1[2] * e2[0] - e1[0] * e2[2];
+  const double nz = e1[0] * e2[1] - e1[1] * e2[0];
+  const double det = nx * r.d[0] + ny * r.d[1] + nz * r.d[2];
+  const double abs_det = fabs(det);
+  const double u = product_sign(rx * e2[0] + ry * e2[1] + rz * e2[2], det);
+  const double v = product_sign(rx * e1[0] + ry * e1[1] + rz * e1[2], det);
+  const double w = abs_det - u - v;
+  if (u >= -tol && v >= -tol && w >= -tol) {
+    const double t = product_sign(nx * c0 + ny * c1 + nz * c2, det);
+    if (t >= abs_det * r.tmin && abs_det * s.h > t) {
+      const double inv_det = 1.0 / abs_det;
+      s.h = t * inv_det;
+      s.u = u * inv_det;
+      s.v = v * inv_det;
+      s.leaf_idx = idx;
+      return true;
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ double norm_scaled(double t, const double *d) {
+  const double x = t * d[0], y = t * d[1], z = t * d[2];
+  return dsqrt(x * x + y * y + z * z);
+}
+
+// blazert/primitives/sphere.h:364-406
+__device__ __forceinline__ bool isect(const SphRec &S, const Ray &r, Hit &s, uint32_t idx) {
+  const double v0 = r.o[0] - S.c[0], v1 = r.o[1] - S.c[1], v2 = r.o[2] - S.c[2];
+  const double A = r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2];
+  const double B = 2.0 * (v0 * r.d[0] + v1 * r.d[1] + v2 * r.d[2]);
+  const double C = (v0 * v0 + v1 * v1 + v2 * v2) - S.r * S.r;
+  const double D = B * B - 4.0 * A * C;
+  if (D < 0.0) return false;
+  const double Q = dsqrt(D);
+  const double rcpA = 1.0 / A;
+  const double t0 = 0.5 * rcpA * (-B - Q);
+  const double t1 = 0.5 * rcpA * (-B + Q);
+  if ((t0 > r.tmin) && (t0 < s.h)) {
+    s.h = norm_scaled(t0, r.d);
+    s.leaf_idx = idx;
+    return true;
+  }
+  if ((t1 > r.tmin) && (t1 < s.h)) {
+    s.h = norm_scaled(t1, r.d);
+    s.leaf_idx = idx;
+    return true;
+  }
+  return false;
+}
+
+// trans(R) * v and R * v for row-major R
+__device__ __forceinline__ void matTvec(const double *R, const double *v, double *o) {
+  o[0] = R[0] * v[0] + R[3] * v[1] + R[6] * v[2];
+  o[1] = R[1] * v[0] + R[4] * v[1] + R[7] * v[2];
+  o[2] = R[2] * v[0] + R[5] * v[1] + R[8] * v[2];
+}
+__device__ __forceinline__ void matvec(const double *R, const double *v, double *o) {
+  o[0] = R[0] * v[0] + R[1] * v[1] + R[2] * v[2];
+  o[1] = R[3] * v[0] + R[4] * v[1] + R[5] * v[2];
+  o[2] = R[6] * v[0] + R[7] * v[1] + R[8] * v[2];
+}
+
+// Plane: local frame, t = -o_z / d_z, interior / edge / corner branches (blazert/primitives/plane.h:180-270).
+// Returns the branch (0 = miss, 1 interior, 2..5 edges, 6..9 corners) for the given [tnear, tfar].
+__device__ __forceinline__ int plane_branch(const PlaRec &P, const Ray &r, double tnear, double tfar, double *org,
+                                            double *dir, double &t1, bool check_t) {
+  const double oc[3] = {r.o[0] - P.c[0], r.o[1] - P.c[1], r.o[2] - P.c[2]};
+  matTvec(P.R, oc, org);
+  matTvec(P.R, r.d, dir);
+  t1 = -org[2] / dir[2];
+  const double X = org[0] + t1 * dir[0];
+  const double Y = org[1] + t1 * dir[1];
+  const double x_min = -P.dx / 2, x_max = P.dx / 2, y_min = -P.dy / 2, y_max = P.dy / 2;
+  const bool tok = !check_t || ((t1 > tnear) && (t1 < tfar));
+  if (!tok) return 0;
+  if ((X > x_min) && (X < x_max) && (Y > y_min) && (Y < y_max)) return 1;
+  if ((X == x_min) && (X < x_max) && (Y > y_min) && (Y < y_max)) return 2;
+  if ((X > x_min) && (X == x_max) && (Y > y_min) && (Y < y_max)) return 3;
+  if ((X > x_min) && (X < x_max) && (Y == y_min) && (Y < y_max)) return 4;
+  if ((X > x_min) && (X < x_max) && (Y > y_min) && (Y == y_max)) return 5;
+  if ((X == x_min) && (X < x_max) && (Y == y_min) && (Y < y_max)) return 6;
+  if ((X == x_min) && (X < x_max) && (Y > y_min) && (Y == y_max)) return 7;
+  if ((X > x_min) && (X == x_max) && (Y == y_min) && (Y < y_max)) return 8;
+  if ((X > x_min) && (X == x_max) && (Y > y_min) && (Y == y_max)) return 9;
+  return 0;
+}
+
+__device__ __forceinline__ bool isect(const PlaRec &P, const Ray &r, Hit &s, uint32_t idx) {
+  double org[3], dir[3], t1;
+  const int br = plane_branch(P, r, r.tmin, s.h, org, dir, t1, true);
+  if (!br) return false;
+  s.h = norm_scaled(t1, dir);
+  s.leaf_idx = idx;
+  return true;
+}
+
+// Plane normal of the accepted branch (before the final normalize of post_traversal).
+__device__ void plane_normal(const PlaRec &P, const Ray &r, double *nrm) {
+  double org[3], dir[3], t1;
+  const int br = plane_branch(P, r, 0.0, 0.0, org, dir, t1, false);
+  const double sgn = org[2] / fabs(org[2]);
+  if (br == 1) {
+    const double ez[3] = {0.0, 0.0, 1.0};
+    double rz[3];
+    matvec(P.R, ez, rz);
+    nrm[0] = sgn * rz[0];
+    nrm[1] = sgn * rz[1];
+    nrm[2] = sgn * rz[2];
+    return;
+  }
+  const double lx[10] = {0, 0, -1.0, 1.0, 0.0, 0.0, -1.0, -1.0, 1.0, 1.0};
+  const double ly[10] = {0, 0, 0.0, 0.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0};
+  const double l[3] = {lx[br], ly[br], sgn};
+  matvec(P.R, l, nrm);
+}
+
+// Cylinder (blazert/primitives/cylinder.h:181-495). No hit-distance checks: the last accepted cylinder wins.
+// Returns true on a hit with the parametric t and the (pre-normalisation) normal.
+struct CylOut {
+  double t;
+  double n[3];
+};
+
+__device__ __forceinline__ void cyl_ng(const double *R, const double *nn, double *out) {
+  double rn[3];
+  matvec(R, nn, rn);
+  const double len = dsqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+  const double is = 1.0 / len;
+  out[0] = rn[0] * is;
+  out[1] = rn[1] * is;
+  out[2] = rn[2] * is;
+}
+
+__device__ bool cyl_eval(const CylRec &Cy, const Ray &r, double *dir, CylOut &out, bool want_normal) {
+  const double oc[3] = {r.o[0] - Cy.c[0], r.o[1] - Cy.c[1], r.o[2] - Cy.c[2]};
+  double org[3];
+  matTvec(Cy.R, oc, org);
+  matTvec(Cy.R, r.d, dir);
+  const double x0 = org[0], y0 = org[1], z0 = org[2];
+  const double j = dir[0], k = dir[1], l = dir[2];
+  const double a = Cy.a, b = Cy.b, h = Cy.h;
+  const double zt = h / 2.0, zb = -h / 2.0;
+
+  // shell quadratic shared by every region
+  auto shell = [&](bool ordered, bool &hit) {
+    hit = false;
+    const double A = a * a * k * k + b * b * j * j;
+    const double B = a * a * k * y0 + b * b * j * x0;
+    const double C = a * a * k * k + b * b * j * j - j * j * y0 * y0 + 2 * j * k * x0 * y0 - k * k * x0 * x0;
+    if (C < 0) return true;  // reference returns false immediately
+    const double s0 = 1.0 / A * (-B + a * b * dsqrt(C));
+    const double s1 = -1.0 / A * (B + a * b * dsqrt(C));
+    const double i0z = org[2] + dir[2] * s0;
+    const double i1z = org[2] + dir[2] * s1;
+    if ((s1 > 0) && (!ordered || s1 < s0) && (i1z <= zt) && (i1z >= zb)) {
+      out.t = s1;
+      if (want_normal) {
+        const double nn[3] = {2 * (org[0] + dir[0] * s1) / (a * a), 2 * (org[1] + dir[1] * s1) / (b * b), 0};
+        cyl_ng(Cy.R, nn, out.n);
+      }
+      hit = true;
+      return true;
+    }
+    if ((s0 > 0) && (!ordered || s0 < s1) && (i0z <= zt) && (i0z >= zb)) {
+      out.t = s0;
+      if (want_normal) {
+        const double nn[3] = {2 * (org[0] + dir[0] * s0) / (a * a), 2 * (org[1] + dir[1] * s0) / (b * b), 0};
+        cyl_ng(Cy.R, nn, out.n);
+      }
+      hit = true;
+      return true;
+    }
+    return false;  // fall through
+  };
+  auto cap = [&](double zc, double sgn) {
+    out.t = (zc - z0) / l;
+    if (want_normal) {
+      const double nz[3] = {0.0, 0.0, sgn};
+      cyl_ng(Cy.R, nz, out.n);
+    }
+  };
+
+  if (z0 > zt || z0 < zb) {  // areas 1 and 2
+    const bool top = z0 > zt;
+    if (l == 0) return false;
+    const double t0 = top ? (zt - z0) / l : (zb - z0) / l;
+    if (t0 < 0) return false;
+    const double ix = org[0] + t0 * dir[0], iy = org[1] + t0 * dir[1];
+    if (b * b * ix * ix + a * a * iy * iy <= a * a * b * b) {
+      cap(top ? zt : zb, top ? 1.0 : -1.0);
+      return true;
+    }
+    bool hit;
+    if (shell(true, hit)) return hit;
+  } else if ((z0 > zb) && (z0 < zt) && (x0 * x0 * b * b + y0 * y0 * a * a > a * a * b * b)) {  // area 3
+    bool hit;
+    if (shell(true, hit)) return hit;
+  }
+  if ((z0 >= zb) && (z0 <= zt) && (x0 * x0 * b * b + y0 * y0 * a * a <= a * a * b * b)) {  // area 4
+    if (l == 0) {
+      if ((z0 == zb) || (z0 == zt)) return false;
+      bool hit;
+      if (shell(false, hit)) return hit;
+    } else if ((j == 0.0) && (k == 0.0)) {
+      if (b * b * x0 * x0 + a * a * y0 * y0 == a * a * b * b) return false;
+      if (l < 0) {
+        cap(zb, -1.0);
+        return true;
+      }
+      if (l > 0) {
+        cap(zt, 1.0);
+        return true;
+      }
+    } else {
+      const double A = a * a * k * k + b * b * j * j;
+      const double B = a * a * k * y0 + b * b * j * x0;
+      const double C = a * a * k * k + b * b * j * j - j * j * y0 * y0 + 2 * j * k * x0 * y0 - k * k * x0 * x0;
+      if (C < 0) return false;
+      const double s0 = 1.0 / A * (-B + a * b * dsqrt(C));
+      const double s1 = -1.0 / A * (B + a * b * dsqrt(C));
+      const double i0z = org[2] + dir[2] * s0;
+      const double i1z = org[2] + dir[2] * s1;
+      if ((s1 > 0) && (i1z <= zt) && (i1z >= zb)) {
+        out.t = s1;
+        if (want_normal) {
+          const double nn[3] = {2 * (org[0] + dir[0] * s1) / (a * a), 2 * (org[1] + dir[1] * s1) / (b * b), 0};
+          cyl_ng(Cy.R, nn, out.n);
+        }
+        return true;
+      }
+      if ((s0 > 0) && (i0z <= zt) && (i0z >= zb)) {
+        out.t = s0;
+        if (want_normal) {  // reference stores the un-rotated normal here (cylinder.h:462-463)
+          out.n[0] = 2 * (org[0] + dir[0] * s0) / (a * a);
+          out.n[1] = 2 * (org[1] + dir[1] * s0) / (b * b);
+          out.n[2] = 0;
+        }
+        return true;
+      }
+      if (l < 0) {
+        cap(zb, -1.0);
+        return true;
+      }
+      if (l > 0) {
+        cap(zt, 1.0);
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ bool isect(const CylRec &Cy, const Ray &r, Hit &s, uint32_t idx) {
+  double dir[3];
+  CylOut o;
+  if (!cyl_eval(Cy, r, dir, o, false)) return false;
+  s.h = norm_scaled(o.t, dir);
+  s.leaf_idx = idx;
+  return true;
+}
+
+__device__ __forceinline__ void normalize3(double *v) {
+  const double len = dsqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const double il = (len != 0.0) ? 1.0 / len : 0.0;
+  v[0] *= il;
+  v[1] *= il;
+  v[2] *= il;
+}
+
+// post_traversal normals (trimesh.h:133-143, sphere.h:336-343, plane.h:153-159, cylinder.h:154-160)
+__device__ void post_normal(const TriRec *, const BvhView &bvh, const Ray &, const Hit &s, uint32_t pid, double *n) {
+  const uint32_t *f = bvh.faces + 3ull * pid;
+  const double *n0 = bvh.vnormals + 3ull * f[0];
+  const double *n1 = bvh.vnormals + 3ull * f[1];
+  const double *n2 = bvh.vnormals + 3ull * f[2];
+  const double w = (1.0 - s.u) - s.v;
+#pragma unroll
+  for (int k = 0; k < 3; k++) n[k] = (w * n0[k] + s.u * n1[k]) + s.v * n2[k];
+  normalize3(n);
+}
+__device__ void post_normal(const SphRec *recs, const BvhView &, const Ray &r, const Hit &s, uint32_t, double *n) {
+  const SphRec S = recs[s.leaf_idx];
+#pragma unroll
+  for (int k = 0; k < 3; k++) n[k] = (r.o[k] + s.h * r.d[k]) - S.c[k];
+  normalize3(n);
+}
+__device__ void post_normal(const PlaRec *recs, const BvhView &, const Ray &r, const Hit &s, uint32_t, double *n) {
+  plane_normal(recs[s.leaf_idx], r, n);
+  normalize3(n);
+}
+__device__ void post_normal(const CylRec *recs, const BvhView &, const Ray &r, const Hit &s, uint32_t, double *n) {
+  double dir[3];
+  CylOut o;
+  cyl_eval(recs[s.leaf_idx], r, dir, o, true);
+  n[0] = o.n[0];
+  n[1] = o.n[1];
+  n[2] = o.n[2];
+  normalize3(n);
+}
+
+// ------------------------------------------------------------------------------------------------ traversal
+template <class Rec, bool SPLIT>
+__device__ __forceinline__ bool traverse(const BvhView &bvh, const Ray &r, const Ray32 &q, bool any_hit, Hit &s) {
+  const Rec *recs = reinterpret_cast<const Rec *>(bvh.prims);
+  uint32_t st_ref[STACK_CAP];
+  uint32_t st_cnt[STACK_CAP];
+  double st_key[STACK_CAP];
+  float st_key32[STACK_CAP];
+  int sp = 0;
+
+  uint32_t cur = bvh.root_ref, cnt = bvh.root_cnt;
+  bool active;
+  if (SPLIT) {
+    float k32;
+    const float rb[6] = {bvh.root_min32[0], bvh.root_min32[1], bvh.root_min32[2],
+                         bvh.root_max32[0], bvh.root_max32[1], bvh.root_max32[2]};
+    active = slab32(rb, q, k32) && !(f32_up(s.h) < k32);
+  } else {
+    double key;
+    const double rb[6] = {bvh.root_min[0], bvh.root_min[1], bvh.root_min[2],
+                          bvh.root_max[0], bvh.root_max[1], bvh.root_max[2]};
+    active = slab64(rb, r, key) && !(s.h < key);
+  }
+  float h32 = SPLIT ? f32_up(s.h) : 0.0f;
+  bool done = false;
+
+  while (active) {
+    if (!(cur & LEAF_BIT)) {
+      bool near_ok, far_ok;
+      uint32_t nref, ncnt, fref, fcnt;
+      double fkey = 0.0;
+      float fkey32 = 0.0f;
+      if (SPLIT) {
+        const Node32 nd = bvh.nodes32[cur];
+        const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
+        const uint32_t nc = (q.sign >> axis) & 1u;
+        float k0, k1;
+        const bool p0 = slab32(nd.b, q, k0);
+        const bool p1 = slab32(nd.b + 6, q, k1);
+        const bool pn = nc ? p1 : p0, pf = nc ? p0 : p1;
+        const float kn = nc ? k1 : k0, kf = nc ? k0 : k1;
+        near_ok = pn && !(h32 < kn);
+        far_ok = pf;
+        fkey32 = kf;
+        nref = nd.ref[nc];
+        ncnt = nd.cnt[nc] & CNT_MASK;
+        fref = nd.ref[1 - nc];
+        fcnt = nd.cnt[1 - nc] & CNT_MASK;
+      } else {
+        const Node64 nd = bvh.nodes64[cur];
+        const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
+        const uint32_t nc = (r.sign >> axis) & 1u;
+        double k0 = 0.0, k1 = 0.0;
+        const bool p0 = slab64(nd.b, r, k0);
+        const bool p1 = slab64(nd.b + 6, r, k1);
+        const bool pn = nc ? p1 : p0, pf = nc ? p0 : p1;
+        const double kn = nc ? k1 : k0, kf = nc ? k0 : k1;
+        near_ok = pn && !(s.h < kn);
+        far_ok = pf;
+        fkey = kf;
+        nref = nd.ref[nc];
+        ncnt = nd.cnt[nc] & CNT_MASK;
+        fref = nd.ref[1 - nc];
+        fcnt = nd.cnt[1 - nc] & CNT_MASK;
+      }
+      if (far_ok) {
+        st_ref[sp] = fref;
+        st_cnt[sp] = fcnt;
+        if (SPLIT) st_key32[sp] = fkey32;
+        else st_key[sp] = fkey;
+        sp++;
+      }
+      if (near_ok) {
+        cur = nref;
+        cnt = ncnt;
+        continue;
+      }
+    } else {
+      const uint32_t first = cur & ~LEAF_BIT;
+      bool leaf_hit = false;
+      for (uint32_t p = 0; p < cnt; p++) {
+        const Rec rec = recs[first + p];
+        leaf_hit |= isect(rec, r, s, first + p);
+        if (leaf_hit && any_hit) break;
+      }
+      if (leaf_hit) {
+        if (SPLIT) h32 = f32_up(s.h);
+        if (any_hit) {
+          done = true;
+          break;
+        }
+      }
+    }
+    // pop the next far child whose entry distance does not exceed the current hit distance
+    active = false;
+    while (sp > 0) {
+      sp--;
+      const bool ok = SPLIT ? !(h32 < st_key32[sp]) : !(s.h < st_key[sp]);
+      if (ok) {
+        cur = st_ref[sp];
+        cnt = st_cnt[sp];
+        active = true;
+        break;
+      }
+    }
+  }
+  (void)done;
+  return s.h < r.tmax;
+}
+
+template <class Rec, bool SPLIT>
+__global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysView rays, HitsView hits, uint64_t n, uint32_t flags,
+                                               uint32_t first, uint32_t is_tri, double S) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    Ray r;
+    make_ray(rays, i, r);
+    Ray32 q;
+    if (SPLIT) make_ray32(r, S, q);
+    Hit s;
+    s.h = r.tmax;
+    s.u = 0.0;
+    s.v = 0.0;
+    s.leaf_idx = INVALID_ID;
+    const bool hit = traverse<Rec, SPLIT>(bvh, r, q, (flags & FLAG_ANY_HIT) != 0, s);
+    // merge (blazert/scene.h:169-209): triangles replace unconditionally, later types only if strictly closer
+    bool write = false;
+    if (hit) {
+      if (is_tri) write = true;
+      else write = s.h < ((first && !(flags & FLAG_INOUT)) ? DBL_MAX : hits.t[i]);
+    }
+    if (write) {
+      const uint32_t pid = bvh.prim_ids[s.leaf_idx];
+      hits.t[i] = s.h;
+      if (hits.u) hits.u[i] = s.u;
+      if (hits.v) hits.v[i] = s.v;
+      hits.prim_id[i] = pid;
+      hits.geom_id[i] = bvh.geom_id;
+      if (hits.nx) {
+        double nrm[3];
+        post_normal(reinterpret_cast<const Rec *>(bvh.prims), bvh, r, s, pid, nrm);
+        hits.nx[i] = nrm[0];
+        hits.ny[i] = nrm[1];
+        hits.nz[i] = nrm[2];
+      }
+    } else if (first && !(flags & FLAG_INOUT)) {
+      hits.t[i] = DBL_MAX;
+      if (hits.u) hits.u[i] = 0.0;
+      if (hits.v) hits.v[i] = 0.0;
+      hits.prim_id[i] = INVALID_ID;
+      hits.geom_id[i] = INVALID_ID;
+      if (hits.nx) {
+        hits.nx[i] = 0.0;
+        hits.ny[i] = 0.0;
+        hits.nz[i] = 0.0;
+      }
+    }
+  }
+}
+
+__global__ void k_init_hits(HitsView hits, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    hits.t[i] = DBL_MAX;
+    if (hits.u) hits.u[i] = 0.0;
+    if (hits.v) hits.v[i] = 0.0;
+    hits.prim_id[i] = INVALID_ID;
+    hits.geom_id[i] = INVALID_ID;
+    if (hits.nx) {
+      hits.nx[i] = 0.0;
+      hits.ny[i] = 0.0;
+      hits.nz[i] = 0.0;
+    }
+  }
+}
+
+template <class Rec>
+hipError_t launch_typed(const BvhView &bvh, const RaysView &rays, const HitsView &hits, uint64_t n,
+                        const TraceLaunch &cfg, const SplitParams &sp, uint32_t is_tri, hipStream_t stream) {
+  const uint32_t block = cfg.block ? cfg.block : 256;
+  uint64_t blocks = (n + block - 1) / block;
+  if (blocks > (1ull << 30)) blocks = 1ull << 30;
+  if (blocks == 0) return hipSuccess;
+  if (cfg.precision == PREC_FP64)
+    hipLaunchKernelGGL((k_trace<Rec, false>), dim3((uint32_t)blocks), dim3(block), 0, stream, bvh, rays, hits, n,
+                       cfg.flags, cfg.first ? 1u : 0u, is_tri, sp.scene_extent);
+  else
+    hipLaunchKernelGGL((k_trace<Rec, true>), dim3((uint32_t)blocks), dim3(block), 0, stream, bvh, rays, hits, n,
+                       cfg.flags, cfg.first ? 1u : 0u, is_tri, sp.scene_extent);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_trace(GeomType type, const BvhView &bvh, const RaysView &rays, const HitsView &hits, uint64_t n,
+                        const TraceLaunch &cfg, const SplitParams &split, hipStream_t stream) {
+  switch (type) {
+  case TRIANGLES: return launch_typed<TriRec>(bvh, rays, hits, n, cfg, split, 1u, stream);
+  case SPHERES: return launch_typed<SphRec>(bvh, rays, hits, n, cfg, split, 0u, stream);
+  case PLANES: return launch_typed<PlaRec>(bvh, rays, hits, n, cfg, split, 0u, stream);
+  default: return launch_typed<CylRec>(bvh, rays, hits, n, cfg, split, 0u, stream);
+  }
+}
+
+hipError_t launch_init_hits(const HitsView &hits, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > (1ull << 30)) blocks = 1ull << 30;
+  hipLaunchKernelGGL(k_init_hits, dim3((uint32_t)blocks), dim3(256), 0, stream, hits, n);
+  return hipGetLastError();
+}
+
+}  // namespace brt

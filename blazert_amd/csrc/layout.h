@@ -1,0 +1,97 @@
+// Device-resident scene layout shared by the host builder (scene.cpp) and the HIP kernels.
+//
+// One BVH per geometry type, flattened from the reference topology (binary binned-SAH tree numbered in
+// pre-order, blazert/bvh/builder.h:84-115) into "children in parent" records: an internal node holds the
+// two child boxes, so one record fetch drives both slab tests of a traversal step. The root box is kept
+// in the descriptor. Leaf children are encoded in the child reference (LEAF_BIT | first primitive) with
+// the primitive count beside it; primitives are stored contiguously in leaf order.
+#pragma once
+#include <stdint.h>
+
+namespace brt {
+
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t AXIS_SHIFT = 30;           // cnt[1] high bits carry the parent's split axis
+constexpr uint32_t CNT_MASK = (1u << AXIS_SHIFT) - 1u;
+constexpr uint32_t INVALID_ID = 0xFFFFFFFFu;
+
+enum GeomType : uint32_t { TRIANGLES = 0, SPHERES = 1, PLANES = 2, CYLINDERS = 3, NUM_TYPES = 4 };
+
+// FP64 internal node: 128 B, one cache line. Child c box: b[6c+0..2] = min xyz, b[6c+3..5] = max xyz.
+struct alignas(128) Node64 {
+  double b[12];
+  uint32_t ref[2];  // child reference: internal node index, or LEAF_BIT | first primitive (leaf order)
+  uint32_t cnt[2];  // leaf primitive count (0 for an internal child); cnt[1] >> 30 = split axis
+  uint32_t pad[4];
+};
+static_assert(sizeof(Node64) == 128, "Node64 must be one 128-byte line");
+
+// FP32 internal node for split-precision traversal: child boxes rounded outward (min down, max up).
+struct alignas(64) Node32 {
+  float b[12];
+  uint32_t ref[2];
+  uint32_t cnt[2];
+};
+static_assert(sizeof(Node32) == 64, "Node32 must be 64 bytes");
+
+// Triangle record in leaf order: v0, e1 = v0 - v1, e2 = v2 - v0 (blazert/primitives/trimesh.h:26-27).
+struct alignas(16) TriRec {
+  double v0[3];
+  double e1[3];
+  double e2[3];
+  double pad;
+};
+static_assert(sizeof(TriRec) == 80, "TriRec size");
+
+struct alignas(32) SphRec {
+  double c[3];
+  double r;
+};
+static_assert(sizeof(SphRec) == 32, "SphRec size");
+
+// Plane: center, extents and the row-major rotation (blazert/primitives/plane.h:19-39).
+struct alignas(128) PlaRec {
+  double c[3];
+  double dx, dy;
+  double R[9];
+  double pad[2];
+};
+static_assert(sizeof(PlaRec) == 128, "PlaRec size");
+
+// Cylinder: center, semi axes a/b, height and the row-major rotation (blazert/primitives/cylinder.h:19-40).
+struct alignas(128) CylRec {
+  double c[3];
+  double a, b, h;
+  double R[9];
+  double pad;
+};
+static_assert(sizeof(CylRec) == 128, "CylRec size");
+
+// Kernel-side view of one uploaded geometry BVH (passed by value).
+struct BvhView {
+  const Node64 *nodes64;
+  const Node32 *nodes32;
+  const void *prims;          // TriRec / SphRec / PlaRec / CylRec in leaf order
+  const uint32_t *prim_ids;   // leaf order -> user primitive id
+  const uint32_t *faces;      // triangles only: user faces (3 per primitive id)
+  const double *vnormals;     // triangles only: vertex normals (3 per vertex)
+  double root_min[3], root_max[3];
+  float root_min32[3], root_max32[3];
+  uint32_t root_ref, root_cnt;
+  uint32_t geom_id;
+  uint32_t n_prims;
+  uint32_t n_nodes;
+  uint32_t depth;             // deepest level of the built tree
+};
+
+// Structure-of-arrays rays / hits in device memory (mirrors brt_rays / brt_hits of the C ABI).
+struct RaysView {
+  const double *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+};
+struct HitsView {
+  double *t, *u, *v;
+  uint32_t *prim_id, *geom_id;
+  double *nx, *ny, *nz;
+};
+
+}  // namespace brt

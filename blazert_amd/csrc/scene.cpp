@@ -1,0 +1,875 @@
+// Host side of the engine: scene construction, binned-SAH build, flattening, device upload and the C ABI.
+//
+// The builder reproduces the reference tree exactly (blazert/bvh/builder.h:46-226, binbuffer.h:15-167,
+// bbox.h:27-39): same binning rounding, same sweep, same cut position, same std::partition (libstdc++) and
+// the same bad-split fallback, numbered in pre-order. Identical topology is what makes the GPU's hit ids and
+// tie order match the reference bit for bit. Independent subtrees are built on separate threads; each thread
+// numbers its subtree locally and the results are spliced in pre-order, so the output does not depend on the
+// thread count.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/blazert_amd.h"
+#include "engine.h"
+#include "layout.h"
+
+namespace brt {
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+// ------------------------------------------------------------------------------------------------ vec helpers
+// blaze conventions: dot = (a0 b0 + a1 b1) + a2 b2, normalize = v * (len != 0 ? 1/len : 0),
+// vec / s = vec * (1/s), matrix-vector rows summed left to right.
+struct V3 {
+  double x[3];
+  double &operator[](int i) { return x[i]; }
+  double operator[](int i) const { return x[i]; }
+};
+inline V3 sub(const V3 &a, const V3 &b) { return {{a[0] - b[0], a[1] - b[1], a[2] - b[2]}}; }
+inline double dot(const V3 &a, const V3 &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline V3 cross(const V3 &a, const V3 &b) {
+  return {{a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}};
+}
+inline V3 scale_by_inverse(const V3 &a, double s) {
+  const double is = 1.0 / s;
+  return {{a[0] * is, a[1] * is, a[2] * is}};
+}
+inline V3 normalized(const V3 &a) {
+  const double len = std::sqrt(dot(a, a));
+  const double il = (len != 0.0) ? 1.0 / len : 0.0;
+  return {{a[0] * il, a[1] * il, a[2] * il}};
+}
+inline V3 rot_apply(const double *R, const V3 &v) {
+  return {{R[0] * v[0] + R[1] * v[1] + R[2] * v[2], R[3] * v[0] + R[4] * v[1] + R[5] * v[2],
+           R[6] * v[0] + R[7] * v[1] + R[8] * v[2]}};
+}
+inline double ref_min(double a, double b) { return (b < a) ? b : a; }  // std::min(a, b)
+inline double ref_max(double a, double b) { return (a < b) ? b : a; }  // std::max(a, b)
+
+// ------------------------------------------------------------------------------------------------ collections
+// Per-primitive bounding boxes and centers in structure-of-arrays form (the collection precompute of
+// trimesh.h:78-130, sphere.h:301-329, plane.h:89-150, cylinder.h:91-147).
+struct Boxes {
+  std::vector<double> lo, hi, ctr;  // 3 per primitive
+  uint32_t n = 0;
+  void resize(uint32_t m) {
+    n = m;
+    lo.assign(3ull * m, 0.0);
+    hi.assign(3ull * m, 0.0);
+    ctr.assign(3ull * m, 0.0);
+  }
+};
+
+struct Geometry {
+  bool present = false;
+  uint32_t geom_id = INVALID_ID;
+  Boxes boxes;
+  // inputs (copied)
+  std::vector<double> vertices;  // triangles
+  std::vector<uint32_t> faces;
+  std::vector<double> vnormals;
+  std::vector<double> centers, radii, dxs, dys, rot, sa, sb, heights;
+  // build output
+  std::vector<brt_tree_node> tree;  // reference form, pre-order
+  std::vector<uint32_t> order;      // primitive ids in leaf order
+  brt_build_stats stats{};
+  // flattened
+  std::vector<Node64> n64;
+  std::vector<Node32> n32;
+  std::vector<unsigned char> recs;
+  size_t rec_size = 0;
+  BvhView view{};
+  double extent = 0.0;  // max finite |coordinate| of all boxes (split-precision error bound)
+  bool extent_ok = true;
+};
+
+void mesh_precompute(Geometry &g, uint32_t nv) {
+  const uint32_t nf = (uint32_t)(g.faces.size() / 3);
+  g.boxes.resize(nf);
+  g.vnormals.assign(3ull * nv, 0.0);
+  auto vert = [&](uint32_t i) { return V3{{g.vertices[3ull * i], g.vertices[3ull * i + 1], g.vertices[3ull * i + 2]}}; };
+  for (uint32_t f = 0; f < nf; f++) {
+    const uint32_t i0 = g.faces[3ull * f], i1 = g.faces[3ull * f + 1], i2 = g.faces[3ull * f + 2];
+    const V3 v0 = vert(i0), v1 = vert(i1), v2 = vert(i2);
+    const V3 sum{{(v0[0] + v1[0]) + v2[0], (v0[1] + v1[1]) + v2[1], (v0[2] + v1[2]) + v2[2]}};
+    const V3 c = scale_by_inverse(sum, 3.0);
+    V3 lo = v0, hi = v0;
+    for (const V3 *p : {&v1, &v2})
+      for (int k = 0; k < 3; k++) {
+        lo[k] = ref_min((*p)[k], lo[k]);
+        hi[k] = ref_max((*p)[k], hi[k]);
+      }
+    for (int k = 0; k < 3; k++) {
+      g.boxes.lo[3ull * f + k] = lo[k];
+      g.boxes.hi[3ull * f + k] = hi[k];
+      g.boxes.ctr[3ull * f + k] = c[k];
+    }
+    // face normal normalize(cross(v1 - v0, v2 - v0)); each vertex accumulates fn / 3 in face order
+    const V3 fn3 = scale_by_inverse(normalized(cross(sub(v1, v0), sub(v2, v0))), 3.0);
+    for (uint32_t vi : {i0, i1, i2})
+      for (int k = 0; k < 3; k++) g.vnormals[3ull * vi + k] = g.vnormals[3ull * vi + k] + fn3[k];
+  }
+}
+
+void sphere_precompute(Geometry &g) {
+  const uint32_t n = (uint32_t)g.radii.size();
+  g.boxes.resize(n);
+  for (uint32_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) {
+      const double c = g.centers[3ull * i + k];
+      g.boxes.lo[3ull * i + k] = c - g.radii[i];
+      g.boxes.hi[3ull * i + k] = c + g.radii[i];
+      g.boxes.ctr[3ull * i + k] = c;
+    }
+}
+
+// Box of six points: first element wins ties as in std::min / std::max over an initializer list.
+void six_point_box(const V3 *pts, double *lo, double *hi) {
+  for (int k = 0; k < 3; k++) {
+    double mn = pts[0][k], mx = pts[0][k];
+    for (int p = 1; p < 6; p++) {
+      if (pts[p][k] < mn) mn = pts[p][k];
+      if (mx < pts[p][k]) mx = pts[p][k];
+    }
+    lo[k] = mn;
+    hi[k] = mx;
+  }
+}
+
+void plane_precompute(Geometry &g) {
+  const uint32_t n = (uint32_t)g.dxs.size();
+  g.boxes.resize(n);
+  const double th = DBL_MIN;  // std::numeric_limits<double>::min(), plane.h:29
+  for (uint32_t i = 0; i < n; i++) {
+    const double *R = &g.rot[9ull * i];
+    const V3 c{{g.centers[3ull * i], g.centers[3ull * i + 1], g.centers[3ull * i + 2]}};
+    const double dx = g.dxs[i], dy = g.dys[i];
+    const V3 loc[6] = {{{-dx / 2.0, -dy / 2.0, 0.0}}, {{dx / 2.0, -dy / 2.0, 0.0}}, {{-dx / 2.0, dy / 2.0, 0.0}},
+                       {{dx / 2.0, dy / 2.0, 0.0}},   {{0.0, 0.0, -th}},           {{0.0, 0.0, th}}};
+    V3 pts[6];
+    for (int p = 0; p < 6; p++) {
+      const V3 r = rot_apply(R, loc[p]);
+      pts[p] = {{c[0] + r[0], c[1] + r[1], c[2] + r[2]}};
+    }
+    six_point_box(pts, &g.boxes.lo[3ull * i], &g.boxes.hi[3ull * i]);
+    for (int k = 0; k < 3; k++) g.boxes.ctr[3ull * i + k] = c[k];
+  }
+}
+
+void cylinder_precompute(Geometry &g) {
+  const uint32_t n = (uint32_t)g.sa.size();
+  g.boxes.resize(n);
+  for (uint32_t i = 0; i < n; i++) {
+    const double *R = &g.rot[9ull * i];
+    const V3 c{{g.centers[3ull * i], g.centers[3ull * i + 1], g.centers[3ull * i + 2]}};
+    const V3 ra = rot_apply(R, V3{{g.sa[i], 0.0, 0.0}});
+    const V3 rb = rot_apply(R, V3{{0.0, g.sb[i], 0.0}});
+    const V3 rh = rot_apply(R, V3{{0.0, 0.0, g.heights[i] / 2.0}});
+    V3 pts[6];
+    for (int k = 0; k < 3; k++) {
+      pts[0][k] = c[k] + ra[k];  // a1
+      pts[1][k] = c[k] + rb[k];  // b1
+      pts[2][k] = c[k] - ra[k];  // a2
+      pts[3][k] = c[k] - rb[k];  // b2
+      pts[4][k] = c[k] + rh[k];  // h1
+      pts[5][k] = c[k] - rh[k];  // h2
+    }
+    six_point_box(pts, &g.boxes.lo[3ull * i], &g.boxes.hi[3ull * i]);
+    for (int k = 0; k < 3; k++) g.boxes.ctr[3ull * i + k] = c[k];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ builder
+struct BuildCounters {
+  std::atomic<uint32_t> max_depth{0}, leaves{0}, branches{0}, bad_splits{0}, max_prims{0};
+  std::atomic<uint32_t> min_prims{0xFFFFFFFFu};
+  void atomic_max(std::atomic<uint32_t> &a, uint32_t v) {
+    uint32_t cur = a.load();
+    while (cur < v && !a.compare_exchange_weak(cur, v)) {
+    }
+  }
+  void atomic_min(std::atomic<uint32_t> &a, uint32_t v) {
+    uint32_t cur = a.load();
+    while (v < cur && !a.compare_exchange_weak(cur, v)) {
+    }
+  }
+};
+
+struct Builder {
+  const Boxes &B;
+  uint32_t *idx;
+  uint32_t min_leaf, max_depth, nbins;
+  BuildCounters cnt;
+
+  struct BinAcc {
+    double lo[3], hi[3];
+    uint32_t count;
+    double cost;
+  };
+
+  void bounds(uint32_t begin, uint32_t end, double *lo, double *hi) const {
+    for (int k = 0; k < 3; k++) {
+      lo[k] = DBL_MAX;
+      hi[k] = -DBL_MAX;
+    }
+    for (uint32_t i = begin; i < end; i++) {
+      const uint32_t p = idx[i];
+      for (int k = 0; k < 3; k++) {
+        lo[k] = ref_min(B.lo[3ull * p + k], lo[k]);
+        hi[k] = ref_max(B.hi[3ull * p + k], hi[k]);
+      }
+    }
+  }
+
+  static double surface(const double *lo, const double *hi) {
+    const double b0 = std::fabs(hi[0] - lo[0]), b1 = std::fabs(hi[1] - lo[1]), b2 = std::fabs(hi[2] - lo[2]);
+    return 2.0 * (b0 * b1 + b1 * b2 + b2 * b0);
+  }
+
+  // unsigned(std::round(x)) as the reference's x86-64 build evaluates it (64-bit truncation, low 32 bits)
+  static uint32_t bin_of(double x, uint32_t last) {
+    const double r = std::round(x);
+    uint32_t u = 0;
+    if (r == r && r < 9.2e18 && r > -9.2e18) u = (uint32_t)(uint64_t)(int64_t)r;
+    return u < last ? u : last;
+  }
+
+  // find_best_split_binned: returns the axis and writes the three cut positions
+  uint32_t best_split(uint32_t begin, uint32_t end, const double *lo, const double *hi, double *cut,
+                      std::vector<BinAcc> &bins) const {
+    bins.assign(3ull * nbins, BinAcc{{DBL_MAX, DBL_MAX, DBL_MAX}, {-DBL_MAX, -DBL_MAX, -DBL_MAX}, 0u, 0.0});
+    double inv[3];
+    for (int k = 0; k < 3; k++) {
+      const double size = hi[k] - lo[k];
+      inv[k] = (size > 0.0) ? 1.0 / size : 0.0;
+    }
+    const double scale = (double)(nbins - 1);
+    for (uint32_t i = begin; i < end; i++) {
+      const uint32_t p = idx[i];
+      for (uint32_t j = 0; j < 3; j++) {
+        const uint32_t b = bin_of(((B.ctr[3ull * p + j] - lo[j]) * inv[j]) * scale, nbins - 1);
+        BinAcc &bin = bins[j * nbins + b];
+        bin.count++;
+        for (int k = 0; k < 3; k++) {
+          bin.lo[k] = ref_min(B.lo[3ull * p + k], bin.lo[k]);
+          bin.hi[k] = ref_max(B.hi[3ull * p + k], bin.hi[k]);
+        }
+      }
+    }
+    double min_cost[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    for (uint32_t j = 0; j < 3; j++) {
+      BinAcc *ax = &bins[j * nbins];
+      double alo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, ahi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+      uint64_t count = 0;
+      for (uint32_t i = nbins - 1; i > 0; i--) {  // right-to-left sweep: cost of the right side
+        for (int k = 0; k < 3; k++) {
+          alo[k] = ref_min(ax[i].lo[k], alo[k]);
+          ahi[k] = ref_max(ax[i].hi[k], ahi[k]);
+        }
+        count += ax[i].count;
+        ax[i].cost = (double)count * surface(alo, ahi);
+      }
+      for (int k = 0; k < 3; k++) {
+        alo[k] = DBL_MAX;
+        ahi[k] = -DBL_MAX;
+      }
+      count = 0;
+      uint32_t min_bin = 1;
+      for (uint32_t i = 0; i + 1 < nbins; i++) {  // left-to-right sweep: full cost, first minimum wins
+        for (int k = 0; k < 3; k++) {
+          alo[k] = ref_min(ax[i].lo[k], alo[k]);
+          ahi[k] = ref_max(ax[i].hi[k], ahi[k]);
+        }
+        count += ax[i].count;
+        const double cost = (double)count * surface(alo, ahi) + ax[i + 1].cost;
+        if (cost < min_cost[j]) {
+          min_cost[j] = cost;
+          min_bin = i + 1;
+        }
+      }
+      cut[j] = (double)min_bin * ((hi[j] - lo[j]) / (double)nbins) + lo[j];
+    }
+    uint32_t axis = 0;
+    if (min_cost[0] > min_cost[1]) axis = 1;
+    if (min_cost[axis] > min_cost[2]) axis = 2;
+    return axis;
+  }
+
+  // Builds [begin, end) at `depth` into `out` (pre-order, indices local to `out` starting at out.size()).
+  void build(std::vector<brt_tree_node> &out, uint32_t begin, uint32_t end, uint32_t depth, int par_levels) {
+    std::vector<BinAcc> bins;
+    build_rec(out, begin, end, depth, par_levels, bins);
+  }
+
+  uint32_t build_rec(std::vector<brt_tree_node> &out, uint32_t begin, uint32_t end, uint32_t depth, int par_levels,
+                     std::vector<BinAcc> &bins) {
+    const uint32_t self = (uint32_t)out.size();
+    out.emplace_back();
+    cnt.atomic_max(cnt.max_depth, depth);
+    const uint32_t n = end - begin;
+    double lo[3], hi[3];
+    bounds(begin, end, lo, hi);
+    brt_tree_node node{};
+    std::memcpy(node.min, lo, sizeof lo);
+    std::memcpy(node.max, hi, sizeof hi);
+    node.children[0] = node.children[1] = INVALID_ID;
+    if (n <= min_leaf || depth >= max_depth) {
+      node.leaf = 1;
+      node.axis = INVALID_ID;
+      node.prim_begin = begin;
+      node.prim_count = n;
+      out[self] = node;
+      cnt.leaves++;
+      cnt.atomic_max(cnt.max_prims, n);
+      cnt.atomic_min(cnt.min_prims, n);
+      return self;
+    }
+    double cut[3];
+    uint32_t axis = best_split(begin, end, lo, hi, cut, bins);
+    uint32_t mid = begin;
+    for (int attempt = 0; attempt < 3; attempt++) {
+      const double c = cut[axis];
+      const double *ctr = B.ctr.data();
+      const uint32_t ax = axis;
+      uint32_t *m = std::partition(idx + begin, idx + end,
+                                   [ctr, ax, c](uint32_t p) { return ctr[3ull * p + ax] < c; });
+      mid = (uint32_t)(m - idx);
+      if (mid == begin || mid == end) {
+        cnt.bad_splits++;
+        mid = begin + (n >> 1);
+        axis = (axis + 1) % 3;
+      } else {
+        break;
+      }
+    }
+    node.leaf = 0;
+    node.axis = axis;
+    out[self] = node;
+    cnt.branches++;
+    if (par_levels > 0 && n >= 32768) {
+      // left subtree on another thread into its own vector, right subtree here; splice in pre-order
+      std::vector<brt_tree_node> left;
+      left.reserve(2ull * (mid - begin) / 3 + 16);
+      std::thread th([&] { build(left, begin, mid, depth + 1, par_levels - 1); });
+      std::vector<brt_tree_node> right;
+      right.reserve(2ull * (end - mid) / 3 + 16);
+      build(right, mid, end, depth + 1, par_levels - 1);
+      th.join();
+      const uint32_t lbase = (uint32_t)out.size();
+      const uint32_t rbase = lbase + (uint32_t)left.size();
+      for (auto &x : left)
+        for (auto &ch : x.children)
+          if (ch != INVALID_ID) ch += lbase;
+      for (auto &x : right)
+        for (auto &ch : x.children)
+          if (ch != INVALID_ID) ch += rbase;
+      out.insert(out.end(), left.begin(), left.end());
+      out.insert(out.end(), right.begin(), right.end());
+      out[self].children[0] = lbase;
+      out[self].children[1] = rbase;
+    } else {
+      const uint32_t l = build_rec(out, begin, mid, depth + 1, 0, bins);
+      out[self].children[0] = l;
+      const uint32_t r = build_rec(out, mid, end, depth + 1, 0, bins);
+      out[self].children[1] = r;
+    }
+    return self;
+  }
+};
+
+float f32_down(double x) {
+  float f = (float)x;
+  if ((double)f > x) f = std::nextafter(f, -INFINITY);
+  return f;
+}
+float f32_up(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = std::nextafter(f, INFINITY);
+  return f;
+}
+
+// Flattens the reference-form tree into children-in-parent records and leaf-order primitive records.
+void flatten(Geometry &g, GeomType type) {
+  const auto &T = g.tree;
+  std::vector<uint32_t> internal(T.size(), INVALID_ID);
+  uint32_t ni = 0;
+  for (size_t i = 0; i < T.size(); i++)
+    if (!T[i].leaf) internal[i] = ni++;
+  g.n64.assign(ni, Node64{});
+  g.n32.assign(ni, Node32{});
+  double extent = 0.0;
+  bool ok = true;
+  auto note = [&](double v) {
+    if (!std::isfinite(v)) ok = false;
+    else extent = std::max(extent, std::fabs(v));
+  };
+  auto child_ref = [&](uint32_t c, uint32_t &ref, uint32_t &cnt) {
+    if (T[c].leaf) {
+      ref = LEAF_BIT | T[c].prim_begin;
+      cnt = T[c].prim_count;
+    } else {
+      ref = internal[c];
+      cnt = 0;
+    }
+  };
+  for (size_t i = 0; i < T.size(); i++) {
+    if (T[i].leaf) continue;
+    Node64 &a = g.n64[internal[i]];
+    Node32 &b = g.n32[internal[i]];
+    for (int c = 0; c < 2; c++) {
+      const brt_tree_node &ch = T[T[i].children[c]];
+      for (int k = 0; k < 3; k++) {
+        a.b[6 * c + k] = ch.min[k];
+        a.b[6 * c + 3 + k] = ch.max[k];
+        b.b[6 * c + k] = f32_down(ch.min[k]);
+        b.b[6 * c + 3 + k] = f32_up(ch.max[k]);
+        note(ch.min[k]);
+        note(ch.max[k]);
+      }
+      child_ref(T[i].children[c], a.ref[c], a.cnt[c]);
+      b.ref[c] = a.ref[c];
+      b.cnt[c] = a.cnt[c];
+    }
+    a.cnt[1] |= T[i].axis << AXIS_SHIFT;
+    b.cnt[1] |= T[i].axis << AXIS_SHIFT;
+  }
+  BvhView &v = g.view;
+  v = BvhView{};
+  for (int k = 0; k < 3; k++) {
+    v.root_min[k] = T[0].min[k];
+    v.root_max[k] = T[0].max[k];
+    v.root_min32[k] = f32_down(T[0].min[k]);
+    v.root_max32[k] = f32_up(T[0].max[k]);
+    note(T[0].min[k]);
+    note(T[0].max[k]);
+  }
+  child_ref(0, v.root_ref, v.root_cnt);
+  v.geom_id = g.geom_id;
+  v.n_prims = g.boxes.n;
+  v.n_nodes = ni;
+  v.depth = g.stats.max_tree_depth;
+  g.extent = extent;
+  g.extent_ok = ok && extent < 1e30;
+
+  const uint32_t n = g.boxes.n;
+  switch (type) {
+  case TRIANGLES: {
+    g.rec_size = sizeof(TriRec);
+    g.recs.assign((size_t)n * sizeof(TriRec), 0);
+    TriRec *r = reinterpret_cast<TriRec *>(g.recs.data());
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t p = g.order[i];
+      const double *v0 = &g.vertices[3ull * g.faces[3ull * p]];
+      const double *v1 = &g.vertices[3ull * g.faces[3ull * p + 1]];
+      const double *v2 = &g.vertices[3ull * g.faces[3ull * p + 2]];
+      for (int k = 0; k < 3; k++) {
+        r[i].v0[k] = v0[k];
+        r[i].e1[k] = v0[k] - v1[k];  // Triangle::c = a - b
+        r[i].e2[k] = v2[k] - v0[k];  // Triangle::b = c - a
+      }
+    }
+    break;
+  }
+  case SPHERES: {
+    g.rec_size = sizeof(SphRec);
+    g.recs.assign((size_t)n * sizeof(SphRec), 0);
+    SphRec *r = reinterpret_cast<SphRec *>(g.recs.data());
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t p = g.order[i];
+      for (int k = 0; k < 3; k++) r[i].c[k] = g.centers[3ull * p + k];
+      r[i].r = g.radii[p];
+    }
+    break;
+  }
+  case PLANES: {
+    g.rec_size = sizeof(PlaRec);
+    g.recs.assign((size_t)n * sizeof(PlaRec), 0);
+    PlaRec *r = reinterpret_cast<PlaRec *>(g.recs.data());
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t p = g.order[i];
+      for (int k = 0; k < 3; k++) r[i].c[k] = g.centers[3ull * p + k];
+      r[i].dx = g.dxs[p];
+      r[i].dy = g.dys[p];
+      for (int k = 0; k < 9; k++) r[i].R[k] = g.rot[9ull * p + k];
+    }
+    break;
+  }
+  default: {
+    g.rec_size = sizeof(CylRec);
+    g.recs.assign((size_t)n * sizeof(CylRec), 0);
+    CylRec *r = reinterpret_cast<CylRec *>(g.recs.data());
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t p = g.order[i];
+      for (int k = 0; k < 3; k++) r[i].c[k] = g.centers[3ull * p + k];
+      r[i].a = g.sa[p];
+      r[i].b = g.sb[p];
+      r[i].h = g.heights[p];
+      for (int k = 0; k < 9; k++) r[i].R[k] = g.rot[9ull * p + k];
+    }
+    break;
+  }
+  }
+}
+
+unsigned hw_threads() {
+  const unsigned h = std::thread::hardware_concurrency();
+  return h ? std::min(h, 64u) : 4u;
+}
+
+void build_geometry(Geometry &g, GeomType type, const brt_build_options &o) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t n = g.boxes.n;
+  g.order.resize(n);
+  std::iota(g.order.begin(), g.order.end(), 0u);
+  Builder b{g.boxes, g.order.data(), o.min_leaf_primitives, o.max_tree_depth, o.bin_size};
+  int par = 0;
+  for (unsigned t = hw_threads(); t > 1; t >>= 1) par++;
+  g.tree.clear();
+  g.tree.reserve(n ? 2ull * n : 1);
+  b.build(g.tree, 0, n, 0, par);
+  g.tree.shrink_to_fit();
+  brt_build_stats &s = g.stats;
+  s.max_tree_depth = b.cnt.max_depth;
+  s.leaf_nodes = b.cnt.leaves;
+  s.branch_nodes = b.cnt.branches;
+  s.bad_splits = b.cnt.bad_splits;
+  s.max_num_prims_per_leaf = b.cnt.max_prims;
+  s.min_num_prims_per_leaf = b.cnt.min_prims;
+  s.num_primitives = n;
+  s.geom_id = g.geom_id;
+  flatten(g, type);
+  s.build_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  s.device_bytes = g.n64.size() * sizeof(Node64) + g.n32.size() * sizeof(Node32) + g.recs.size() +
+                   g.order.size() * 4 + g.faces.size() * 4 + g.vnormals.size() * 8;
+}
+
+// ------------------------------------------------------------------------------------------------ device
+struct DeviceScene {
+  int device = -1;
+  std::vector<void *> allocs;
+  BvhView views[NUM_TYPES]{};
+  ~DeviceScene() {
+    for (void *p : allocs) (void)hipFree(p);
+  }
+  template <class T>
+  hipError_t put(const std::vector<T> &v, const T **dst) {
+    *dst = nullptr;
+    if (v.empty()) return hipSuccess;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, v.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    allocs.push_back(p);
+    e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    *dst = static_cast<const T *>(p);
+    return e;
+  }
+};
+
+}  // namespace
+}  // namespace brt
+
+using namespace brt;
+
+struct brt_scene {
+  brt_build_options opts{4, 28, 64};
+  bool committed = false;
+  uint32_t geometries = 0;
+  Geometry geo[NUM_TYPES];
+  std::unique_ptr<DeviceScene> dev;
+  std::mutex upload_mu;
+};
+
+namespace {
+
+int add_common(brt_scene *s, uint32_t type, uint32_t *geom_id) {
+  if (geom_id) *geom_id = BRT_INVALID_ID;
+  if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (s->committed) return fail(BRT_ERR_STATE, "geometry added after commit");
+  if (s->geo[type].present) return fail(BRT_ERR_STATE, "geometry type already added (one add per type)");
+  return BRT_OK;
+}
+
+void finish_add(brt_scene *s, uint32_t type, uint32_t *geom_id) {
+  Geometry &g = s->geo[type];
+  g.present = true;
+  g.geom_id = s->geometries++;
+  if (geom_id) *geom_id = g.geom_id;
+}
+
+int upload(brt_scene *s) {
+  std::lock_guard<std::mutex> lk(s->upload_mu);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(BRT_ERR_NO_DEVICE, "no HIP device");
+  if (s->dev) {
+    if (s->dev->device != dev) return fail(BRT_ERR_STATE, "scene was uploaded to another device");
+    return BRT_OK;
+  }
+  auto d = std::make_unique<DeviceScene>();
+  d->device = dev;
+  for (uint32_t t = 0; t < NUM_TYPES; t++) {
+    Geometry &g = s->geo[t];
+    if (!g.present || g.boxes.n == 0) continue;
+    if (g.stats.max_tree_depth >= 64) return fail(BRT_ERR_UNSUPPORTED, "tree deeper than the 64-entry device stack");
+    BvhView v = g.view;
+    hipError_t e = d->put(g.n64, &v.nodes64);
+    if (e == hipSuccess) e = d->put(g.n32, &v.nodes32);
+    const unsigned char *recs = nullptr;
+    if (e == hipSuccess) e = d->put(g.recs, &recs);
+    v.prims = recs;
+    if (e == hipSuccess) e = d->put(g.order, &v.prim_ids);
+    if (t == TRIANGLES) {
+      if (e == hipSuccess) e = d->put(g.faces, &v.faces);
+      if (e == hipSuccess) e = d->put(g.vnormals, &v.vnormals);
+    }
+    if (e != hipSuccess)
+      return fail(e == hipErrorOutOfMemory ? BRT_ERR_OUT_OF_MEMORY : BRT_ERR_DEVICE,
+                  std::string("upload failed: ") + hipGetErrorString(e));
+    d->views[t] = v;
+  }
+  s->dev = std::move(d);
+  return BRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *brt_last_error(void) { return g_error.c_str(); }
+const char *brt_version(void) { return "blazert_amd 0.1 (gfx950)"; }
+
+int brt_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int brt_scene_create(const brt_build_options *opts, brt_scene **out) {
+  if (!out) return fail(BRT_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  brt_scene *s = new (std::nothrow) brt_scene();
+  if (!s) return fail(BRT_ERR_OUT_OF_MEMORY, "scene allocation failed");
+  if (opts) {
+    if (opts->bin_size < 2 || opts->min_leaf_primitives == 0) {
+      delete s;
+      return fail(BRT_ERR_INVALID_ARGUMENT, "bin_size must be >= 2 and min_leaf_primitives >= 1");
+    }
+    s->opts = *opts;
+  }
+  *out = s;
+  return BRT_OK;
+}
+
+int brt_scene_destroy(brt_scene *scene) {
+  delete scene;
+  return BRT_OK;
+}
+
+int brt_scene_add_mesh(brt_scene *s, const double *vertices, uint32_t nv, const uint32_t *faces, uint32_t nf,
+                       uint32_t *geom_id) {
+  int rc = add_common(s, TRIANGLES, geom_id);
+  if (rc) return rc;
+  if ((nv && !vertices) || (nf && !faces)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL vertex or face array");
+  for (uint64_t i = 0; i < 3ull * nf; i++)
+    if (faces[i] >= nv) return fail(BRT_ERR_INVALID_ARGUMENT, "face index out of range");
+  if (nf >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many triangles");
+  Geometry &g = s->geo[TRIANGLES];
+  g.vertices.assign(vertices, vertices + 3ull * nv);
+  g.faces.assign(faces, faces + 3ull * nf);
+  mesh_precompute(g, nv);
+  finish_add(s, TRIANGLES, geom_id);
+  return BRT_OK;
+}
+
+int brt_scene_add_spheres(brt_scene *s, const double *centers, const double *radii, uint32_t n, uint32_t *geom_id) {
+  int rc = add_common(s, SPHERES, geom_id);
+  if (rc) return rc;
+  if (n && (!centers || !radii)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL sphere array");
+  if (n >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many spheres");
+  Geometry &g = s->geo[SPHERES];
+  g.centers.assign(centers, centers + 3ull * n);
+  g.radii.assign(radii, radii + n);
+  sphere_precompute(g);
+  finish_add(s, SPHERES, geom_id);
+  return BRT_OK;
+}
+
+int brt_scene_add_planes(brt_scene *s, const double *centers, const double *dxs, const double *dys,
+                         const double *rotations, uint32_t n, uint32_t *geom_id) {
+  int rc = add_common(s, PLANES, geom_id);
+  if (rc) return rc;
+  if (n && (!centers || !dxs || !dys || !rotations)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL plane array");
+  if (n >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many planes");
+  Geometry &g = s->geo[PLANES];
+  g.centers.assign(centers, centers + 3ull * n);
+  g.dxs.assign(dxs, dxs + n);
+  g.dys.assign(dys, dys + n);
+  g.rot.assign(rotations, rotations + 9ull * n);
+  plane_precompute(g);
+  finish_add(s, PLANES, geom_id);
+  return BRT_OK;
+}
+
+int brt_scene_add_cylinders(brt_scene *s, const double *centers, const double *a, const double *b,
+                            const double *heights, const double *rotations, uint32_t n, uint32_t *geom_id) {
+  int rc = add_common(s, CYLINDERS, geom_id);
+  if (rc) return rc;
+  if (n && (!centers || !a || !b || !heights || !rotations))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "NULL cylinder array");
+  if (n >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many cylinders");
+  Geometry &g = s->geo[CYLINDERS];
+  g.centers.assign(centers, centers + 3ull * n);
+  g.sa.assign(a, a + n);
+  g.sb.assign(b, b + n);
+  g.heights.assign(heights, heights + n);
+  g.rot.assign(rotations, rotations + 9ull * n);
+  cylinder_precompute(g);
+  finish_add(s, CYLINDERS, geom_id);
+  return BRT_OK;
+}
+
+int brt_scene_commit(brt_scene *s) {
+  if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (s->committed) return BRT_OK;
+  for (uint32_t t = 0; t < NUM_TYPES; t++)
+    if (s->geo[t].present) build_geometry(s->geo[t], (GeomType)t, s->opts);
+  s->committed = true;
+  return BRT_OK;
+}
+
+int brt_scene_upload(brt_scene *s) {
+  if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  return upload(s);
+}
+
+int brt_scene_build_stats(const brt_scene *s, uint32_t type, brt_build_stats *stats) {
+  if (!s || !stats || type >= NUM_TYPES) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  if (!s->geo[type].present) return fail(BRT_ERR_INVALID_ARGUMENT, "geometry type not in scene");
+  *stats = s->geo[type].stats;
+  return BRT_OK;
+}
+
+int brt_scene_export_tree(const brt_scene *s, uint32_t type, brt_tree_node *nodes, uint32_t capacity,
+                          uint32_t *prim_order, uint32_t *num_nodes) {
+  if (!s || type >= NUM_TYPES || !num_nodes) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  const Geometry &g = s->geo[type];
+  if (!g.present) return fail(BRT_ERR_INVALID_ARGUMENT, "geometry type not in scene");
+  *num_nodes = (uint32_t)g.tree.size();
+  if (nodes) {
+    if (capacity < g.tree.size()) return fail(BRT_ERR_INVALID_ARGUMENT, "capacity too small");
+    std::memcpy(nodes, g.tree.data(), g.tree.size() * sizeof(brt_tree_node));
+  }
+  if (prim_order && !g.order.empty()) std::memcpy(prim_order, g.order.data(), g.order.size() * 4);
+  return BRT_OK;
+}
+
+int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hits, uint64_t n,
+                  const brt_launch *launch, void *stream) {
+  brt_scene *s = const_cast<brt_scene *>(cs);
+  if (!s || !rays || !hits) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  if (n == 0) return BRT_OK;
+  if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "ray origin / direction arrays are required");
+  if (!hits->t || !hits->prim_id || !hits->geom_id)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "hit t / prim_id / geom_id arrays are required");
+  if ((hits->nx != nullptr) != (hits->ny != nullptr) || (hits->nx != nullptr) != (hits->nz != nullptr))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "normal arrays must be all set or all NULL");
+  brt_launch L{BRT_PRECISION_FP64, 0, 0, 0};
+  if (launch) L = *launch;
+  if (L.precision > BRT_PRECISION_FP32_TRAV_FP64_HIT) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (L.block_size && (L.block_size % 64 || L.block_size > 1024))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 up to 1024");
+  int rc = upload(s);
+  if (rc) return rc;
+  RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+  HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  bool first = true;
+  for (uint32_t t = 0; t < NUM_TYPES; t++) {
+    const Geometry &g = s->geo[t];
+    if (!g.present || g.boxes.n == 0) continue;
+    if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT && !g.extent_ok)
+      return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
+    TraceLaunch cfg{L.precision, L.flags, L.block_size, first};
+    SplitParams sp{g.extent};
+    hipError_t e = launch_trace((GeomType)t, s->dev->views[t], rv, hv, n, cfg, sp, st);
+    if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+    first = false;
+  }
+  if (first && !(L.flags & BRT_FLAG_HITS_INOUT)) {
+    hipError_t e = launch_init_hits(hv, n, st);
+    if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+  }
+  return BRT_OK;
+}
+
+int brt_intersect1(const brt_scene *s, const double org[3], const double dir[3], double tmin, double tmax,
+                   uint32_t flags, brt_hit1 *hit, int *hit_out) {
+  if (!s || !org || !dir || !hit) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  // one ray through the batched path, keeping RayHit's in/out semantics
+  double host[8 + 8] = {org[0], org[1], org[2], dir[0], dir[1], dir[2], tmin, tmax};
+  double *d = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&d), 16 * sizeof(double) + 2 * sizeof(uint32_t));
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+  uint32_t *ids = reinterpret_cast<uint32_t *>(d + 16);
+  host[8] = hit->t;
+  host[9] = hit->u;
+  host[10] = hit->v;
+  host[11] = hit->normal[0];
+  host[12] = hit->normal[1];
+  host[13] = hit->normal[2];
+  const uint32_t hid[2] = {hit->prim_id, hit->geom_id};
+  e = hipMemcpy(d, host, sizeof host, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ids, hid, sizeof hid, hipMemcpyHostToDevice);
+  int rc = BRT_OK;
+  if (e == hipSuccess) {
+    brt_rays r{d, d + 1, d + 2, d + 3, d + 4, d + 5, d + 6, d + 7};
+    brt_hits h{d + 8, d + 9, d + 10, ids, ids + 1, d + 11, d + 12, d + 13};
+    brt_launch L{BRT_PRECISION_FP64, flags | BRT_FLAG_HITS_INOUT, 64, 0};
+    rc = brt_intersect(s, &r, &h, 1, &L, nullptr);
+    if (rc == BRT_OK) e = hipDeviceSynchronize();
+    if (rc == BRT_OK && e == hipSuccess) e = hipMemcpy(host, d, sizeof host, hipMemcpyDeviceToHost);
+    uint32_t oid[2] = {0, 0};
+    if (rc == BRT_OK && e == hipSuccess) e = hipMemcpy(oid, ids, sizeof oid, hipMemcpyDeviceToHost);
+    if (rc == BRT_OK && e == hipSuccess) {
+      const bool changed = oid[1] != hid[1] || oid[0] != hid[0] || host[8] != hit->t;
+      hit->t = host[8];
+      hit->u = host[9];
+      hit->v = host[10];
+      hit->normal[0] = host[11];
+      hit->normal[1] = host[12];
+      hit->normal[2] = host[13];
+      hit->prim_id = oid[0];
+      hit->geom_id = oid[1];
+      if (hit_out) *hit_out = changed ? 1 : 0;
+    }
+  }
+  (void)hipFree(d);
+  if (rc != BRT_OK) return rc;
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("intersect1: ") + hipGetErrorString(e));
+  return BRT_OK;
+}
+
+}  // extern "C"

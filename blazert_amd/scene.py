@@ -1,0 +1,216 @@
+"""Python mirror of blazeRT's scene API over the C ABI.
+
+``Scene`` follows ``blazert::BlazertScene<T>`` (blazert/scene.h:38-116): one add per geometry type, each add
+returns the geometry id (or ``INVALID_ID`` = unsigned(-1) for a second add of a type or an add after commit),
+``commit()`` builds the BVHs, ``intersect1(scene, ray, rayhit)`` is the single-ray query. ``Scene.intersect``
+is the batched form over device-resident rays (torch tensors on the GPU; torch is used for device memory and
+streams only, the traversal is the HIP code in libblazert_amd.so).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+from .capi import INVALID_ID, check, lib
+
+DBL_MAX = float(np.finfo(np.float64).max)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _up(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _f64(a, cols=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(-1, cols) if cols else a.reshape(-1)
+
+
+class Scene:
+    """blazert::Scene<double> (blazert/blazert.h:17-20) on the MI355X."""
+
+    def __init__(self, min_leaf_primitives: int = 4, max_tree_depth: int = 28, bin_size: int = 64):
+        opts = capi.BuildOptions(min_leaf_primitives, max_tree_depth, bin_size)
+        h = C.c_void_p()
+        check(lib().brt_scene_create(C.byref(opts), C.byref(h)))
+        self._h = h
+        self.has_been_committed = False
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().brt_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _add(self, fn, *args) -> int:
+        gid = C.c_uint32(INVALID_ID)
+        rc = fn(self._h, *args, C.byref(gid))
+        if rc == capi.ERR_STATE:
+            return INVALID_ID  # reference semantics: unsigned(-1), no exception (blazert/scene.h:237-239)
+        check(rc)
+        return gid.value
+
+    def add_mesh(self, vertices, triangles) -> int:
+        v = _f64(vertices, 3)
+        f = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1, 3)
+        return self._add(lib().brt_scene_add_mesh, _dp(v), len(v), _up(f), len(f))
+
+    def add_spheres(self, centers, radii) -> int:
+        c = _f64(centers, 3)
+        r = _f64(radii)
+        if len(c) != len(r):
+            raise ValueError("centers and radii must have the same length")
+        return self._add(lib().brt_scene_add_spheres, _dp(c), _dp(r), len(r))
+
+    def add_planes(self, centers, dxs, dys, rotations) -> int:
+        c = _f64(centers, 3)
+        dx, dy = _f64(dxs), _f64(dys)
+        R = _f64(rotations, 9)
+        if not (len(c) == len(dx) == len(dy) == len(R)):
+            raise ValueError("plane arrays must have the same length")
+        return self._add(lib().brt_scene_add_planes, _dp(c), _dp(dx), _dp(dy), _dp(R), len(dx))
+
+    def add_cylinders(self, centers, semi_axes_a, semi_axes_b, heights, rotations) -> int:
+        c = _f64(centers, 3)
+        a, b, h = _f64(semi_axes_a), _f64(semi_axes_b), _f64(heights)
+        R = _f64(rotations, 9)
+        if not (len(c) == len(a) == len(b) == len(h) == len(R)):
+            raise ValueError("cylinder arrays must have the same length")
+        return self._add(lib().brt_scene_add_cylinders, _dp(c), _dp(a), _dp(b), _dp(h), _dp(R), len(a))
+
+    def commit(self) -> bool:
+        check(lib().brt_scene_commit(self._h))
+        self.has_been_committed = True
+        return True
+
+    def upload(self) -> None:
+        check(lib().brt_scene_upload(self._h))
+
+    def build_stats(self, geom_type: int) -> dict:
+        s = capi.BuildStats()
+        check(lib().brt_scene_build_stats(self._h, geom_type, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in capi.BuildStats._fields_}
+
+    def export_tree(self, geom_type: int):
+        """Reference-form tree (pre-order nodes) and the primitive order of the leaves."""
+        n = C.c_uint32(0)
+        check(lib().brt_scene_export_tree(self._h, geom_type, None, 0, None, C.byref(n)))
+        dt = np.dtype([("min", np.float64, 3), ("max", np.float64, 3), ("leaf", np.uint32), ("axis", np.uint32),
+                       ("children", np.uint32, 2), ("prim_begin", np.uint32), ("prim_count", np.uint32)])
+        nodes = np.zeros(n.value, dtype=dt)
+        nprim = self.build_stats(geom_type)["num_primitives"]
+        order = np.zeros(max(nprim, 1), dtype=np.uint32)
+        check(lib().brt_scene_export_tree(self._h, geom_type, nodes.ctypes.data_as(C.c_void_p), n.value, _up(order),
+                                          C.byref(n)))
+        return nodes, order[:nprim]
+
+    # -------------------------------------------------------------------------------------------- batched
+    def intersect(self, org, dirs, tmin=None, tmax=None, *, precision: int = capi.PRECISION_FP64,
+                  any_hit: bool = False, normals: bool = False, hits: dict | None = None, block_size: int = 0,
+                  stream=None) -> dict:
+        """Batched intersect1 over n rays. org / dirs: (n,3) float64 CUDA tensors or tuples of three columns.
+
+        Returns a dict of CUDA tensors t, u, v (float64), prim_id, geom_id (int32 holding uint32 bits) and,
+        with normals=True, normal (n,3). Asynchronous on `stream` (torch's current stream by default).
+        With ``hits`` given (same dict layout), those tensors are used as in/out RayHits.
+        """
+        import torch
+
+        cols_o = _columns(org)
+        cols_d = _columns(dirs)
+        n = cols_o[0].numel()
+        dev = cols_o[0].device
+        tmn = None if tmin is None else _scalar_col(tmin, n, dev)
+        tmx = None if tmax is None else _scalar_col(tmax, n, dev)
+        flags = capi.FLAG_ANY_HIT if any_hit else 0
+        if hits is None:
+            hits = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in ("t", "u", "v")}
+            hits["prim_id"] = torch.empty(n, dtype=torch.int32, device=dev)
+            hits["geom_id"] = torch.empty(n, dtype=torch.int32, device=dev)
+            if normals:
+                hits["normal"] = torch.empty((3, n), dtype=torch.float64, device=dev)
+        else:
+            flags |= capi.FLAG_HITS_INOUT
+        nrm = hits.get("normal")
+        rays = capi.Rays(*[c.data_ptr() for c in cols_o], *[c.data_ptr() for c in cols_d],
+                         tmn.data_ptr() if tmn is not None else None, tmx.data_ptr() if tmx is not None else None)
+        H = capi.Hits(hits["t"].data_ptr(), hits["u"].data_ptr(), hits["v"].data_ptr(), hits["prim_id"].data_ptr(),
+                      hits["geom_id"].data_ptr(), *([nrm[k].data_ptr() for k in range(3)] if nrm is not None
+                                                    else [None, None, None]))
+        L = capi.Launch(precision, flags, block_size, 0)
+        st = stream if stream is not None else torch.cuda.current_stream(dev)
+        keep = (cols_o, cols_d, tmn, tmx)  # noqa: F841 (inputs alive until the launch is enqueued)
+        check(lib().brt_intersect(self._h, C.byref(rays), C.byref(H), n, C.byref(L), C.c_void_p(st.cuda_stream)))
+        return hits
+
+
+def _columns(x):
+    """(n,3) tensor or three columns -> three contiguous float64 columns on the same device."""
+    import torch
+
+    if isinstance(x, (list, tuple)) and len(x) == 3:
+        return [torch.as_tensor(c, dtype=torch.float64).contiguous() for c in x]
+    x = torch.as_tensor(x, dtype=torch.float64)
+    if not (x.dim() == 2 and x.shape[1] == 3):
+        raise ValueError("expected an (n,3) tensor or a tuple of three columns")
+    x = x.t()
+    return [x[k].contiguous() for k in range(3)]
+
+
+def _scalar_col(v, n, dev):
+    import torch
+
+    t = torch.as_tensor(v, dtype=torch.float64, device=dev)
+    if t.dim() == 0:
+        t = t.expand(n)
+    return t.contiguous()
+
+
+# ------------------------------------------------------------------------------------------------ drop-in
+@dataclass
+class Ray:
+    """blazert::Ray<double> (blazert/ray.h:20-66): origin, direction, min/max hit distance, any_hit."""
+    origin: tuple
+    direction: tuple
+    min_hit_distance: float = 0.0
+    max_hit_distance: float = DBL_MAX
+    any_hit: bool = False
+
+
+@dataclass
+class RayHit:
+    """blazert::RayHit<double> (blazert/ray.h:72-85) with the reference defaults."""
+    normal: list = field(default_factory=lambda: [0.0, 0.0, 0.0])
+    uv: list = field(default_factory=lambda: [0.0, 0.0])
+    hit_distance: float = DBL_MAX
+    prim_id: int = INVALID_ID
+    geom_id: int = INVALID_ID
+
+
+def intersect1(scene: Scene, ray: Ray, rayhit: RayHit) -> bool:
+    """intersect1(scene, ray, rayhit) (blazert/scene.h:161-211): one ray on the GPU, rayhit updated in place."""
+    h = capi.Hit1((C.c_double * 3)(*rayhit.normal), rayhit.uv[0], rayhit.uv[1], rayhit.hit_distance,
+                  rayhit.prim_id, rayhit.geom_id)
+    o = np.asarray(ray.origin, dtype=np.float64)
+    d = np.asarray(ray.direction, dtype=np.float64)
+    res = C.c_int(0)
+    check(lib().brt_intersect1(scene.handle, _dp(o), _dp(d), float(ray.min_hit_distance), float(ray.max_hit_distance),
+                               capi.FLAG_ANY_HIT if ray.any_hit else 0, C.byref(h), C.byref(res)))
+    rayhit.normal = [h.normal[0], h.normal[1], h.normal[2]]
+    rayhit.uv = [h.u, h.v]
+    rayhit.hit_distance = h.t
+    rayhit.prim_id = h.prim_id
+    rayhit.geom_id = h.geom_id
+    return bool(res.value)

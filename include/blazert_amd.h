@@ -1,0 +1,166 @@
+/*
+ * blazert_amd.h -- C ABI of the MI355X closest-hit ray-query engine.
+ *
+ * Drop-in boundary for blazeRT's scene API (cstatz/blazert, header-only C++17):
+ *   blazert::Scene<T> = BlazertScene<T>               blazert/blazert.h:15-21, blazert/scene.h:38-116
+ *   Scene::add_mesh / add_spheres / add_planes / add_cylinders   blazert/scene.h:75-81, 227-346
+ *   Scene::commit()                                   blazert/scene.h:91-115
+ *   intersect1(scene, ray, rayhit)                    blazert/scene.h:161-211
+ *   Ray<T> / RayHit<T>                                blazert/ray.h:20-85
+ *   BVHBuildOptions / BVHBuildStatistics              blazert/bvh/options.h:14-28, blazert/bvh/statistics.h:10-37
+ *
+ * Plain C types only. All functions return BRT_OK (0) or a negative BRT_ERR_* code; brt_last_error()
+ * returns a message for the calling thread's last failure. Geometry is copied at add time (the reference
+ * keeps references to caller-owned vectors; here the caller may free its arrays after the call).
+ * After brt_scene_commit() the scene is immutable and every intersect entry point is reentrant.
+ */
+#ifndef BLAZERT_AMD_H
+#define BLAZERT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BRT_OK 0
+#define BRT_ERR_INVALID_ARGUMENT (-1)
+#define BRT_ERR_STATE (-2)        /* e.g. second add of a type, add after commit, intersect before commit */
+#define BRT_ERR_DEVICE (-3)       /* HIP runtime failure */
+#define BRT_ERR_NO_DEVICE (-4)    /* no usable gfx950 device */
+#define BRT_ERR_OUT_OF_MEMORY (-5)
+#define BRT_ERR_UNSUPPORTED (-6)  /* e.g. tree deeper than the device stack */
+
+#define BRT_INVALID_ID 0xFFFFFFFFu
+
+/* geometry types; intersect1 evaluates them in this fixed order (blazert/scene.h:169-209) */
+#define BRT_GEOM_TRIANGLES 0u
+#define BRT_GEOM_SPHERES 1u
+#define BRT_GEOM_PLANES 2u
+#define BRT_GEOM_CYLINDERS 3u
+
+/* traversal precision (launch option) */
+#define BRT_PRECISION_FP64 0u              /* drop-in: FP64 everywhere, bit-identical to the reference */
+#define BRT_PRECISION_FP32_TRAV_FP64_HIT 1u /* FP32 conservative slab tests, FP64 primitive tests */
+
+/* launch flags */
+#define BRT_FLAG_HITS_INOUT 1u  /* hits hold a prior RayHit: its t is the merge threshold, misses leave it */
+#define BRT_FLAG_ANY_HIT 2u     /* Ray::AnyHit::yes (blazert/ray.h:24,50; blazert/bvh/accel.h:109-110) */
+
+typedef struct brt_scene brt_scene;
+
+/* BVHBuildOptions (blazert/bvh/options.h:21-27): defaults 4 / 28 / 64 */
+typedef struct brt_build_options {
+  uint32_t min_leaf_primitives;
+  uint32_t max_tree_depth;
+  uint32_t bin_size;
+} brt_build_options;
+
+/* BVHBuildStatistics (blazert/bvh/statistics.h:10-37) plus the flattened sizes */
+typedef struct brt_build_stats {
+  uint32_t max_tree_depth;
+  uint32_t leaf_nodes;
+  uint32_t max_num_prims_per_leaf;
+  uint32_t min_num_prims_per_leaf;
+  uint32_t branch_nodes;
+  uint32_t bad_splits;
+  double build_time_s;
+  uint32_t num_primitives;
+  uint32_t geom_id;
+  uint64_t device_bytes; /* nodes (FP64 + FP32) + primitive records + ids + post-processing data */
+} brt_build_stats;
+
+/* Reference-form tree node (pre-order numbering as built by blazert/bvh/builder.h:84-115). */
+typedef struct brt_tree_node {
+  double min[3], max[3];
+  uint32_t leaf, axis;
+  uint32_t children[2];
+  uint32_t prim_begin, prim_count; /* range into the exported primitive order */
+} brt_tree_node;
+
+/* Device rays, structure of arrays (device pointers). tmin/tmax may be NULL: 0 and DBL_MAX
+ * (blazert/ray.h:172-173). Directions need not be unit length: like Ray<T>, the engine normalizes them for the
+ * primitive tests and uses 1/raw direction for the slab tests (blazert/ray.h:56-65). */
+typedef struct brt_rays {
+  const double *ox, *oy, *oz;
+  const double *dx, *dy, *dz;
+  const double *tmin, *tmax;
+} brt_rays;
+
+/* Device hits, structure of arrays. nx/ny/nz may be NULL (no normals computed). Without
+ * BRT_FLAG_HITS_INOUT every ray is written: a miss is t = DBL_MAX, ids = BRT_INVALID_ID, u = v = 0. */
+typedef struct brt_hits {
+  double *t, *u, *v;
+  uint32_t *prim_id, *geom_id;
+  double *nx, *ny, *nz;
+} brt_hits;
+
+typedef struct brt_launch {
+  uint32_t precision;  /* BRT_PRECISION_* */
+  uint32_t flags;      /* BRT_FLAG_* */
+  uint32_t block_size; /* 0 = library default */
+  uint32_t reserved;
+} brt_launch;
+
+/* One ray / hit in host memory, for the single-ray drop-in (RayHit<T>, blazert/ray.h:72-85). */
+typedef struct brt_hit1 {
+  double normal[3];
+  double u, v;
+  double t;
+  uint32_t prim_id, geom_id;
+} brt_hit1;
+
+const char *brt_last_error(void);
+const char *brt_version(void);
+
+/* Number of usable gfx950 devices (0 when none; never fails). */
+int brt_device_count(void);
+
+/* BlazertScene<T>() with build_options (blazert/scene.h:42,73); opts may be NULL for the defaults. */
+int brt_scene_create(const brt_build_options *opts, brt_scene **out);
+/* Releases host and device memory of the scene. */
+int brt_scene_destroy(brt_scene *scene);
+
+/* add_mesh (blazert/scene.h:227-240): vertices xyz (3*nv doubles), faces (3*nf uint32).
+ * geom_id receives the geometry id, or BRT_INVALID_ID with BRT_ERR_STATE on a second add / add after commit. */
+int brt_scene_add_mesh(brt_scene *scene, const double *vertices, uint32_t num_vertices, const uint32_t *faces,
+                       uint32_t num_faces, uint32_t *geom_id);
+/* add_spheres (blazert/scene.h:258-271) */
+int brt_scene_add_spheres(brt_scene *scene, const double *centers, const double *radii, uint32_t n,
+                          uint32_t *geom_id);
+/* add_planes (blazert/scene.h:293-307); rotations are row-major 3x3 (9 doubles each) */
+int brt_scene_add_planes(brt_scene *scene, const double *centers, const double *dxs, const double *dys,
+                         const double *rotations, uint32_t n, uint32_t *geom_id);
+/* add_cylinders (blazert/scene.h:331-346) */
+int brt_scene_add_cylinders(brt_scene *scene, const double *centers, const double *semi_axes_a,
+                            const double *semi_axes_b, const double *heights, const double *rotations, uint32_t n,
+                            uint32_t *geom_id);
+
+/* commit (blazert/scene.h:91-115): builds every BVH on the host (binned SAH, reference topology) and
+ * flattens it. Device upload happens on the first intersect, or explicitly with brt_scene_upload. */
+int brt_scene_commit(brt_scene *scene);
+/* Copies the committed scene to the current HIP device (idempotent). */
+int brt_scene_upload(brt_scene *scene);
+
+/* Statistics of one geometry type's build (BRT_ERR_INVALID_ARGUMENT if the type is absent). */
+int brt_scene_build_stats(const brt_scene *scene, uint32_t geom_type, brt_build_stats *stats);
+
+/* Reference-form export of one built tree (for inspection and parity tests). Call with nodes == NULL to get
+ * *num_nodes; prim_order receives num_primitives ids in leaf order. */
+int brt_scene_export_tree(const brt_scene *scene, uint32_t geom_type, brt_tree_node *nodes, uint32_t capacity,
+                          uint32_t *prim_order, uint32_t *num_nodes);
+
+/* Batched intersect1 (blazert/scene.h:161-211) for n rays resident on the device, enqueued on `stream`
+ * (a hipStream_t, NULL = default stream). launch may be NULL (FP64, no flags). Asynchronous. */
+int brt_intersect(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,
+                  const brt_launch *launch, void *stream);
+
+/* Single-ray drop-in for intersect1(scene, Ray<T>{org, dir, tmin, tmax}, rayhit): host memory in/out, blocking.
+ * hit is in/out exactly like RayHit<T>; *hit_out receives the bool result. */
+int brt_intersect1(const brt_scene *scene, const double org[3], const double dir[3], double tmin, double tmax,
+                   uint32_t flags, brt_hit1 *hit, int *hit_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BLAZERT_AMD_H */

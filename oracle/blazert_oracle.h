@@ -1,0 +1,124 @@
+/*
+ * blazert_oracle.h -- CPU restatement of blazeRT's closest-hit path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This is the parity oracle. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker or the timed CPU baseline. The product path
+ * (blazert_amd/) never links, loads or calls anything in this directory.
+ *
+ * Every function restates one reference function (file:line under the blazert repository):
+ *   ray precompute          blazert/ray.h:56-65
+ *   slab test               blazert/bvh/intersect.h:32-51
+ *   leaf loop               blazert/bvh/intersect.h:66-78
+ *   traversal               blazert/bvh/accel.h:79-120
+ *   binned SAH builder      blazert/bvh/builder.h:46-226, blazert/bvh/binbuffer.h:15-167, blazert/bvh/bbox.h:27-39
+ *   triangles               blazert/primitives/trimesh.h:19-181
+ *   spheres                 blazert/primitives/sphere.h:60-177
+ *   planes                  blazert/primitives/plane.h:71-270
+ *   cylinders               blazert/primitives/cylinder.h:74-495
+ *   scene merge             blazert/scene.h:161-211
+ *
+ * Arithmetic conventions (blaze 3.7 is not available, so these are restated from its published
+ * semantics and documented as unpinned at the bit level):
+ *   dot(a,b) = (a0*b0 + a1*b1) + a2*b2; norm(a) = sqrt(dot(a,a));
+ *   normalize(a) = a * (len != 0 ? 1/len : 0);  vec / scalar = vec * (1/scalar);
+ *   mat*vec row i = (R[i0]*v0 + R[i1]*v1) + R[i2]*v2; cross as in the textbook component order.
+ * Compiled with -ffp-contract=off (no FMA contraction), like the HIP product kernels.
+ */
+#ifndef BLAZERT_ORACLE_H
+#define BLAZERT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_TRIANGLES = 0, ORC_SPHERES = 1, ORC_PLANES = 2, ORC_CYLINDERS = 3, ORC_NUM_TYPES = 4 };
+
+/* reference Stack<unsigned, BLAZERT_MAX_TREE_DEPTH> capacity (blazert/defines.h:5-7, blazert/bvh/accel.h:83) */
+#define ORC_REF_STACK_CAPACITY 28u
+
+typedef struct orc_ray {
+  double org[3];
+  double dir[3]; /* normalize(raw direction) */
+  double inv[3]; /* 1 / raw direction */
+  uint32_t sign[3]; /* raw direction < 0 */
+  double tmin, tmax;
+  int any_hit;
+} orc_ray;
+
+typedef struct orc_hit {
+  double normal[3];
+  double u, v;
+  double t;
+  uint32_t prim_id, geom_id;
+} orc_hit;
+
+/* per-ray traversal trace used for tie exclusion and algorithmic byte counts */
+typedef struct orc_trace {
+  uint32_t node_visits;  /* nodes popped and slab-tested */
+  uint32_t prim_tests;   /* primitive intersection calls */
+  uint32_t max_stack;    /* largest reference stack size reached */
+  uint32_t overflow;     /* reference stack would exceed its 28 slots (undefined behaviour there) */
+  uint32_t tie;          /* two distinct tested candidates within 1e-12 relative of the final t */
+  uint32_t multi_cyl;    /* >= 2 cylinder accepts (order dependent: last one wins) */
+} orc_trace;
+
+typedef struct orc_node {
+  double min[3], max[3];
+  uint32_t leaf, axis;
+  uint32_t children[2];
+  uint32_t prim_begin, prim_count; /* range into orc_bvh.prim_order */
+} orc_node;
+
+typedef struct orc_stats {
+  uint32_t max_tree_depth, leaf_nodes, max_num_prims_per_leaf, min_num_prims_per_leaf, branch_nodes, bad_splits;
+} orc_stats;
+
+typedef struct orc_scene orc_scene;
+
+/* scene construction mirrors blazert/scene.h:227-346; returns geom id or 0xFFFFFFFF */
+orc_scene *orc_scene_new(uint32_t min_leaf_primitives, uint32_t max_tree_depth, uint32_t bin_size);
+void orc_scene_free(orc_scene *s);
+uint32_t orc_add_mesh(orc_scene *s, const double *vertices, uint32_t nv, const uint32_t *faces, uint32_t nf);
+uint32_t orc_add_spheres(orc_scene *s, const double *centers, const double *radii, uint32_t n);
+uint32_t orc_add_planes(orc_scene *s, const double *centers, const double *dxs, const double *dys,
+                        const double *rot9, uint32_t n);
+uint32_t orc_add_cylinders(orc_scene *s, const double *centers, const double *a, const double *b,
+                           const double *heights, const double *rot9, uint32_t n);
+int orc_commit(orc_scene *s);
+
+/* collection precompute accessors (boxes / centers) for known-answer tests */
+int orc_prim_box(const orc_scene *s, int type, uint32_t prim, double bmin[3], double bmax[3]);
+int orc_prim_center(const orc_scene *s, int type, uint32_t prim, double c[3]);
+
+/* built tree export (reference pre-order layout) */
+uint32_t orc_tree_size(const orc_scene *s, int type);
+int orc_tree_export(const orc_scene *s, int type, orc_node *nodes, uint32_t *prim_order, orc_stats *stats);
+
+void orc_ray_init(orc_ray *r, const double o[3], const double d[3], double tmin, double tmax, int any_hit);
+int orc_intersect_node(double *min_distance, double *max_distance, const double bmin[3], const double bmax[3],
+                       const orc_ray *ray);
+
+/* single primitive call sequence prepare -> intersect(prim) -> post (the reference tests' helper) */
+int orc_intersect_single(const orc_scene *s, int type, uint32_t prim, const orc_ray *ray, orc_hit *hit);
+/* low-level traverse of one geometry type (blazert/bvh/accel.h:60-120) */
+int orc_traverse(const orc_scene *s, int type, const orc_ray *ray, orc_hit *hit, orc_trace *trace);
+/* intersect1 (blazert/scene.h:161-211); hit is in/out like the reference RayHit */
+int orc_intersect1(const orc_scene *s, const orc_ray *ray, orc_hit *hit, orc_trace *trace);
+
+/* batched intersect1 over SoA rays, OpenMP over rays. hits start as a default RayHit.
+ * trace arrays may be NULL. Returns number of hits. */
+uint64_t orc_intersect_batch(const orc_scene *s, uint64_t n, const double *ox, const double *oy, const double *oz,
+                             const double *dx, const double *dy, const double *dz, const double *tmin,
+                             const double *tmax, double *t, double *u, double *v, uint32_t *prim_id,
+                             uint32_t *geom_id, double *nx, double *ny, double *nz, orc_trace *trace, int nthreads,
+                             int any_hit);
+
+/* brute-force closest candidate per type (no BVH): every primitive tested from a fresh intersector */
+int orc_bruteforce_type(const orc_scene *s, int type, const orc_ray *ray, orc_hit *hit);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,256 @@
+"""ctypes binding of the CPU oracle (oracle/blazert_oracle.c). TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg import this module, and only
+as the checker or the timed CPU baseline; the product package ``blazert_amd`` never imports it.
+The oracle restates blazeRT's ``intersect1`` path (blazert/scene.h:161-211 and the functions it calls).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+TRIANGLES, SPHERES, PLANES, CYLINDERS = 0, 1, 2, 3
+INVALID = 0xFFFFFFFF
+REF_STACK_CAPACITY = 28
+
+
+class Ray(C.Structure):
+    _fields_ = [("org", C.c_double * 3), ("dir", C.c_double * 3), ("inv", C.c_double * 3),
+                ("sign", C.c_uint32 * 3), ("tmin", C.c_double), ("tmax", C.c_double), ("any_hit", C.c_int)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("normal", C.c_double * 3), ("u", C.c_double), ("v", C.c_double), ("t", C.c_double),
+                ("prim_id", C.c_uint32), ("geom_id", C.c_uint32)]
+
+    @staticmethod
+    def default() -> "Hit":
+        h = Hit()
+        h.t = np.finfo(np.float64).max
+        h.prim_id = INVALID
+        h.geom_id = INVALID
+        return h
+
+
+class Trace(C.Structure):
+    _fields_ = [("node_visits", C.c_uint32), ("prim_tests", C.c_uint32), ("max_stack", C.c_uint32),
+                ("overflow", C.c_uint32), ("tie", C.c_uint32), ("multi_cyl", C.c_uint32)]
+
+
+TRACE_DTYPE = np.dtype([("node_visits", np.uint32), ("prim_tests", np.uint32), ("max_stack", np.uint32),
+                        ("overflow", np.uint32), ("tie", np.uint32), ("multi_cyl", np.uint32)])
+
+
+class Node(C.Structure):
+    _fields_ = [("min", C.c_double * 3), ("max", C.c_double * 3), ("leaf", C.c_uint32), ("axis", C.c_uint32),
+                ("children", C.c_uint32 * 2), ("prim_begin", C.c_uint32), ("prim_count", C.c_uint32)]
+
+
+NODE_DTYPE = np.dtype([("min", np.float64, 3), ("max", np.float64, 3), ("leaf", np.uint32), ("axis", np.uint32),
+                       ("children", np.uint32, 2), ("prim_begin", np.uint32), ("prim_count", np.uint32)])
+assert NODE_DTYPE.itemsize == C.sizeof(Node)
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("max_tree_depth", "leaf_nodes", "max_num_prims_per_leaf",
+                                          "min_num_prims_per_leaf", "branch_nodes", "bad_splits")]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc). Returns the library path."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        up = C.POINTER(C.c_uint32)
+        L.orc_scene_new.restype = P
+        L.orc_scene_new.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_scene_free.argtypes = [P]
+        L.orc_add_mesh.restype = C.c_uint32
+        L.orc_add_mesh.argtypes = [P, dp, C.c_uint32, up, C.c_uint32]
+        L.orc_add_spheres.restype = C.c_uint32
+        L.orc_add_spheres.argtypes = [P, dp, dp, C.c_uint32]
+        L.orc_add_planes.restype = C.c_uint32
+        L.orc_add_planes.argtypes = [P, dp, dp, dp, dp, C.c_uint32]
+        L.orc_add_cylinders.restype = C.c_uint32
+        L.orc_add_cylinders.argtypes = [P, dp, dp, dp, dp, dp, C.c_uint32]
+        L.orc_commit.argtypes = [P]
+        L.orc_prim_box.argtypes = [P, C.c_int, C.c_uint32, dp, dp]
+        L.orc_prim_center.argtypes = [P, C.c_int, C.c_uint32, dp]
+        L.orc_tree_size.restype = C.c_uint32
+        L.orc_tree_size.argtypes = [P, C.c_int]
+        L.orc_tree_export.argtypes = [P, C.c_int, C.c_void_p, up, C.POINTER(Stats)]
+        L.orc_ray_init.argtypes = [C.POINTER(Ray), dp, dp, C.c_double, C.c_double, C.c_int]
+        L.orc_intersect_node.argtypes = [dp, dp, dp, dp, C.POINTER(Ray)]
+        L.orc_intersect_single.argtypes = [P, C.c_int, C.c_uint32, C.POINTER(Ray), C.POINTER(Hit)]
+        L.orc_traverse.argtypes = [P, C.c_int, C.POINTER(Ray), C.POINTER(Hit), C.POINTER(Trace)]
+        L.orc_intersect1.argtypes = [P, C.POINTER(Ray), C.POINTER(Hit), C.POINTER(Trace)]
+        L.orc_intersect_batch.restype = C.c_uint64
+        L.orc_intersect_batch.argtypes = [P, C.c_uint64] + [dp] * 8 + [dp, dp, dp, up, up, dp, dp, dp, C.c_void_p,
+                                                                        C.c_int, C.c_int]
+        L.orc_bruteforce_type.argtypes = [P, C.c_int, C.POINTER(Ray), C.POINTER(Hit)]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32)) if a is not None else None
+
+
+def _c3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def make_ray(org, direction, tmin=0.0, tmax=None, any_hit=False) -> Ray:
+    r = Ray()
+    if tmax is None:
+        tmax = np.finfo(np.float64).max
+    lib().orc_ray_init(C.byref(r), _c3(org), _c3(direction), float(tmin), float(tmax), int(any_hit))
+    return r
+
+
+def intersect_node(bmin, bmax, ray: Ray, tmin: float, tmax: float):
+    a = C.c_double(tmin)
+    b = C.c_double(tmax)
+    hit = lib().orc_intersect_node(C.byref(a), C.byref(b), _c3(bmin), _c3(bmax), C.byref(ray))
+    return bool(hit), a.value, b.value
+
+
+class Scene:
+    """Oracle scene with blazeRT's add/commit/intersect1 semantics."""
+
+    def __init__(self, min_leaf_primitives=4, max_tree_depth=28, bin_size=64):
+        self._keep = []
+        self.h = lib().orc_scene_new(min_leaf_primitives, max_tree_depth, bin_size)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_free(self.h)
+            self.h = None
+
+    def _k(self, *arrs):
+        self._keep.extend(arrs)
+
+    def add_mesh(self, vertices, faces) -> int:
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+        self._k(v, f)
+        return lib().orc_add_mesh(self.h, _dp(v), len(v), _up(f), len(f))
+
+    def add_spheres(self, centers, radii) -> int:
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        r = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+        return lib().orc_add_spheres(self.h, _dp(c), _dp(r), len(r))
+
+    def add_planes(self, centers, dxs, dys, rotations) -> int:
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        dx = np.ascontiguousarray(dxs, dtype=np.float64).reshape(-1)
+        dy = np.ascontiguousarray(dys, dtype=np.float64).reshape(-1)
+        R = np.ascontiguousarray(rotations, dtype=np.float64).reshape(-1, 9)
+        return lib().orc_add_planes(self.h, _dp(c), _dp(dx), _dp(dy), _dp(R), len(dx))
+
+    def add_cylinders(self, centers, semi_axes_a, semi_axes_b, heights, rotations) -> int:
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        a = np.ascontiguousarray(semi_axes_a, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(semi_axes_b, dtype=np.float64).reshape(-1)
+        h = np.ascontiguousarray(heights, dtype=np.float64).reshape(-1)
+        R = np.ascontiguousarray(rotations, dtype=np.float64).reshape(-1, 9)
+        return lib().orc_add_cylinders(self.h, _dp(c), _dp(a), _dp(b), _dp(h), _dp(R), len(a))
+
+    def commit(self) -> bool:
+        return bool(lib().orc_commit(self.h))
+
+    def prim_box(self, type_, prim):
+        mn = np.zeros(3)
+        mx = np.zeros(3)
+        rc = lib().orc_prim_box(self.h, type_, prim, _dp(mn), _dp(mx))
+        if rc:
+            raise IndexError(prim)
+        return mn, mx
+
+    def prim_center(self, type_, prim):
+        c = np.zeros(3)
+        if lib().orc_prim_center(self.h, type_, prim, _dp(c)):
+            raise IndexError(prim)
+        return c
+
+    def tree(self, type_):
+        n = lib().orc_tree_size(self.h, type_)
+        nodes = np.zeros(n, dtype=NODE_DTYPE)
+        nprim = 0 if n == 0 else None
+        st = Stats()
+        # prim_order length = collection size = sum of leaf counts
+        lib().orc_tree_export(self.h, type_, nodes.ctypes.data_as(C.c_void_p), None, C.byref(st))
+        nprim = int(nodes["prim_count"][nodes["leaf"] == 1].sum())
+        order = np.zeros(max(nprim, 1), dtype=np.uint32)
+        lib().orc_tree_export(self.h, type_, nodes.ctypes.data_as(C.c_void_p), _up(order), C.byref(st))
+        stats = {name: getattr(st, name) for name, _ in Stats._fields_}
+        return nodes, order[:nprim], stats
+
+    def intersect_single(self, type_, prim, ray: Ray):
+        h = Hit.default()
+        hit = lib().orc_intersect_single(self.h, type_, prim, C.byref(ray), C.byref(h))
+        return bool(hit), h
+
+    def traverse(self, type_, ray: Ray):
+        h = Hit.default()
+        tr = Trace()
+        hit = lib().orc_traverse(self.h, type_, C.byref(ray), C.byref(h), C.byref(tr))
+        return bool(hit), h, tr
+
+    def intersect1(self, ray: Ray, hit: Hit | None = None):
+        h = hit if hit is not None else Hit.default()
+        tr = Trace()
+        res = lib().orc_intersect1(self.h, C.byref(ray), C.byref(h), C.byref(tr))
+        return bool(res), h, tr
+
+    def bruteforce(self, type_, ray: Ray):
+        h = Hit.default()
+        hit = lib().orc_bruteforce_type(self.h, type_, C.byref(ray), C.byref(h))
+        return bool(hit), h
+
+    def intersect_batch(self, org, dirs, tmin=None, tmax=None, normals=True, trace=False, nthreads=0, any_hit=False):
+        """Batched intersect1 on the CPU. org/dirs: (n,3) float64. Returns a dict of numpy arrays."""
+        org = np.ascontiguousarray(org, dtype=np.float64).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        n = len(org)
+        cols = [np.ascontiguousarray(org[:, k]) for k in range(3)] + [np.ascontiguousarray(dirs[:, k])
+                                                                       for k in range(3)]
+        tmn = None if tmin is None else np.ascontiguousarray(np.broadcast_to(tmin, (n,)), dtype=np.float64)
+        tmx = None if tmax is None else np.ascontiguousarray(np.broadcast_to(tmax, (n,)), dtype=np.float64)
+        out = {k: np.zeros(n) for k in ("t", "u", "v")}
+        out["prim_id"] = np.zeros(n, dtype=np.uint32)
+        out["geom_id"] = np.zeros(n, dtype=np.uint32)
+        nrm = [np.zeros(n) for _ in range(3)] if normals else [None, None, None]
+        tr = np.zeros(n, dtype=TRACE_DTYPE) if trace else None
+        lib().orc_intersect_batch(self.h, n, *[_dp(c) for c in cols], _dp(tmn), _dp(tmx), _dp(out["t"]),
+                                  _dp(out["u"]), _dp(out["v"]), _up(out["prim_id"]), _up(out["geom_id"]),
+                                  *[_dp(c) for c in nrm], tr.ctypes.data_as(C.c_void_p) if tr is not None else None,
+                                  int(nthreads), int(any_hit))
+        if normals:
+            out["normal"] = np.stack(nrm, axis=1)
+        if trace:
+            out["trace"] = tr
+        return out

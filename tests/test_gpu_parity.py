@@ -1,0 +1,308 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on identical seeded inputs.
+
+FP64 path: bit-exact on every field (t, u, v, prim_id, geom_id, normal) for every ray except those where the
+reference's fixed 28-slot stack would have overflowed (undefined behaviour in the reference; counted, and zero
+for all sets here). Split-precision path: hit ids must agree on every ray that is neither a near-tie nor
+order-dependent (two cylinders accepted), and agreeing hits must have identical t/u/v (|dt|/t <= 1e-12 bar).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from blazert_amd import workloads as W  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+INVALID = 0xFFFFFFFF
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import blazert_amd  # noqa: F401  (raises if libblazert_amd.so is missing)
+    return torch.device("cuda:0")
+
+
+def gpu_intersect(scene, o, d, dev, tmin=None, tmax=None, precision=0, any_hit=False, normals=True):
+    O = torch.from_numpy(np.ascontiguousarray(o)).to(dev)
+    D = torch.from_numpy(np.ascontiguousarray(d)).to(dev)
+    kw = {}
+    if tmin is not None:
+        kw["tmin"] = torch.from_numpy(np.ascontiguousarray(tmin, dtype=np.float64)).to(dev)
+    if tmax is not None:
+        kw["tmax"] = torch.from_numpy(np.ascontiguousarray(tmax, dtype=np.float64)).to(dev)
+    h = scene.intersect(O, D, precision=precision, any_hit=any_hit, normals=normals, **kw)
+    torch.cuda.synchronize()
+    out = {k: h[k].cpu().numpy() for k in ("t", "u", "v")}
+    out["prim_id"] = h["prim_id"].cpu().numpy().view(np.uint32)
+    out["geom_id"] = h["geom_id"].cpu().numpy().view(np.uint32)
+    if normals:
+        out["normal"] = h["normal"].cpu().numpy().T
+    return out
+
+
+def make_pair(oracle, fill, **opts):
+    from blazert_amd import Scene
+    p = Scene(**opts)
+    fill(p)
+    p.commit()
+    o = oracle.Scene(*(opts.get(k, dflt) for k, dflt in (("min_leaf_primitives", 4), ("max_tree_depth", 28),
+                                                           ("bin_size", 64))))
+    fill(o)
+    o.commit()
+    return p, o
+
+
+def assert_bit_exact(g, r, mask=None, normals=True):
+    m = np.ones(len(g["t"]), bool) if mask is None else mask
+    fields = ["t", "u", "v", "prim_id", "geom_id"]
+    for k in fields:
+        a, b = np.asarray(g[k])[m], np.asarray(r[k])[m]
+        bad = np.nonzero(~_same(a, b))[0]
+        assert len(bad) == 0, f"{k}: {len(bad)} mismatches, first rays {np.nonzero(m)[0][bad[:5]]}"
+    if normals:
+        same = _same(np.asarray(g["normal"])[m], np.asarray(r["normal"])[m])
+        assert same.all(), f"normal: {int((~same).any(axis=1).sum())} mismatches"
+
+
+def _same(a, b):
+    """Bitwise equality; two NaNs count as equal (their payload bits differ between CPU and GPU)."""
+    if a.dtype != np.float64:
+        return a == b
+    return (a.view(np.uint64) == b.view(np.uint64)) | (np.isnan(a) & np.isnan(b))
+
+
+# ------------------------------------------------------------------------------------------------ known answers
+DOC = json.load(open(os.path.join(GOLDEN, "known_answers.json")))
+
+
+def _fill_case(case):
+    def fill(s):
+        for g in case.get("add_order") or [case["geometry"]]:
+            if g == "mesh":
+                s.add_mesh(np.array(case["mesh"]["vertices"], float), np.array(case["mesh"]["faces"], np.uint32))
+            elif g == "spheres":
+                s.add_spheres(np.array(case["spheres"]["centers"], float), case["spheres"]["radii"])
+            elif g == "planes":
+                d = case["planes"]
+                s.add_planes(np.array(d["centers"], float), d["dxs"], d["dys"], np.array(d["rotations"], float))
+            else:
+                d = case["cylinders"]
+                s.add_cylinders(np.array(d["centers"], float), d["a"], d["b"], d["heights"],
+                                np.array(d["rotations"], float))
+    return fill
+
+
+@pytest.mark.parametrize("case", [pytest.param(c, id=c["name"]) for c in DOC["cases"]
+                                  if c["kind"] in ("primitive", "scene", "traverse", "traverse_distance")])
+def test_known_answers_on_gpu(dev, oracle_mod, case):
+    p, o = make_pair(oracle_mod, _fill_case(case))
+    org = np.array([case["org"]], float)
+    dirs = np.array([case["dir"]], float)
+    g = gpu_intersect(p, org, dirs, dev)
+    r = o.intersect_batch(org, dirs)
+    assert_bit_exact(g, r)
+    assert (g["geom_id"][0] != INVALID) == case["hit"]
+    if case["hit"]:
+        eps = DOC["approx_epsilon"]
+        assert abs(g["t"][0] - case["t"]) < eps * (1 + abs(case["t"]))
+
+
+def test_scene_primitives_golden_image(dev, oracle_mod):
+    """examples/baseline/scene_primitives.png reproduced pixel for pixel (cylinder quirks included)."""
+    sc = W.scene_primitives()
+    p, o = make_pair(oracle_mod, lambda s: (s.add_cylinders(*sc["cylinders"]), s.add_spheres(*sc["spheres"])))
+    org, dirs = W.scene_primitives_rays()
+    g = gpu_intersect(p, org, dirs, dev)
+    labels = np.load(os.path.join(GOLDEN, "scene_primitives_labels.npz"))["labels"]
+    mine = np.where(g["geom_id"] == INVALID, 0, np.where(g["prim_id"] == 0, 1, 2))
+    assert int((mine != labels).sum()) == 0
+    assert_bit_exact(g, o.intersect_batch(org, dirs))
+
+
+def test_sphere_field_full(dev, oracle_mod):
+    """1,024 spheres, 512x512 pinhole rays, tmax = +inf."""
+    c, r = W.sphere_field()
+    p, o = make_pair(oracle_mod, lambda s: s.add_spheres(c, r))
+    org, dirs = W.sphere_field_rays()
+    tmax = np.full(len(org), np.inf)
+    g = gpu_intersect(p, org, dirs, dev, tmax=tmax)
+    ref = o.intersect_batch(org, dirs, tmax=tmax, trace=True)
+    assert ref["trace"]["overflow"].sum() == 0
+    assert_bit_exact(g, ref)
+    assert (g["geom_id"] != INVALID).mean() > 0.05
+
+
+@pytest.fixture(scope="module")
+def ico8(oracle_mod):
+    v, f = W.icosphere(8)
+    return make_pair(oracle_mod, lambda s: s.add_mesh(v, f))
+
+
+def test_icosphere8_incoherent_subset(dev, ico8):
+    p, o = ico8
+    org, dirs = W.icosphere_incoherent_rays(1 << 18)
+    g = gpu_intersect(p, org, dirs, dev)
+    ref = o.intersect_batch(org, dirs, trace=True)
+    ok = ref["trace"]["overflow"] == 0
+    assert ok.mean() > 0.999
+    assert_bit_exact(g, ref, ok)
+
+
+def test_icosphere8_coherent_subset(dev, ico8):
+    p, o = ico8
+    org, dirs = W.icosphere_coherent_rays(640, 360)
+    g = gpu_intersect(p, org, dirs, dev)
+    ref = o.intersect_batch(org, dirs, trace=True)
+    ok = ref["trace"]["overflow"] == 0
+    assert_bit_exact(g, ref, ok)
+
+
+def test_icosphere8_full_incoherent_properties(dev, ico8):
+    """All 4,194,304 incoherent rays: size-independent properties (the subset above is compared bit for bit)."""
+    p, _ = ico8
+    org, dirs = W.icosphere_incoherent_rays()
+    g = gpu_intersect(p, org, dirs, dev, normals=False)
+    hit = g["geom_id"] != INVALID
+    assert 0.05 < hit.mean() < 0.9
+    assert (g["t"][hit] > 0).all() and (g["t"][~hit] == np.finfo(np.float64).max).all()
+    assert (g["prim_id"][hit] < 1310720).all()
+    u, v = g["u"][hit], g["v"][hit]
+    tol = 1e-9
+    assert (u >= -tol).all() and (v >= -tol).all() and (u + v <= 1 + tol).all()
+    # tracing a prefix of the set as its own batch gives the same bits (batch-size independence)
+    g2 = gpu_intersect(p, org[:1 << 18], dirs[:1 << 18], dev, normals=False)
+    assert np.array_equal(g2["t"].view(np.uint64), g["t"][:1 << 18].view(np.uint64))
+
+
+@pytest.fixture(scope="module")
+def mixed(oracle_mod):
+    m = W.mixed_scene(n_tri_target=40000, n_sph=10000, n_cyl=2000, n_pla=100, seed=17)
+
+    def fill(s):
+        s.add_mesh(*m["mesh"])
+        s.add_spheres(*m["spheres"])
+        s.add_cylinders(*m["cylinders"])
+        s.add_planes(*m["planes"])
+    return make_pair(oracle_mod, fill)
+
+
+def test_mixed_fp64_exact(dev, mixed):
+    p, o = mixed
+    org, dirs = W.mixed_rays(1 << 18, seed=5)
+    g = gpu_intersect(p, org, dirs, dev)
+    ref = o.intersect_batch(org, dirs, trace=True)
+    ok = ref["trace"]["overflow"] == 0
+    assert_bit_exact(g, ref, ok)
+    assert len(np.unique(g["geom_id"])) == 5  # four geometry types and misses all present
+
+
+def test_mixed_split_precision_agreement(dev, mixed):
+    p, o = mixed
+    org, dirs = W.mixed_rays(1 << 18, seed=5)
+    g32 = gpu_intersect(p, org, dirs, dev, precision=1)
+    ref = o.intersect_batch(org, dirs, trace=True)
+    tr = ref["trace"]
+    clean = (tr["overflow"] == 0) & (tr["tie"] == 0) & (tr["multi_cyl"] == 0)
+    same = (g32["geom_id"] == ref["geom_id"]) & (g32["prim_id"] == ref["prim_id"])
+    assert same[clean].mean() >= 0.9999, same[clean].mean()
+    m = same & (ref["geom_id"] != INVALID)
+    rel = np.abs(g32["t"][m] - ref["t"][m]) / np.abs(ref["t"][m])
+    assert rel.max() <= 1e-12
+
+
+def test_icosphere_split_precision_bit_identical(dev, ico8):
+    p, o = ico8
+    org, dirs = W.icosphere_incoherent_rays(1 << 18)
+    g32 = gpu_intersect(p, org, dirs, dev, precision=1)
+    ref = o.intersect_batch(org, dirs, trace=True)
+    ok = (ref["trace"]["overflow"] == 0) & (ref["trace"]["tie"] == 0)
+    assert_bit_exact(g32, ref, ok)
+
+
+def test_any_hit_matches_oracle(dev, mixed):
+    p, o = mixed
+    org, dirs = W.mixed_rays(1 << 16, seed=9)
+    g = gpu_intersect(p, org, dirs, dev, any_hit=True)
+    ref = o.intersect_batch(org, dirs, trace=True, any_hit=True)
+    assert_bit_exact(g, ref, ref["trace"]["overflow"] == 0)
+
+
+def test_degenerate_and_axis_parallel_rays(dev, oracle_mod):
+    """Zero direction components (1/d = inf), origins on box planes (0*inf = NaN slab terms), -0.0, tmin/tmax."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1], [-1, 0, 0], [0, -1, 0]], float)
+    f = np.array([[0, 1, 2], [0, 2, 3], [0, 3, 1], [1, 4, 2], [0, 5, 6], [4, 3, 1], [2, 4, 3]], np.uint32)
+    cen = np.array([[0.5, 0.5, 0.5], [2, 2, 2], [0, 0, 0]])
+    rad = np.array([0.25, 1.0, 0.5])
+
+    def fill(s):
+        s.add_mesh(v, f)
+        s.add_spheres(cen, rad)
+        s.add_planes(np.array([[0, 0, 0]], float), [2.0], [2.0], np.eye(3).reshape(1, 9))
+        s.add_cylinders(np.array([[0, 0, 1.0]]), [0.5], [0.5], [1.0], np.eye(3).reshape(1, 9))
+
+    p, o = make_pair(oracle_mod, fill)
+    vals = np.array([-1.0, -0.5, -0.0, 0.0, 0.25, 0.5, 1.0, 2.0])
+    r = W.Rng(3)
+    n = 4096
+    org = vals[(r.uniform(3 * n) * len(vals)).astype(int)].reshape(n, 3)
+    dirs = np.array([-1.0, -0.0, 0.0, 1.0, 0.5])[(r.uniform(3 * n) * 5).astype(int)].reshape(n, 3)
+    tmin = np.where(r.uniform(n) < 0.2, 0.3, 0.0)
+    tmax = np.where(r.uniform(n) < 0.2, 1.5, np.finfo(np.float64).max)
+    g = gpu_intersect(p, org, dirs, dev, tmin=tmin, tmax=tmax)
+    ref = o.intersect_batch(org, dirs, tmin=tmin, tmax=tmax)
+    # rays with an all-zero direction normalise to 0 and have 1/d = inf on every axis; they are compared too
+    assert_bit_exact(g, ref)
+
+
+def test_hits_inout_semantics(dev, oracle_mod):
+    """RayHit in/out: a prior hit distance is the merge threshold for non-triangle types; misses keep it."""
+    c, r = W.sphere_field(64, seed=4)
+    p, _ = make_pair(oracle_mod, lambda s: s.add_spheres(c, r))
+    org, dirs = W.camera_rays(64, 64, eye=(0, 0, 30))
+    O = torch.from_numpy(org).to(dev)
+    D = torch.from_numpy(dirs).to(dev)
+    base = p.intersect(O, D)
+    torch.cuda.synchronize()
+    prior = {k: v.clone() for k, v in base.items()}
+    prior["t"].fill_(5.0)
+    prior["prim_id"].fill_(123)
+    prior["geom_id"].fill_(7)
+    out = p.intersect(O, D, hits=prior)
+    torch.cuda.synchronize()
+    t0 = base["t"].cpu().numpy()
+    closer = t0 < 5.0
+    assert np.array_equal(out["t"].cpu().numpy()[closer], t0[closer])
+    assert (out["t"].cpu().numpy()[~closer] == 5.0).all()
+    assert (out["prim_id"].cpu().numpy()[~closer] == 123).all()
+
+
+def test_intersect1_dropin(dev, oracle_mod):
+    from blazert_amd import Ray, RayHit, intersect1
+    sc = W.scene_primitives()
+    p, o = make_pair(oracle_mod, lambda s: (s.add_cylinders(*sc["cylinders"]), s.add_spheres(*sc["spheres"])))
+    for org, d in [((0.0, 5.0, 20.0), (0.0, -0.2, -1.0)), ((2.0, 1.0, 0.0), (-1.0, 0.0, 0.0)),
+                   ((0.0, 5.0, 20.0), (0.3, 0.3, -1.0))]:
+        rh = RayHit()
+        hit = intersect1(p, Ray(org, d), rh)
+        ok, ref, _ = o.intersect1(oracle_mod.make_ray(org, d))
+        assert hit == ok
+        assert rh.prim_id == ref.prim_id and rh.geom_id == ref.geom_id
+        assert np.float64(rh.hit_distance).view(np.uint64) == np.float64(ref.t).view(np.uint64)
+
+
+def test_empty_and_single_prim_scenes(dev, oracle_mod):
+    from blazert_amd import Scene
+    s = Scene()
+    s.commit()
+    org, dirs = W.camera_rays(8, 8, eye=(0, 0, 3))
+    g = gpu_intersect(s, org, dirs, dev)
+    assert (g["geom_id"] == INVALID).all() and (g["t"] == np.finfo(np.float64).max).all()
+    p, o = make_pair(oracle_mod, lambda x: x.add_spheres(np.zeros((1, 3)), [1.0]))
+    assert_bit_exact(gpu_intersect(p, org, dirs, dev), o.intersect_batch(org, dirs))
