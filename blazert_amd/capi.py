@@ -51,9 +51,13 @@ class Hits(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("t", "u", "v", "prim_id", "geom_id", "nx", "ny", "nz")]
 
 
+SCHEDULE_PERSISTENT = 0
+SCHEDULE_PER_THREAD = 1
+
+
 class Launch(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("flags", C.c_uint32), ("block_size", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("schedule", C.c_uint32), ("refill", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class Hit1(C.Structure):

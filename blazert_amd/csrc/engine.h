@@ -6,11 +6,17 @@
 
 namespace brt {
 
+constexpr uint32_t SCHEDULE_PERSISTENT = 0;  // persistent waves with per-lane ray refill (default)
+constexpr uint32_t SCHEDULE_PER_THREAD = 1;  // one ray per thread, grid-stride
+
 struct TraceLaunch {
   uint32_t precision;  // BRT_PRECISION_*
   uint32_t flags;      // BRT_FLAG_*
   uint32_t block;      // threads per block
   bool first;          // first geometry type of the scene: initialises every hit record
+  uint32_t schedule;   // SCHEDULE_*
+  uint32_t refill;     // persistent: refill a wave once this many lanes are idle (1..64)
+  unsigned long long *counter;  // persistent: device ray counter for this launch (zeroed on the stream)
 };
 
 // Per-scene constants for the split-precision path: the largest finite |coordinate| of any node box.

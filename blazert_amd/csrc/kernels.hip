@@ -15,6 +15,8 @@
 #include <float.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "engine.h"
 
 namespace brt {
@@ -487,164 +489,309 @@ __device__ void post_normal(const CylRec *recs, const BvhView &, const Ray &r, c
 }
 
 // ------------------------------------------------------------------------------------------------ traversal
-template <class Rec, bool SPLIT>
-__device__ __forceinline__ bool traverse(const BvhView &bvh, const Ray &r, const Ray32 &q, bool any_hit, Hit &s) {
-  const Rec *recs = reinterpret_cast<const Rec *>(bvh.prims);
-  uint32_t st_ref[STACK_CAP];
-  uint32_t st_cnt[STACK_CAP];
-  double st_key[STACK_CAP];
-  float st_key32[STACK_CAP];
-  int sp = 0;
+// Per-lane traversal state. The DFS of the reference is advanced one unit at a time (one internal node, or one
+// whole leaf), so a persistent wave can retire finished rays and pull new ones between steps.
+struct Trav {
+  uint32_t cur, cnt;  // current node reference and leaf count
+  bool active;        // traversal still running
+  float h32;          // split precision: hit distance rounded up to FP32
+};
 
-  uint32_t cur = bvh.root_ref, cnt = bvh.root_cnt;
-  bool active;
+// Traversal stack: the top RING entries of each lane live in LDS (slot-major, lane-minor, so a wave's accesses
+// spread over all banks); older entries spill to private memory only when a ray goes deeper than RING, and come
+// back when the ring drains. Invariant: entries [lo, sp) are in LDS, [0, lo) in the private spill area.
+constexpr int RING = 8;
+
+template <bool SPLIT>
+struct RingStack {
+  using Key = typename std::conditional<SPLIT, float, double>::type;
+  Key *lkey;  // this lane's slot 0 in LDS; slot j is at [64 * j]
+  uint32_t *lref, *lcnt;
+  Key pkey[STACK_CAP];
+  uint32_t pref[STACK_CAP], pcnt[STACK_CAP];
+  int sp, lo;
+
+  __device__ __forceinline__ void reset() {
+    sp = 0;
+    lo = 0;
+  }
+  __device__ __forceinline__ void push(uint32_t r, uint32_t c, Key k) {
+    if (sp - lo == RING) {
+      const int j = (lo & (RING - 1)) * 64;
+      pkey[lo] = lkey[j];
+      pref[lo] = lref[j];
+      pcnt[lo] = lcnt[j];
+      lo++;
+    }
+    const int j = (sp & (RING - 1)) * 64;
+    lkey[j] = k;
+    lref[j] = r;
+    lcnt[j] = c;
+    sp++;
+  }
+  // requires sp > 0
+  __device__ __forceinline__ void pop(uint32_t &r, uint32_t &c, Key &k) {
+    sp--;
+    if (sp < lo) {
+      lo = sp;
+      r = pref[sp];
+      c = pcnt[sp];
+      k = pkey[sp];
+    } else {
+      const int j = (sp & (RING - 1)) * 64;
+      r = lref[j];
+      c = lcnt[j];
+      k = lkey[j];
+    }
+  }
+};
+
+// LDS bytes of the ring stacks of one block
+template <bool SPLIT>
+constexpr size_t ring_lds_bytes(uint32_t block) {
+  return (size_t)(block / 64) * RING * 64 * (sizeof(typename RingStack<SPLIT>::Key) + 8);
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void ring_bind(RingStack<SPLIT> &st, unsigned char *smem) {
+  using Key = typename RingStack<SPLIT>::Key;
+  const uint32_t waves = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  Key *keys = reinterpret_cast<Key *>(smem);
+  uint32_t *refs = reinterpret_cast<uint32_t *>(keys + (size_t)waves * RING * 64);
+  uint32_t *cnts = refs + (size_t)waves * RING * 64;
+  st.lkey = keys + (size_t)wave * RING * 64 + lane;
+  st.lref = refs + (size_t)wave * RING * 64 + lane;
+  st.lcnt = cnts + (size_t)wave * RING * 64 + lane;
+  st.reset();
+}
+
+// Root test (the reference pushes node 0 and slab-tests it on the first pop with the ray's tmax).
+template <bool SPLIT>
+__device__ __forceinline__ void trav_begin(const BvhView &bvh, const Ray &r, const Ray32 &q, const Hit &s, Trav &t) {
+  t.cur = bvh.root_ref;
+  t.cnt = bvh.root_cnt;
   if (SPLIT) {
     float k32;
     const float rb[6] = {bvh.root_min32[0], bvh.root_min32[1], bvh.root_min32[2],
                          bvh.root_max32[0], bvh.root_max32[1], bvh.root_max32[2]};
-    active = slab32(rb, q, k32) && !(f32_up(s.h) < k32);
+    t.h32 = f32_up(s.h);
+    t.active = slab32(rb, q, k32) && !(t.h32 < k32);
   } else {
     double key;
     const double rb[6] = {bvh.root_min[0], bvh.root_min[1], bvh.root_min[2],
                           bvh.root_max[0], bvh.root_max[1], bvh.root_max[2]};
-    active = slab64(rb, r, key) && !(s.h < key);
+    t.h32 = 0.0f;
+    t.active = slab64(rb, r, key) && !(s.h < key);
   }
-  float h32 = SPLIT ? f32_up(s.h) : 0.0f;
-  bool done = false;
-
-  while (active) {
-    if (!(cur & LEAF_BIT)) {
-      bool near_ok, far_ok;
-      uint32_t nref, ncnt, fref, fcnt;
-      double fkey = 0.0;
-      float fkey32 = 0.0f;
-      if (SPLIT) {
-        const Node32 nd = bvh.nodes32[cur];
-        const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
-        const uint32_t nc = (q.sign >> axis) & 1u;
-        float k0, k1;
-        const bool p0 = slab32(nd.b, q, k0);
-        const bool p1 = slab32(nd.b + 6, q, k1);
-        const bool pn = nc ? p1 : p0, pf = nc ? p0 : p1;
-        const float kn = nc ? k1 : k0, kf = nc ? k0 : k1;
-        near_ok = pn && !(h32 < kn);
-        far_ok = pf;
-        fkey32 = kf;
-        nref = nd.ref[nc];
-        ncnt = nd.cnt[nc] & CNT_MASK;
-        fref = nd.ref[1 - nc];
-        fcnt = nd.cnt[1 - nc] & CNT_MASK;
-      } else {
-        const Node64 nd = bvh.nodes64[cur];
-        const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
-        const uint32_t nc = (r.sign >> axis) & 1u;
-        double k0 = 0.0, k1 = 0.0;
-        const bool p0 = slab64(nd.b, r, k0);
-        const bool p1 = slab64(nd.b + 6, r, k1);
-        const bool pn = nc ? p1 : p0, pf = nc ? p0 : p1;
-        const double kn = nc ? k1 : k0, kf = nc ? k0 : k1;
-        near_ok = pn && !(s.h < kn);
-        far_ok = pf;
-        fkey = kf;
-        nref = nd.ref[nc];
-        ncnt = nd.cnt[nc] & CNT_MASK;
-        fref = nd.ref[1 - nc];
-        fcnt = nd.cnt[1 - nc] & CNT_MASK;
-      }
-      if (far_ok) {
-        st_ref[sp] = fref;
-        st_cnt[sp] = fcnt;
-        if (SPLIT) st_key32[sp] = fkey32;
-        else st_key[sp] = fkey;
-        sp++;
-      }
-      if (near_ok) {
-        cur = nref;
-        cnt = ncnt;
-        continue;
-      }
-    } else {
-      const uint32_t first = cur & ~LEAF_BIT;
-      bool leaf_hit = false;
-      for (uint32_t p = 0; p < cnt; p++) {
-        const Rec rec = recs[first + p];
-        leaf_hit |= isect(rec, r, s, first + p);
-        if (leaf_hit && any_hit) break;
-      }
-      if (leaf_hit) {
-        if (SPLIT) h32 = f32_up(s.h);
-        if (any_hit) {
-          done = true;
-          break;
-        }
-      }
-    }
-    // pop the next far child whose entry distance does not exceed the current hit distance
-    active = false;
-    while (sp > 0) {
-      sp--;
-      const bool ok = SPLIT ? !(h32 < st_key32[sp]) : !(s.h < st_key[sp]);
-      if (ok) {
-        cur = st_ref[sp];
-        cnt = st_cnt[sp];
-        active = true;
-        break;
-      }
-    }
-  }
-  (void)done;
-  return s.h < r.tmax;
 }
 
+// One unit of the DFS: an internal node (test both children, push far, descend near) or a leaf (test its
+// primitives in order, blazert/bvh/intersect.h:66-78), followed by pops until a node passes or the stack is empty.
+template <class Rec, bool SPLIT>
+__device__ __forceinline__ void trav_step(const BvhView &bvh, const Ray &r, const Ray32 &q, bool any_hit, Hit &s,
+                                          Trav &t, RingStack<SPLIT> &st) {
+  if (!(t.cur & LEAF_BIT)) {
+    bool near_ok, far_ok;
+    uint32_t nref, ncnt, fref, fcnt;
+    typename std::conditional<SPLIT, float, double>::type fkey;
+    if constexpr (SPLIT) {
+      const Node32 nd = bvh.nodes32[t.cur];
+      const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
+      const uint32_t nc = (q.sign >> axis) & 1u;
+      float k0, k1;
+      const bool p0 = slab32(nd.b, q, k0);
+      const bool p1 = slab32(nd.b + 6, q, k1);
+      near_ok = (nc ? p1 : p0) && !(t.h32 < (nc ? k1 : k0));
+      far_ok = nc ? p0 : p1;
+      fkey = nc ? k0 : k1;
+      nref = nd.ref[nc];
+      ncnt = nd.cnt[nc] & CNT_MASK;
+      fref = nd.ref[1 - nc];
+      fcnt = nd.cnt[1 - nc] & CNT_MASK;
+    } else {
+      const Node64 nd = bvh.nodes64[t.cur];
+      const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
+      const uint32_t nc = (r.sign >> axis) & 1u;
+      double k0 = 0.0, k1 = 0.0;
+      const bool p0 = slab64(nd.b, r, k0);
+      const bool p1 = slab64(nd.b + 6, r, k1);
+      near_ok = (nc ? p1 : p0) && !(s.h < (nc ? k1 : k0));
+      far_ok = nc ? p0 : p1;
+      fkey = nc ? k0 : k1;
+      nref = nd.ref[nc];
+      ncnt = nd.cnt[nc] & CNT_MASK;
+      fref = nd.ref[1 - nc];
+      fcnt = nd.cnt[1 - nc] & CNT_MASK;
+    }
+    if (far_ok) st.push(fref, fcnt, fkey);
+    if (near_ok) {
+      t.cur = nref;
+      t.cnt = ncnt;
+      return;
+    }
+  } else {
+    const Rec *recs = reinterpret_cast<const Rec *>(bvh.prims);
+    const uint32_t first = t.cur & ~LEAF_BIT;
+    bool leaf_hit = false;
+    for (uint32_t p = 0; p < t.cnt; p++) {
+      const Rec rec = recs[first + p];
+      leaf_hit |= isect(rec, r, s, first + p);
+      if (leaf_hit && any_hit) break;
+    }
+    if (leaf_hit) {
+      if (SPLIT) t.h32 = f32_up(s.h);
+      if (any_hit) {
+        t.active = false;
+        return;
+      }
+    }
+  }
+  // pop the next far child whose entry distance does not exceed the current hit distance
+  while (st.sp > 0) {
+    uint32_t ref, cnt;
+    typename RingStack<SPLIT>::Key key;
+    st.pop(ref, cnt, key);
+    const bool ok = SPLIT ? !(t.h32 < (float)key) : !(s.h < (double)key);
+    if (ok) {
+      t.cur = ref;
+      t.cnt = cnt;
+      return;
+    }
+  }
+  t.active = false;
+}
+
+// Merge into the hit record (blazert/scene.h:169-209): triangles replace unconditionally, later types only if
+// strictly closer; the first type of the scene also writes the default RayHit for misses.
+template <class Rec>
+__device__ __forceinline__ void finish_ray(const BvhView &bvh, const HitsView &hits, uint64_t i, const Ray &r,
+                                           const Hit &s, uint32_t flags, uint32_t first, uint32_t is_tri) {
+  const bool hit = s.h < r.tmax;
+  bool write = false;
+  if (hit) {
+    if (is_tri) write = true;
+    else write = s.h < ((first && !(flags & FLAG_INOUT)) ? DBL_MAX : hits.t[i]);
+  }
+  if (write) {
+    const uint32_t pid = bvh.prim_ids[s.leaf_idx];
+    if (hits.merged) hits.merged[i] = 1u;
+    hits.t[i] = s.h;
+    if (hits.u) hits.u[i] = s.u;
+    if (hits.v) hits.v[i] = s.v;
+    hits.prim_id[i] = pid;
+    hits.geom_id[i] = bvh.geom_id;
+    if (hits.nx) {
+      double nrm[3];
+      post_normal(reinterpret_cast<const Rec *>(bvh.prims), bvh, r, s, pid, nrm);
+      hits.nx[i] = nrm[0];
+      hits.ny[i] = nrm[1];
+      hits.nz[i] = nrm[2];
+    }
+  } else if (first && !(flags & FLAG_INOUT)) {
+    hits.t[i] = DBL_MAX;
+    if (hits.u) hits.u[i] = 0.0;
+    if (hits.v) hits.v[i] = 0.0;
+    hits.prim_id[i] = INVALID_ID;
+    hits.geom_id[i] = INVALID_ID;
+    if (hits.nx) {
+      hits.nx[i] = 0.0;
+      hits.ny[i] = 0.0;
+      hits.nz[i] = 0.0;
+    }
+  }
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void start_ray(const RaysView &rays, uint64_t i, double S, Ray &r, Ray32 &q, Hit &s) {
+  make_ray(rays, i, r);
+  if (SPLIT) make_ray32(r, S, q);
+  s.h = r.tmax;  // prepare_traversal
+  s.u = 0.0;
+  s.v = 0.0;
+  s.leaf_idx = INVALID_ID;
+}
+
+// One ray per thread (grid-stride).
 template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysView rays, HitsView hits, uint64_t n, uint32_t flags,
                                                uint32_t first, uint32_t is_tri, double S) {
+  extern __shared__ unsigned char smem[];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
+  RingStack<SPLIT> st;
+  ring_bind<SPLIT>(st, smem);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     Ray r;
-    make_ray(rays, i, r);
     Ray32 q;
-    if (SPLIT) make_ray32(r, S, q);
     Hit s;
-    s.h = r.tmax;
-    s.u = 0.0;
-    s.v = 0.0;
-    s.leaf_idx = INVALID_ID;
-    const bool hit = traverse<Rec, SPLIT>(bvh, r, q, (flags & FLAG_ANY_HIT) != 0, s);
-    // merge (blazert/scene.h:169-209): triangles replace unconditionally, later types only if strictly closer
-    bool write = false;
-    if (hit) {
-      if (is_tri) write = true;
-      else write = s.h < ((first && !(flags & FLAG_INOUT)) ? DBL_MAX : hits.t[i]);
-    }
-    if (write) {
-      const uint32_t pid = bvh.prim_ids[s.leaf_idx];
-      hits.t[i] = s.h;
-      if (hits.u) hits.u[i] = s.u;
-      if (hits.v) hits.v[i] = s.v;
-      hits.prim_id[i] = pid;
-      hits.geom_id[i] = bvh.geom_id;
-      if (hits.nx) {
-        double nrm[3];
-        post_normal(reinterpret_cast<const Rec *>(bvh.prims), bvh, r, s, pid, nrm);
-        hits.nx[i] = nrm[0];
-        hits.ny[i] = nrm[1];
-        hits.nz[i] = nrm[2];
+    start_ray<SPLIT>(rays, i, S, r, q, s);
+    Trav t;
+    trav_begin<SPLIT>(bvh, r, q, s, t);
+    st.reset();
+    while (t.active) trav_step<Rec, SPLIT>(bvh, r, q, any_hit, s, t, st);
+    finish_ray<Rec>(bvh, hits, i, r, s, flags, first, is_tri);
+  }
+}
+
+// Persistent waves: every lane owns one ray at a time; when at least `refill` lanes of a wave are idle, one lane
+// claims that many ray indices from the launch's counter with a single atomic and the idle lanes start them.
+// Skewed per-ray work (many rays miss the root box, a few visit hundreds of nodes) then no longer idles the
+// wave until its slowest ray ends. Every lane leaves the loop once the counter has passed n and its own ray is
+// done, so the grid always drains.
+template <class Rec, bool SPLIT>
+__global__ void __launch_bounds__(256) k_trace_persistent(BvhView bvh, RaysView rays, HitsView hits, uint64_t n,
+                                                          uint32_t flags, uint32_t first, uint32_t is_tri, double S,
+                                                          unsigned long long *counter, uint32_t refill) {
+  extern __shared__ unsigned char smem[];
+  const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
+  const uint32_t lane = __lane_id();
+  RingStack<SPLIT> st;
+  ring_bind<SPLIT>(st, smem);
+  Ray r;
+  Ray32 q;
+  Hit s;
+  Trav t;
+  t.active = false;
+  bool has_ray = false;
+  bool exhausted = false;  // wave-uniform
+  uint64_t idx = 0;
+  for (;;) {
+    if (!exhausted) {
+      const unsigned long long idle = __ballot(!has_ray);
+      const uint32_t nidle = (uint32_t)__popcll(idle);
+      if (nidle >= refill || nidle == 64u) {
+        unsigned long long base = 0;
+        const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
+        if (lane == leader) base = atomicAdd(counter, (unsigned long long)nidle);
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)base, (int)leader);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(base >> 32), (int)leader);
+        base = ((unsigned long long)hi << 32) | lo;
+        if (base + nidle >= n) exhausted = true;
+        if (!has_ray) {
+          const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+          const uint64_t cand = base + rank;
+          if (cand < n) {
+            idx = cand;
+            has_ray = true;
+            start_ray<SPLIT>(rays, idx, S, r, q, s);
+            trav_begin<SPLIT>(bvh, r, q, s, t);
+            st.reset();
+          }
+        }
       }
-    } else if (first && !(flags & FLAG_INOUT)) {
-      hits.t[i] = DBL_MAX;
-      if (hits.u) hits.u[i] = 0.0;
-      if (hits.v) hits.v[i] = 0.0;
-      hits.prim_id[i] = INVALID_ID;
-      hits.geom_id[i] = INVALID_ID;
-      if (hits.nx) {
-        hits.nx[i] = 0.0;
-        hits.ny[i] = 0.0;
-        hits.nz[i] = 0.0;
+    }
+    if (!__ballot(has_ray)) break;
+    if (has_ray) {
+      if (t.active) trav_step<Rec, SPLIT>(bvh, r, q, any_hit, s, t, st);
+      if (!t.active) {
+        finish_ray<Rec>(bvh, hits, idx, r, s, flags, first, is_tri);
+        has_ray = false;
       }
     }
   }
 }
+
+__global__ void k_zero_counter(unsigned long long *counter) { *counter = 0ull; }
 
 __global__ void k_init_hits(HitsView hits, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -662,20 +809,44 @@ __global__ void k_init_hits(HitsView hits, uint64_t n) {
   }
 }
 
-template <class Rec>
-hipError_t launch_typed(const BvhView &bvh, const RaysView &rays, const HitsView &hits, uint64_t n,
-                        const TraceLaunch &cfg, const SplitParams &sp, uint32_t is_tri, hipStream_t stream) {
+template <class Rec, bool SPLIT>
+hipError_t launch_kernel(const BvhView &bvh, const RaysView &rays, const HitsView &hits, uint64_t n,
+                         const TraceLaunch &cfg, const SplitParams &sp, uint32_t is_tri, hipStream_t stream) {
   const uint32_t block = cfg.block ? cfg.block : 256;
+  const uint32_t first = cfg.first ? 1u : 0u;
+  const size_t lds = ring_lds_bytes<SPLIT>(block);
+  if (cfg.schedule == SCHEDULE_PERSISTENT) {
+    if (!cfg.counter) return hipErrorInvalidValue;
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_trace_persistent<Rec, SPLIT>,
+                                                       (int)block, lds);
+    if (e != hipSuccess) return e;
+    uint64_t blocks = (uint64_t)cus * (uint64_t)(per_cu > 0 ? per_cu : 1);
+    const uint64_t need = (n + block - 1) / block;
+    if (blocks > need) blocks = need;
+    if (blocks == 0) return hipSuccess;
+    const uint32_t refill = cfg.refill ? cfg.refill : 16u;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
+    hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
+                       hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill);
+    return hipGetLastError();
+  }
   uint64_t blocks = (n + block - 1) / block;
   if (blocks > (1ull << 30)) blocks = 1ull << 30;
   if (blocks == 0) return hipSuccess;
-  if (cfg.precision == PREC_FP64)
-    hipLaunchKernelGGL((k_trace<Rec, false>), dim3((uint32_t)blocks), dim3(block), 0, stream, bvh, rays, hits, n,
-                       cfg.flags, cfg.first ? 1u : 0u, is_tri, sp.scene_extent);
-  else
-    hipLaunchKernelGGL((k_trace<Rec, true>), dim3((uint32_t)blocks), dim3(block), 0, stream, bvh, rays, hits, n,
-                       cfg.flags, cfg.first ? 1u : 0u, is_tri, sp.scene_extent);
+  hipLaunchKernelGGL((k_trace<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays, hits, n,
+                     cfg.flags, first, is_tri, sp.scene_extent);
   return hipGetLastError();
+}
+
+template <class Rec>
+hipError_t launch_typed(const BvhView &bvh, const RaysView &rays, const HitsView &hits, uint64_t n,
+                        const TraceLaunch &cfg, const SplitParams &sp, uint32_t is_tri, hipStream_t stream) {
+  if (cfg.precision == PREC_FP64) return launch_kernel<Rec, false>(bvh, rays, hits, n, cfg, sp, is_tri, stream);
+  return launch_kernel<Rec, true>(bvh, rays, hits, n, cfg, sp, is_tri, stream);
 }
 
 }  // namespace

@@ -92,6 +92,7 @@ struct HitsView {
   double *t, *u, *v;
   uint32_t *prim_id, *geom_id;
   double *nx, *ny, *nz;
+  uint32_t *merged;  // optional: set to 1 where a geometry type's hit was merged (intersect1's return value)
 };
 
 }  // namespace brt

@@ -563,10 +563,14 @@ void build_geometry(Geometry &g, GeomType type, const brt_build_options &o) {
 }
 
 // ------------------------------------------------------------------------------------------------ device
+constexpr uint32_t NUM_COUNTERS = 256;
+
 struct DeviceScene {
   int device = -1;
   std::vector<void *> allocs;
   BvhView views[NUM_TYPES]{};
+  unsigned long long *counters = nullptr;  // ray counters of the persistent schedule, one per launch in flight
+  std::atomic<uint32_t> next_counter{0};
   ~DeviceScene() {
     for (void *p : allocs) (void)hipFree(p);
   }
@@ -625,6 +629,13 @@ int upload(brt_scene *s) {
   }
   auto d = std::make_unique<DeviceScene>();
   d->device = dev;
+  {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, NUM_COUNTERS * sizeof(unsigned long long));
+    if (e != hipSuccess) return fail(BRT_ERR_OUT_OF_MEMORY, std::string("counter allocation: ") + hipGetErrorString(e));
+    d->allocs.push_back(p);
+    d->counters = static_cast<unsigned long long *>(p);
+  }
   for (uint32_t t = 0; t < NUM_TYPES; t++) {
     Geometry &g = s->geo[t];
     if (!g.present || g.boxes.n == 0) continue;
@@ -796,15 +807,19 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
     return fail(BRT_ERR_INVALID_ARGUMENT, "hit t / prim_id / geom_id arrays are required");
   if ((hits->nx != nullptr) != (hits->ny != nullptr) || (hits->nx != nullptr) != (hits->nz != nullptr))
     return fail(BRT_ERR_INVALID_ARGUMENT, "normal arrays must be all set or all NULL");
-  brt_launch L{BRT_PRECISION_FP64, 0, 0, 0};
+  brt_launch L{};
+  L.precision = BRT_PRECISION_FP64;
+  L.schedule = BRT_SCHEDULE_PERSISTENT;
   if (launch) L = *launch;
   if (L.precision > BRT_PRECISION_FP32_TRAV_FP64_HIT) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (L.schedule > BRT_SCHEDULE_PER_THREAD) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown schedule");
+  if (L.refill > 64) return fail(BRT_ERR_INVALID_ARGUMENT, "refill must be 0..64");
   if (L.block_size && (L.block_size % 64 || L.block_size > 1024))
     return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 up to 1024");
   int rc = upload(s);
   if (rc) return rc;
   RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
-  HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz};
+  HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz, nullptr};
   hipStream_t st = static_cast<hipStream_t>(stream);
   bool first = true;
   for (uint32_t t = 0; t < NUM_TYPES; t++) {
@@ -812,7 +827,9 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
     if (!g.present || g.boxes.n == 0) continue;
     if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT && !g.extent_ok)
       return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
-    TraceLaunch cfg{L.precision, L.flags, L.block_size, first};
+    TraceLaunch cfg{L.precision, L.flags, L.block_size, first, L.schedule, L.refill, nullptr};
+    if (L.schedule == BRT_SCHEDULE_PERSISTENT)
+      cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
     SplitParams sp{g.extent};
     hipError_t e = launch_trace((GeomType)t, s->dev->views[t], rv, hv, n, cfg, sp, st);
     if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
@@ -828,42 +845,44 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
 int brt_intersect1(const brt_scene *s, const double org[3], const double dir[3], double tmin, double tmax,
                    uint32_t flags, brt_hit1 *hit, int *hit_out) {
   if (!s || !org || !dir || !hit) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
-  // one ray through the batched path, keeping RayHit's in/out semantics
-  double host[8 + 8] = {org[0], org[1], org[2], dir[0], dir[1], dir[2], tmin, tmax};
-  double *d = nullptr;
-  hipError_t e = hipMalloc(reinterpret_cast<void **>(&d), 16 * sizeof(double) + 2 * sizeof(uint32_t));
+  // one ray through the batched kernels, keeping RayHit's in/out semantics: 8 ray doubles, 6 hit doubles,
+  // then prim id, geom id and the merged flag
+  double host[14] = {org[0], org[1], org[2], dir[0], dir[1], dir[2], tmin, tmax,
+                     hit->t, hit->u, hit->v, hit->normal[0], hit->normal[1], hit->normal[2]};
+  uint32_t ids[3] = {hit->prim_id, hit->geom_id, 0u};
+  char *d = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&d), sizeof host + sizeof ids);
   if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
-  uint32_t *ids = reinterpret_cast<uint32_t *>(d + 16);
-  host[8] = hit->t;
-  host[9] = hit->u;
-  host[10] = hit->v;
-  host[11] = hit->normal[0];
-  host[12] = hit->normal[1];
-  host[13] = hit->normal[2];
-  const uint32_t hid[2] = {hit->prim_id, hit->geom_id};
+  double *dd = reinterpret_cast<double *>(d);
+  uint32_t *di = reinterpret_cast<uint32_t *>(d + sizeof host);
   e = hipMemcpy(d, host, sizeof host, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(ids, hid, sizeof hid, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(di, ids, sizeof ids, hipMemcpyHostToDevice);
   int rc = BRT_OK;
-  if (e == hipSuccess) {
-    brt_rays r{d, d + 1, d + 2, d + 3, d + 4, d + 5, d + 6, d + 7};
-    brt_hits h{d + 8, d + 9, d + 10, ids, ids + 1, d + 11, d + 12, d + 13};
-    brt_launch L{BRT_PRECISION_FP64, flags | BRT_FLAG_HITS_INOUT, 64, 0};
-    rc = brt_intersect(s, &r, &h, 1, &L, nullptr);
-    if (rc == BRT_OK) e = hipDeviceSynchronize();
-    if (rc == BRT_OK && e == hipSuccess) e = hipMemcpy(host, d, sizeof host, hipMemcpyDeviceToHost);
-    uint32_t oid[2] = {0, 0};
-    if (rc == BRT_OK && e == hipSuccess) e = hipMemcpy(oid, ids, sizeof oid, hipMemcpyDeviceToHost);
-    if (rc == BRT_OK && e == hipSuccess) {
-      const bool changed = oid[1] != hid[1] || oid[0] != hid[0] || host[8] != hit->t;
+  brt_scene *ms = const_cast<brt_scene *>(s);
+  if (e == hipSuccess && !ms->committed) rc = fail(BRT_ERR_STATE, "scene not committed");
+  if (e == hipSuccess && rc == BRT_OK) rc = upload(ms);
+  if (e == hipSuccess && rc == BRT_OK) {
+    RaysView rv{dd, dd + 1, dd + 2, dd + 3, dd + 4, dd + 5, dd + 6, dd + 7};
+    HitsView hv{dd + 8, dd + 9, dd + 10, di, di + 1, dd + 11, dd + 12, dd + 13, di + 2};
+    for (uint32_t t = 0; t < NUM_TYPES && e == hipSuccess; t++) {
+      const Geometry &g = ms->geo[t];
+      if (!g.present || g.boxes.n == 0) continue;
+      TraceLaunch cfg{BRT_PRECISION_FP64, flags | BRT_FLAG_HITS_INOUT, 64, false, SCHEDULE_PER_THREAD, 0, nullptr};
+      e = launch_trace((GeomType)t, ms->dev->views[t], rv, hv, 1, cfg, SplitParams{g.extent}, nullptr);
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(host, d, sizeof host, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ids, di, sizeof ids, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
       hit->t = host[8];
       hit->u = host[9];
       hit->v = host[10];
       hit->normal[0] = host[11];
       hit->normal[1] = host[12];
       hit->normal[2] = host[13];
-      hit->prim_id = oid[0];
-      hit->geom_id = oid[1];
-      if (hit_out) *hit_out = changed ? 1 : 0;
+      hit->prim_id = ids[0];
+      hit->geom_id = ids[1];
+      if (hit_out) *hit_out = ids[2] ? 1 : 0;
     }
   }
   (void)hipFree(d);

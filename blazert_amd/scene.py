@@ -43,9 +43,13 @@ class Scene:
         self.has_been_committed = False
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().brt_scene_destroy(self._h)
-            self._h = None
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h:
+            try:
+                lib().brt_scene_destroy(h)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
 
     def __del__(self):
         self.close()
@@ -119,7 +123,7 @@ class Scene:
     # -------------------------------------------------------------------------------------------- batched
     def intersect(self, org, dirs, tmin=None, tmax=None, *, precision: int = capi.PRECISION_FP64,
                   any_hit: bool = False, normals: bool = False, hits: dict | None = None, block_size: int = 0,
-                  stream=None) -> dict:
+                  schedule: int = capi.SCHEDULE_PERSISTENT, refill: int = 0, stream=None) -> dict:
         """Batched intersect1 over n rays. org / dirs: (n,3) float64 CUDA tensors or tuples of three columns.
 
         Returns a dict of CUDA tensors t, u, v (float64), prim_id, geom_id (int32 holding uint32 bits) and,
@@ -149,7 +153,7 @@ class Scene:
         H = capi.Hits(hits["t"].data_ptr(), hits["u"].data_ptr(), hits["v"].data_ptr(), hits["prim_id"].data_ptr(),
                       hits["geom_id"].data_ptr(), *([nrm[k].data_ptr() for k in range(3)] if nrm is not None
                                                     else [None, None, None]))
-        L = capi.Launch(precision, flags, block_size, 0)
+        L = capi.Launch(precision, flags, block_size, schedule, refill)
         st = stream if stream is not None else torch.cuda.current_stream(dev)
         keep = (cols_o, cols_d, tmn, tmx)  # noqa: F841 (inputs alive until the launch is enqueued)
         check(lib().brt_intersect(self._h, C.byref(rays), C.byref(H), n, C.byref(L), C.c_void_p(st.cuda_stream)))

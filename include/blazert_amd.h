@@ -95,11 +95,17 @@ typedef struct brt_hits {
   double *nx, *ny, *nz;
 } brt_hits;
 
+/* work distribution of a launch */
+#define BRT_SCHEDULE_PERSISTENT 0u /* persistent waves; idle lanes pull new rays from a device counter */
+#define BRT_SCHEDULE_PER_THREAD 1u /* one ray per thread */
+
 typedef struct brt_launch {
   uint32_t precision;  /* BRT_PRECISION_* */
   uint32_t flags;      /* BRT_FLAG_* */
-  uint32_t block_size; /* 0 = library default */
-  uint32_t reserved;
+  uint32_t block_size; /* threads per block, multiple of 64 up to 1024; 0 = library default (256) */
+  uint32_t schedule;   /* BRT_SCHEDULE_* */
+  uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default (16) */
+  uint32_t reserved[3];
 } brt_launch;
 
 /* One ray / hit in host memory, for the single-ray drop-in (RayHit<T>, blazert/ray.h:72-85). */
@@ -151,7 +157,9 @@ int brt_scene_export_tree(const brt_scene *scene, uint32_t geom_type, brt_tree_n
                           uint32_t *prim_order, uint32_t *num_nodes);
 
 /* Batched intersect1 (blazert/scene.h:161-211) for n rays resident on the device, enqueued on `stream`
- * (a hipStream_t, NULL = default stream). launch may be NULL (FP64, no flags). Asynchronous. */
+ * (a hipStream_t, NULL = default stream). launch may be NULL (FP64, no flags, persistent). Asynchronous.
+ * The persistent schedule uses one of 256 per-scene device counters per launch, taken round robin, so more than
+ * 256 launches of one scene in flight at once on different streams would share a counter. */
 int brt_intersect(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,
                   const brt_launch *launch, void *stream);
 

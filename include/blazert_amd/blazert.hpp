@@ -1,0 +1,241 @@
+// blazert_amd/blazert.hpp -- C++17 drop-in for blazeRT's scene API on the MI355X engine.
+//
+// Mirrors the reference's public types so code written against blazert::Scene<T> compiles unchanged for
+// T = double (blazert/blazert.h:15-21, blazert/scene.h:38-116, blazert/ray.h:20-85):
+//   blazert::Vec3r<T>, Vec2r<T>, Vec3ui, Mat3r<T>, Vec3rList<T>, Vec3iList, Mat3rList<T>  (blazert/datatypes.h:31-59)
+//   blazert::Ray<T>{origin, direction, min_hit_distance, max_hit_distance, cull_back_face, any_hit}
+//   blazert::RayHit<T>{normal, uv, hit_distance, prim_id, geom_id}
+//   blazert::Scene<T>::add_mesh / add_spheres / add_planes / add_cylinders / commit, build_options
+//   bool blazert::intersect1(const Scene<T>&, const Ray<T>&, RayHit<T>&)
+// The reference lets users substitute their own vector types (blazert/datatypes.h:28); these are small
+// aggregate types with operator[] (and operator() for matrices), which is all the scene API needs.
+// Geometry is copied into the engine at add time. intersect1 is one ray per call (a GPU round trip); batches
+// of device-resident rays go through brt_intersect() in blazert_amd.h.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../blazert_amd.h"
+
+namespace blazert {
+
+template <typename T>
+struct Vec3r {
+  std::array<T, 3> v{};
+  Vec3r() = default;
+  Vec3r(T x, T y, T z) : v{x, y, z} {}
+  explicit Vec3r(T s) : v{s, s, s} {}
+  T &operator[](size_t i) { return v[i]; }
+  const T &operator[](size_t i) const { return v[i]; }
+  static constexpr size_t size() { return 3; }
+};
+
+template <typename T>
+struct Vec2r {
+  std::array<T, 2> v{};
+  Vec2r() = default;
+  Vec2r(T x, T y) : v{x, y} {}
+  T &operator[](size_t i) { return v[i]; }
+  const T &operator[](size_t i) const { return v[i]; }
+};
+
+struct Vec3ui {
+  std::array<unsigned int, 3> v{};
+  Vec3ui() = default;
+  Vec3ui(unsigned int a, unsigned int b, unsigned int c) : v{a, b, c} {}
+  unsigned int &operator[](size_t i) { return v[i]; }
+  const unsigned int &operator[](size_t i) const { return v[i]; }
+};
+
+// Row-major 3x3 (blaze::rowMajor in the reference), constructible from nested initializer lists.
+template <typename T>
+struct Mat3r {
+  std::array<T, 9> m{};
+  Mat3r() = default;
+  Mat3r(std::initializer_list<std::initializer_list<T>> rows) {
+    size_t i = 0;
+    for (const auto &r : rows) {
+      size_t j = 0;
+      for (T x : r) m[3 * i + j++] = x;
+      ++i;
+    }
+  }
+  static Mat3r identity() { return Mat3r{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}; }
+  T &operator()(size_t i, size_t j) { return m[3 * i + j]; }
+  const T &operator()(size_t i, size_t j) const { return m[3 * i + j]; }
+};
+
+template <typename T>
+using Vec3rList = std::vector<Vec3r<T>>;
+using Vec3iList = std::vector<Vec3ui>;
+template <typename T>
+using Mat3rList = std::vector<Mat3r<T>>;
+
+// blazert/bvh/options.h:14-28
+template <typename T>
+struct BVHBuildOptions {
+  unsigned int min_leaf_primitives = 4;
+  unsigned int max_tree_depth = 28;
+  unsigned int bin_size = 64;
+};
+
+// blazert/ray.h:20-66: direction is normalised by the engine; 1/direction and the sign use the raw argument.
+template <typename T>
+class Ray {
+ public:
+  enum class CullBackFace { yes, no };
+  enum class AnyHit { yes, no };
+  const Vec3r<T> origin;
+  const Vec3r<T> direction;  // as given (the engine normalises it exactly like the reference constructor)
+  T min_hit_distance;
+  T max_hit_distance;
+  CullBackFace cull_back_face;
+  AnyHit any_hit;
+  Ray() = delete;
+  Ray(const Vec3r<T> &origin, const Vec3r<T> &direction, T min_hit_distance = T(0.),
+      T max_hit_distance = std::numeric_limits<T>::max(), CullBackFace cull_back_face = CullBackFace::no,
+      AnyHit any_hit = AnyHit::no)
+      : origin(origin), direction(direction), min_hit_distance(min_hit_distance), max_hit_distance(max_hit_distance),
+        cull_back_face(cull_back_face), any_hit(any_hit) {}
+};
+
+// blazert/ray.h:72-85
+template <typename T>
+struct RayHit {
+  Vec3r<T> normal;
+  Vec2r<T> uv;
+  T hit_distance = std::numeric_limits<T>::max();
+  unsigned int prim_id = static_cast<unsigned int>(-1);
+  unsigned int geom_id = static_cast<unsigned int>(-1);
+};
+
+template <typename T>
+class BlazertScene;
+
+template <typename T>
+bool intersect1(const BlazertScene<T> &scene, const Ray<T> &ray, RayHit<T> &rayhit);
+
+// blazert/scene.h:38-116 for T = double. add_* return the geometry id, or unsigned(-1) for a second add of a
+// geometry type or any add after commit (blazert/scene.h:230-239).
+template <typename T>
+class BlazertScene {
+  static_assert(std::is_same<T, double>::value, "the MI355X engine implements Scene<double>");
+
+ public:
+  BVHBuildOptions<T> build_options;
+  bool has_been_committed = false;
+
+  BlazertScene() = default;
+
+  unsigned int add_mesh(const Vec3rList<T> &vertices, const Vec3iList &triangles) {
+    std::vector<double> v(3 * vertices.size());
+    std::vector<uint32_t> f(3 * triangles.size());
+    for (size_t i = 0; i < vertices.size(); i++)
+      for (int k = 0; k < 3; k++) v[3 * i + k] = vertices[i][k];
+    for (size_t i = 0; i < triangles.size(); i++)
+      for (int k = 0; k < 3; k++) f[3 * i + k] = triangles[i][k];
+    uint32_t gid = BRT_INVALID_ID;
+    brt_scene_add_mesh(ensure(), v.data(), (uint32_t)vertices.size(), f.data(), (uint32_t)triangles.size(), &gid);
+    return gid;
+  }
+
+  unsigned int add_spheres(const Vec3rList<T> &centers, const std::vector<T> &radii) {
+    const std::vector<double> c = flat(centers);
+    uint32_t gid = BRT_INVALID_ID;
+    brt_scene_add_spheres(ensure(), c.data(), radii.data(), (uint32_t)radii.size(), &gid);
+    return gid;
+  }
+
+  unsigned int add_planes(const Vec3rList<T> &centers, const std::vector<T> &dxs, const std::vector<T> &dys,
+                          const Mat3rList<T> &rotations) {
+    const std::vector<double> c = flat(centers), r = flat(rotations);
+    uint32_t gid = BRT_INVALID_ID;
+    brt_scene_add_planes(ensure(), c.data(), dxs.data(), dys.data(), r.data(), (uint32_t)dxs.size(), &gid);
+    return gid;
+  }
+
+  unsigned int add_cylinders(const Vec3rList<T> &centers, const std::vector<T> &semi_axes_a,
+                             const std::vector<T> &semi_axes_b, const std::vector<T> &heights,
+                             const Mat3rList<T> &rotations) {
+    const std::vector<double> c = flat(centers), r = flat(rotations);
+    uint32_t gid = BRT_INVALID_ID;
+    brt_scene_add_cylinders(ensure(), c.data(), semi_axes_a.data(), semi_axes_b.data(), heights.data(), r.data(),
+                            (uint32_t)semi_axes_a.size(), &gid);
+    return gid;
+  }
+
+  // blazert/scene.h:91-115 (build_options are read when the first geometry is added)
+  bool commit() {
+    if (brt_scene_commit(ensure()) != BRT_OK) throw std::runtime_error(brt_last_error());
+    has_been_committed = true;
+    return has_been_committed;
+  }
+
+  brt_scene *native() const { return handle_.get(); }
+
+ private:
+  struct Deleter {
+    void operator()(brt_scene *s) const { brt_scene_destroy(s); }
+  };
+  std::unique_ptr<brt_scene, Deleter> handle_;
+
+  brt_scene *ensure() {
+    if (!handle_) {
+      const brt_build_options o{build_options.min_leaf_primitives, build_options.max_tree_depth,
+                                build_options.bin_size};
+      brt_scene *s = nullptr;
+      if (brt_scene_create(&o, &s) != BRT_OK) throw std::runtime_error(brt_last_error());
+      handle_.reset(s);
+    }
+    return handle_.get();
+  }
+
+  static std::vector<double> flat(const Vec3rList<T> &a) {
+    std::vector<double> o(3 * a.size());
+    for (size_t i = 0; i < a.size(); i++)
+      for (int k = 0; k < 3; k++) o[3 * i + k] = a[i][k];
+    return o;
+  }
+  static std::vector<double> flat(const Mat3rList<T> &a) {
+    std::vector<double> o(9 * a.size());
+    for (size_t i = 0; i < a.size(); i++)
+      for (int k = 0; k < 9; k++) o[9 * i + k] = a[i].m[k];
+    return o;
+  }
+};
+
+template <typename T>
+using Scene = BlazertScene<T>;
+
+// blazert/scene.h:161-211: rayhit is in/out; returns whether a hit was merged into it.
+template <typename T>
+bool intersect1(const BlazertScene<T> &scene, const Ray<T> &ray, RayHit<T> &rayhit) {
+  const double o[3] = {ray.origin[0], ray.origin[1], ray.origin[2]};
+  const double d[3] = {ray.direction[0], ray.direction[1], ray.direction[2]};
+  brt_hit1 h{};
+  for (int k = 0; k < 3; k++) h.normal[k] = rayhit.normal[k];
+  h.u = rayhit.uv[0];
+  h.v = rayhit.uv[1];
+  h.t = rayhit.hit_distance;
+  h.prim_id = rayhit.prim_id;
+  h.geom_id = rayhit.geom_id;
+  int hit = 0;
+  const uint32_t flags = ray.any_hit == Ray<T>::AnyHit::yes ? BRT_FLAG_ANY_HIT : 0u;
+  if (brt_intersect1(scene.native(), o, d, ray.min_hit_distance, ray.max_hit_distance, flags, &h, &hit) != BRT_OK)
+    throw std::runtime_error(brt_last_error());
+  for (int k = 0; k < 3; k++) rayhit.normal[k] = h.normal[k];
+  rayhit.uv[0] = h.u;
+  rayhit.uv[1] = h.v;
+  rayhit.hit_distance = h.t;
+  rayhit.prim_id = h.prim_id;
+  rayhit.geom_id = h.geom_id;
+  return hit != 0;
+}
+
+}  // namespace blazert
