@@ -56,7 +56,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-rays", type=int, default=1 << 20)
+    ap.add_argument("--cpu-rays", type=int, default=1 << 22)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -65,6 +67,15 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate_copy:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": a.numel() * 8, "copies": 3}))
+        return
 
     from blazert_amd import PRECISION_FP32_TRAV_FP64_HIT, PRECISION_FP64, Scene, workloads as W
 
@@ -84,10 +95,11 @@ def main():
     scene.upload()
     log(f"commit {build_s:.2f}s depth {stats['max_tree_depth']} leaves {stats['leaf_nodes']}")
 
-    O = torch.from_numpy(org).to(dev)
-    D = torch.from_numpy(dirs).to(dev)
-    CO = torch.from_numpy(corg).to(dev)
-    CD = torch.from_numpy(cdirs).to(dev)
+    def cols(a):  # structure-of-arrays columns on the device, made once outside the timed region
+        return tuple(torch.from_numpy(np.ascontiguousarray(a[:, k])).to(dev) for k in range(3))
+
+    O, D = cols(org), cols(dirs)
+    CO, CD = cols(corg), cols(cdirs)
     n = len(org)
 
     ms64, wall64 = time_launch(lambda: scene.intersect(O, D, precision=PRECISION_FP64), args.steps, args.warmup)
@@ -164,7 +176,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = {"value": round(ns / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"first {ns} of the {n} incoherent rays, oracle intersect1 with traversal "
+                               "sample": f"{ns} of the {n} incoherent rays, oracle intersect1 with traversal "
                                          f"trace, OpenMP schedule(dynamic,1024) on {threads} threads, "
                                          f"{cpu_s:.2f} s wall"}
     print(json.dumps(out), flush=True)

@@ -38,26 +38,26 @@ def _run(cmd):
     return r
 
 
-def build(verbose: bool = False) -> str:
-    """Compiles every source (always) and links libblazert_amd.so next to this file."""
+def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
+    """Compiles every source (always) and links libblazert_amd.so next to this file (or `out`)."""
     cc = hipcc()
     tmp = tempfile.mkdtemp(prefix="brt_build_")
     try:
         objs = []
         k_obj = os.path.join(tmp, "kernels.o")
-        _run([cc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "kernels.hip"), "-o", k_obj])
+        _run([cc, *COMMON, *DEVICE, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, "kernels.hip"), "-o", k_obj])
         objs.append(k_obj)
         s_obj = os.path.join(tmp, "scene.o")
         _run([cc, *COMMON, "-x", "c++", "-pthread", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-c", os.path.join(CSRC, "scene.cpp"), "-o", s_obj])
         objs.append(s_obj)
-        out_tmp = LIB + ".tmp"
+        out_tmp = out + ".tmp"
         _run([cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-pthread", *objs, "-o", out_tmp])
-        os.replace(out_tmp, LIB)
+        os.replace(out_tmp, out)
         if verbose:
-            print("built", LIB)
+            print("built", out)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
-    return LIB
+    return out
 
 
 if __name__ == "__main__":

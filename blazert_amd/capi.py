@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libblazert_amd.so")
+LIB_PATH = os.environ.get("BLAZERT_AMD_LIB") or os.path.join(HERE, "libblazert_amd.so")
 
 OK = 0
 ERR_INVALID_ARGUMENT = -1

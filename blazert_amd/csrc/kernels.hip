@@ -45,9 +45,10 @@ struct Ray {
   double o[3];
   double d[3];    // normalize(raw direction)
   double inv[3];  // 1 / raw direction
-  uint32_t sign;  // bit i: raw direction[i] < 0
+  uint32_t sign;  // bit i: raw direction[i] < 0; bit 3: no slab term can be NaN (see make_ray)
   double tmin, tmax;
 };
+constexpr uint32_t RAY_NAN_FREE = 8u;
 
 // blazert/ray.h:56-65 with normalize(v) = v * (len != 0 ? 1/len : 0)
 __device__ __forceinline__ void make_ray(const RaysView &R, uint64_t i, Ray &r) {
@@ -66,6 +67,14 @@ __device__ __forceinline__ void make_ray(const RaysView &R, uint64_t i, Ray &r) 
   }
   r.tmin = R.tmin ? R.tmin[i] : 0.0;
   r.tmax = R.tmax ? R.tmax[i] : DBL_MAX;
+  // With a finite origin, finite non-zero direction components (so 1/d is finite and non-zero), a non-NaN tmin and
+  // finite node boxes (checked on the host), (b - o) * inv is never NaN: the slab chains reduce to plain max/min.
+  bool ok = r.tmin == r.tmin;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ok = ok && fabs(r.o[k]) <= DBL_MAX && raw[k] != 0.0 && fabs(raw[k]) <= DBL_MAX && fabs(r.inv[k]) <= DBL_MAX;
+  }
+  if (ok) r.sign |= RAY_NAN_FREE;
 }
 
 // ------------------------------------------------------------------------------------------------ slab tests
@@ -73,6 +82,19 @@ __device__ __forceinline__ void make_ray(const RaysView &R, uint64_t i, Ray &r) 
 // box fails for every hit distance; otherwise key is the value the current hit distance h must not be below
 // (pass iff !(h < key)); key = -inf when a NaN far-plane term removes h from the reference's min chain.
 __device__ __forceinline__ bool slab64(const double *b, const Ray &r, double &key) {
+  if (r.sign & RAY_NAN_FREE) {
+    double A = r.tmin, M = __builtin_inf();
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const bool s = (r.sign >> i) & 1u;
+      const double nearp = s ? b[3 + i] : b[i];
+      const double farp = s ? b[i] : b[3 + i];
+      A = fmax(A, (nearp - r.o[i]) * r.inv[i]);
+      M = fmin(M, ((farp - r.o[i]) * r.inv[i]) * L1);
+    }
+    key = A;
+    return A <= M;
+  }
   double acc = r.tmin;
   double m[3];
 #pragma unroll
@@ -702,8 +724,10 @@ __device__ __forceinline__ void finish_ray(const BvhView &bvh, const HitsView &h
 }
 
 template <bool SPLIT>
-__device__ __forceinline__ void start_ray(const RaysView &rays, uint64_t i, double S, Ray &r, Ray32 &q, Hit &s) {
+__device__ __forceinline__ void start_ray(const BvhView &bvh, const RaysView &rays, uint64_t i, double S, Ray &r,
+                                          Ray32 &q, Hit &s) {
   make_ray(rays, i, r);
+  if (bvh.nan_boxes) r.sign &= ~RAY_NAN_FREE;
   if (SPLIT) make_ray32(r, S, q);
   s.h = r.tmax;  // prepare_traversal
   s.u = 0.0;
@@ -724,7 +748,7 @@ __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysView rays, HitsV
     Ray r;
     Ray32 q;
     Hit s;
-    start_ray<SPLIT>(rays, i, S, r, q, s);
+    start_ray<SPLIT>(bvh, rays, i, S, r, q, s);
     Trav t;
     trav_begin<SPLIT>(bvh, r, q, s, t);
     st.reset();
@@ -738,8 +762,14 @@ __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysView rays, HitsV
 // Skewed per-ray work (many rays miss the root box, a few visit hundreds of nodes) then no longer idles the
 // wave until its slowest ray ends. Every lane leaves the loop once the counter has passed n and its own ray is
 // done, so the grid always drains.
+// Occupancy target of the persistent kernels (waves per SIMD the register allocation must allow).
+#ifndef BRT_WAVES_PER_EU
+#define BRT_WAVES_PER_EU 4
+#endif
+
 template <class Rec, bool SPLIT>
-__global__ void __launch_bounds__(256) k_trace_persistent(BvhView bvh, RaysView rays, HitsView hits, uint64_t n,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
+k_trace_persistent(BvhView bvh, RaysView rays, HitsView hits, uint64_t n,
                                                           uint32_t flags, uint32_t first, uint32_t is_tri, double S,
                                                           unsigned long long *counter, uint32_t refill) {
   extern __shared__ unsigned char smem[];
@@ -773,7 +803,7 @@ __global__ void __launch_bounds__(256) k_trace_persistent(BvhView bvh, RaysView 
           if (cand < n) {
             idx = cand;
             has_ray = true;
-            start_ray<SPLIT>(rays, idx, S, r, q, s);
+            start_ray<SPLIT>(bvh, rays, idx, S, r, q, s);
             trav_begin<SPLIT>(bvh, r, q, s, t);
             st.reset();
           }
@@ -828,7 +858,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysView &rays, const HitsVie
     const uint64_t need = (n + block - 1) / block;
     if (blocks > need) blocks = need;
     if (blocks == 0) return hipSuccess;
-    const uint32_t refill = cfg.refill ? cfg.refill : 16u;
+    const uint32_t refill = cfg.refill ? cfg.refill : (SPLIT ? 16u : 4u);
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
     hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
                        hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill);

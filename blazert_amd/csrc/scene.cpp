@@ -417,8 +417,9 @@ void flatten(Geometry &g, GeomType type) {
   g.n64.assign(ni, Node64{});
   g.n32.assign(ni, Node32{});
   double extent = 0.0;
-  bool ok = true;
+  bool ok = true, nan_boxes = false;
   auto note = [&](double v) {
+    if (v != v) nan_boxes = true;
     if (!std::isfinite(v)) ok = false;
     else extent = std::max(extent, std::fabs(v));
   };
@@ -467,6 +468,7 @@ void flatten(Geometry &g, GeomType type) {
   v.n_prims = g.boxes.n;
   v.n_nodes = ni;
   v.depth = g.stats.max_tree_depth;
+  v.nan_boxes = nan_boxes ? 1u : 0u;
   g.extent = extent;
   g.extent_ok = ok && extent < 1e30;
 

@@ -104,7 +104,7 @@ typedef struct brt_launch {
   uint32_t flags;      /* BRT_FLAG_* */
   uint32_t block_size; /* threads per block, multiple of 64 up to 1024; 0 = library default (256) */
   uint32_t schedule;   /* BRT_SCHEDULE_* */
-  uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default (16) */
+  uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default */
   uint32_t reserved[3];
 } brt_launch;
 
