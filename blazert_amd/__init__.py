@@ -7,8 +7,8 @@ include/blazert_amd.h).
 """
 from .capi import (FLAG_ANY_HIT, FLAG_HITS_INOUT, GEOM_CYLINDERS, GEOM_PLANES, GEOM_SPHERES, GEOM_TRIANGLES,
                    INVALID_ID, PRECISION_FP32_TRAV_FP64_HIT, PRECISION_FP64, BlazertError)
-from .scene import DBL_MAX, Ray, RayHit, Scene, intersect1
+from .scene import DBL_MAX, Batch, Ray, RayHit, Scene, intersect1
 
-__all__ = ["Scene", "Ray", "RayHit", "intersect1", "INVALID_ID", "DBL_MAX", "BlazertError", "PRECISION_FP64",
+__all__ = ["Scene", "Batch", "Ray", "RayHit", "intersect1", "INVALID_ID", "DBL_MAX", "BlazertError", "PRECISION_FP64",
            "PRECISION_FP32_TRAV_FP64_HIT", "FLAG_ANY_HIT", "FLAG_HITS_INOUT", "GEOM_TRIANGLES", "GEOM_SPHERES",
            "GEOM_PLANES", "GEOM_CYLINDERS"]

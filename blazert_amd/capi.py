@@ -53,11 +53,18 @@ class Hits(C.Structure):
 
 SCHEDULE_PERSISTENT = 0
 SCHEDULE_PER_THREAD = 1
+SORT_NONE = 0
+SORT_OCTANT_MORTON = 1
+SORT_MORTON = 2
 
 
 class Launch(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("flags", C.c_uint32), ("block_size", C.c_uint32),
-                ("schedule", C.c_uint32), ("refill", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("schedule", C.c_uint32), ("refill", C.c_uint32), ("sort", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("rays", Rays), ("hits", Hits), ("n", C.c_uint64)]
 
 
 class Hit1(C.Structure):
@@ -86,6 +93,11 @@ SIGNATURES = {
     "brt_intersect": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_uint64, C.POINTER(Launch), C.c_void_p]),
     "brt_intersect1": (C.c_int, [_P, _D, _D, C.c_double, C.c_double, C.c_uint32, C.POINTER(Hit1),
                                  C.POINTER(C.c_int)]),
+    "brt_batch_create": (C.c_int, [C.POINTER(BatchItem), C.c_uint32, C.POINTER(Launch), C.POINTER(C.c_void_p)]),
+    "brt_batch_launch": (C.c_int, [_P, _P]),
+    "brt_batch_capture": (C.c_int, [_P, _P]),
+    "brt_batch_replay": (C.c_int, [_P, _P]),
+    "brt_batch_destroy": (C.c_int, [_P]),
 }
 
 _lib = None

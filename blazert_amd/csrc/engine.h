@@ -17,7 +17,41 @@ struct TraceLaunch {
   uint32_t schedule;   // SCHEDULE_*
   uint32_t refill;     // persistent: refill a wave once this many lanes are idle (1..64)
   unsigned long long *counter;  // persistent: device ray counter for this launch (zeroed on the stream)
+  const uint32_t *order;        // optional ray permutation (sorted launches); kernels trace rays[order[k]]
 };
+
+// One (scene, geometry type, ray batch) of a multi-scene launch; jobs are sorted by offset.
+struct BatchJob {
+  BvhView bvh;
+  RaysView rays;
+  HitsView hits;
+  uint64_t n;       // rays of this job
+  uint64_t offset;  // prefix sum of n over the preceding jobs
+  uint32_t first;   // this geometry type is the first one of the job's scene
+  uint32_t is_tri;
+  double extent;    // split precision: scene extent
+  const uint32_t *order;
+};
+
+// Enqueues one persistent launch over all jobs of one geometry type (total = sum of n).
+hipError_t launch_trace_batch(GeomType type, const BatchJob *jobs, uint32_t njobs, uint64_t total,
+                              const TraceLaunch &cfg, hipStream_t stream);
+
+// Device buffers of one ray sort (keys, values, radix-sort temporary storage).
+struct SortWorkspace {
+  void *base = nullptr;
+  uint32_t *keys_in = nullptr, *keys_out = nullptr, *vals_in = nullptr, *vals_out = nullptr;
+  void *temp = nullptr;
+  size_t temp_bytes = 0;
+  uint64_t capacity = 0;
+};
+
+// Grows ws to hold n rays (frees the previous buffers; not capturable in a graph).
+hipError_t sort_workspace_reserve(SortWorkspace &ws, uint64_t n);
+// Computes the sort keys (mode 1: direction octant + origin Morton code, mode 2: Morton only) and radix-sorts the
+// ray indices on `stream`; *order points into ws afterwards.
+hipError_t sort_rays(const RaysView &rays, uint64_t n, const double lo[3], const double hi[3], uint32_t mode,
+                     SortWorkspace &ws, hipStream_t stream, const uint32_t **order);
 
 // Per-scene constants for the split-precision path: the largest finite |coordinate| of any node box.
 struct SplitParams {

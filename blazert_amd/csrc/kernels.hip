@@ -17,6 +17,8 @@
 
 #include <type_traits>
 
+#include <hipcub/hipcub.hpp>
+
 #include "engine.h"
 
 namespace brt {
@@ -525,12 +527,19 @@ struct Trav {
 constexpr int RING = 8;
 
 template <bool SPLIT>
+struct SpillArea {  // private (scratch) memory for entries pushed deeper than the ring
+  using Key = typename std::conditional<SPLIT, float, double>::type;
+  Key key[STACK_CAP];
+  uint32_t ref[STACK_CAP], cnt[STACK_CAP];
+};
+
+// The counters and pointers are kept apart from the spill arrays so that they stay in registers.
+template <bool SPLIT>
 struct RingStack {
   using Key = typename std::conditional<SPLIT, float, double>::type;
   Key *lkey;  // this lane's slot 0 in LDS; slot j is at [64 * j]
   uint32_t *lref, *lcnt;
-  Key pkey[STACK_CAP];
-  uint32_t pref[STACK_CAP], pcnt[STACK_CAP];
+  SpillArea<SPLIT> *spill;
   int sp, lo;
 
   __device__ __forceinline__ void reset() {
@@ -540,9 +549,9 @@ struct RingStack {
   __device__ __forceinline__ void push(uint32_t r, uint32_t c, Key k) {
     if (sp - lo == RING) {
       const int j = (lo & (RING - 1)) * 64;
-      pkey[lo] = lkey[j];
-      pref[lo] = lref[j];
-      pcnt[lo] = lcnt[j];
+      spill->key[lo] = lkey[j];
+      spill->ref[lo] = lref[j];
+      spill->cnt[lo] = lcnt[j];
       lo++;
     }
     const int j = (sp & (RING - 1)) * 64;
@@ -556,9 +565,9 @@ struct RingStack {
     sp--;
     if (sp < lo) {
       lo = sp;
-      r = pref[sp];
-      c = pcnt[sp];
-      k = pkey[sp];
+      r = spill->ref[sp];
+      c = spill->cnt[sp];
+      k = spill->key[sp];
     } else {
       const int j = (sp & (RING - 1)) * 64;
       r = lref[j];
@@ -575,7 +584,7 @@ constexpr size_t ring_lds_bytes(uint32_t block) {
 }
 
 template <bool SPLIT>
-__device__ __forceinline__ void ring_bind(RingStack<SPLIT> &st, unsigned char *smem) {
+__device__ __forceinline__ void ring_bind(RingStack<SPLIT> &st, unsigned char *smem, SpillArea<SPLIT> *spill) {
   using Key = typename RingStack<SPLIT>::Key;
   const uint32_t waves = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   Key *keys = reinterpret_cast<Key *>(smem);
@@ -584,6 +593,7 @@ __device__ __forceinline__ void ring_bind(RingStack<SPLIT> &st, unsigned char *s
   st.lkey = keys + (size_t)wave * RING * 64 + lane;
   st.lref = refs + (size_t)wave * RING * 64 + lane;
   st.lcnt = cnts + (size_t)wave * RING * 64 + lane;
+  st.spill = spill;
   st.reset();
 }
 
@@ -626,10 +636,13 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const Ray &r, cons
       near_ok = (nc ? p1 : p0) && !(t.h32 < (nc ? k1 : k0));
       far_ok = nc ? p0 : p1;
       fkey = nc ? k0 : k1;
-      nref = nd.ref[nc];
-      ncnt = nd.cnt[nc] & CNT_MASK;
-      fref = nd.ref[1 - nc];
-      fcnt = nd.cnt[1 - nc] & CNT_MASK;
+      // both fields read into registers and selected: a runtime index into the local record would put the
+      // whole record in scratch memory
+      const uint32_t r0 = nd.ref[0], r1 = nd.ref[1], c0 = nd.cnt[0] & CNT_MASK, c1 = nd.cnt[1] & CNT_MASK;
+      nref = nc ? r1 : r0;
+      ncnt = nc ? c1 : c0;
+      fref = nc ? r0 : r1;
+      fcnt = nc ? c0 : c1;
     } else {
       const Node64 nd = bvh.nodes64[t.cur];
       const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
@@ -640,10 +653,11 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const Ray &r, cons
       near_ok = (nc ? p1 : p0) && !(s.h < (nc ? k1 : k0));
       far_ok = nc ? p0 : p1;
       fkey = nc ? k0 : k1;
-      nref = nd.ref[nc];
-      ncnt = nd.cnt[nc] & CNT_MASK;
-      fref = nd.ref[1 - nc];
-      fcnt = nd.cnt[1 - nc] & CNT_MASK;
+      const uint32_t r0 = nd.ref[0], r1 = nd.ref[1], c0 = nd.cnt[0] & CNT_MASK, c1 = nd.cnt[1] & CNT_MASK;
+      nref = nc ? r1 : r0;
+      ncnt = nc ? c1 : c0;
+      fref = nc ? r0 : r1;
+      fcnt = nc ? c0 : c1;
     }
     if (far_ok) st.push(fref, fcnt, fkey);
     if (near_ok) {
@@ -738,13 +752,15 @@ __device__ __forceinline__ void start_ray(const BvhView &bvh, const RaysView &ra
 // One ray per thread (grid-stride).
 template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysView rays, HitsView hits, uint64_t n, uint32_t flags,
-                                               uint32_t first, uint32_t is_tri, double S) {
+                                               uint32_t first, uint32_t is_tri, double S, const uint32_t *order) {
   extern __shared__ unsigned char smem[];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
+  SpillArea<SPLIT> spill;
   RingStack<SPLIT> st;
-  ring_bind<SPLIT>(st, smem);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  ring_bind<SPLIT>(st, smem, &spill);
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const uint64_t i = order ? (uint64_t)order[k] : k;
     Ray r;
     Ray32 q;
     Hit s;
@@ -771,12 +787,14 @@ template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
 k_trace_persistent(BvhView bvh, RaysView rays, HitsView hits, uint64_t n,
                                                           uint32_t flags, uint32_t first, uint32_t is_tri, double S,
-                                                          unsigned long long *counter, uint32_t refill) {
+                                                          unsigned long long *counter, uint32_t refill,
+                                                          const uint32_t *order) {
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
   const uint32_t lane = __lane_id();
+  SpillArea<SPLIT> spill;
   RingStack<SPLIT> st;
-  ring_bind<SPLIT>(st, smem);
+  ring_bind<SPLIT>(st, smem, &spill);
   Ray r;
   Ray32 q;
   Hit s;
@@ -801,7 +819,7 @@ k_trace_persistent(BvhView bvh, RaysView rays, HitsView hits, uint64_t n,
           const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
           const uint64_t cand = base + rank;
           if (cand < n) {
-            idx = cand;
+            idx = order ? (uint64_t)order[cand] : cand;
             has_ray = true;
             start_ray<SPLIT>(bvh, rays, idx, S, r, q, s);
             trav_begin<SPLIT>(bvh, r, q, s, t);
@@ -821,7 +839,109 @@ k_trace_persistent(BvhView bvh, RaysView rays, HitsView hits, uint64_t n,
   }
 }
 
+// Many scenes in one launch: the claimed index runs over the concatenated rays of all jobs (one job = one scene's
+// BVH of this geometry type + its own rays and hits). A lane finds its job by binary search over the prefix offsets
+// and reads that job's BVH view from the table (a few cached loads per step), so waves freely mix scenes and the
+// uneven batch drains with no per-scene tail.
+__device__ __forceinline__ uint32_t find_job(const BatchJob *jobs, uint32_t njobs, uint64_t g) {
+  uint32_t lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].offset <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class Rec, bool SPLIT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
+k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total, uint32_t flags,
+              unsigned long long *counter, uint32_t refill) {
+  extern __shared__ unsigned char smem[];
+  const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
+  const uint32_t lane = __lane_id();
+  SpillArea<SPLIT> spill;
+  RingStack<SPLIT> st;
+  ring_bind<SPLIT>(st, smem, &spill);
+  Ray r;
+  Ray32 q;
+  Hit s;
+  Trav t;
+  t.active = false;
+  bool has_ray = false;
+  bool exhausted = false;  // wave-uniform
+  uint64_t idx = 0;
+  uint32_t j = 0;
+  for (;;) {
+    if (!exhausted) {
+      const unsigned long long idle = __ballot(!has_ray);
+      const uint32_t nidle = (uint32_t)__popcll(idle);
+      if (nidle >= refill || nidle == 64u) {
+        unsigned long long base = 0;
+        const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
+        if (lane == leader) base = atomicAdd(counter, (unsigned long long)nidle);
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)base, (int)leader);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(base >> 32), (int)leader);
+        base = ((unsigned long long)hi << 32) | lo;
+        if (base + nidle >= total) exhausted = true;
+        if (!has_ray) {
+          const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+          const uint64_t g = base + rank;
+          if (g < total) {
+            j = find_job(jobs, njobs, g);
+            const BatchJob &J = jobs[j];
+            const uint64_t local = g - J.offset;
+            idx = J.order ? (uint64_t)J.order[local] : local;
+            has_ray = true;
+            start_ray<SPLIT>(J.bvh, J.rays, idx, J.extent, r, q, s);
+            trav_begin<SPLIT>(J.bvh, r, q, s, t);
+            st.reset();
+          }
+        }
+      }
+    }
+    if (!__ballot(has_ray)) break;
+    if (has_ray) {
+      const BatchJob &J = jobs[j];
+      if (t.active) trav_step<Rec, SPLIT>(J.bvh, r, q, any_hit, s, t, st);
+      if (!t.active) {
+        finish_ray<Rec>(J.bvh, J.hits, idx, r, s, flags, J.first, J.is_tri);
+        has_ray = false;
+      }
+    }
+  }
+}
+
 __global__ void k_zero_counter(unsigned long long *counter) { *counter = 0ull; }
+
+// 3 x 9-bit Morton interleave
+__device__ __forceinline__ uint32_t spread9(uint32_t x) {
+  x &= 0x1ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// Sort key of ray i: direction octant (3 bits, the children visit order depends only on it) above a 27-bit Morton
+// code of the origin quantised in the scene box (mode 1), or the Morton code alone (mode 2).
+__global__ void k_sort_keys(RaysView rays, uint64_t n, double lx, double ly, double lz, double sx, double sy,
+                            double sz, uint32_t mode, uint32_t *keys, uint32_t *vals) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double q[3] = {(rays.ox[i] - lx) * sx, (rays.oy[i] - ly) * sy, (rays.oz[i] - lz) * sz};
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double c = q[k] != q[k] ? 0.0 : fmin(fmax(q[k], 0.0), 511.0);
+      m |= spread9((uint32_t)c) << k;
+    }
+    uint32_t oct = (rays.dx[i] < 0.0 ? 1u : 0u) | (rays.dy[i] < 0.0 ? 2u : 0u) | (rays.dz[i] < 0.0 ? 4u : 0u);
+    keys[i] = mode == 1u ? ((oct << 27) | m) : m;
+    vals[i] = (uint32_t)i;
+  }
+}
 
 __global__ void k_init_hits(HitsView hits, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -858,17 +978,41 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysView &rays, const HitsVie
     const uint64_t need = (n + block - 1) / block;
     if (blocks > need) blocks = need;
     if (blocks == 0) return hipSuccess;
-    const uint32_t refill = cfg.refill ? cfg.refill : (SPLIT ? 16u : 4u);
+    const uint32_t refill = cfg.refill ? cfg.refill : (SPLIT ? 16u : 8u);
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
     hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
-                       hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill);
+                       hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill, cfg.order);
     return hipGetLastError();
   }
   uint64_t blocks = (n + block - 1) / block;
   if (blocks > (1ull << 30)) blocks = 1ull << 30;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((k_trace<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays, hits, n,
-                     cfg.flags, first, is_tri, sp.scene_extent);
+                     cfg.flags, first, is_tri, sp.scene_extent, cfg.order);
+  return hipGetLastError();
+}
+
+template <class Rec, bool SPLIT>
+hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t total, const TraceLaunch &cfg,
+                               hipStream_t stream) {
+  const uint32_t block = cfg.block ? cfg.block : 256;
+  const size_t lds = ring_lds_bytes<SPLIT>(block);
+  if (!cfg.counter || njobs == 0) return hipErrorInvalidValue;
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_trace_batch<Rec, SPLIT>, (int)block,
+                                                     lds);
+  if (e != hipSuccess) return e;
+  uint64_t blocks = (uint64_t)cus * (uint64_t)(per_cu > 0 ? per_cu : 1);
+  const uint64_t need = (total + block - 1) / block;
+  if (blocks > need) blocks = need;
+  if (blocks == 0) return hipSuccess;
+  const uint32_t refill = cfg.refill ? cfg.refill : (SPLIT ? 16u : 8u);
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
+  hipLaunchKernelGGL((k_trace_batch<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, jobs, njobs, total,
+                     cfg.flags, cfg.counter, refill);
   return hipGetLastError();
 }
 
@@ -888,6 +1032,78 @@ hipError_t launch_trace(GeomType type, const BvhView &bvh, const RaysView &rays,
   case SPHERES: return launch_typed<SphRec>(bvh, rays, hits, n, cfg, split, 0u, stream);
   case PLANES: return launch_typed<PlaRec>(bvh, rays, hits, n, cfg, split, 0u, stream);
   default: return launch_typed<CylRec>(bvh, rays, hits, n, cfg, split, 0u, stream);
+  }
+}
+
+hipError_t sort_workspace_reserve(SortWorkspace &ws, uint64_t n) {
+  if (n > 0x7fffffffull) return hipErrorInvalidValue;
+  if (ws.capacity >= n && ws.base) return hipSuccess;
+  size_t temp = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                    (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, 30);
+  if (e != hipSuccess) return e;
+  if (ws.base) {
+    e = hipFree(ws.base);
+    ws.base = nullptr;
+    ws.capacity = 0;
+    if (e != hipSuccess) return e;
+  }
+  const size_t arr = ((size_t)n * 4 + 255) & ~(size_t)255;
+  void *p = nullptr;
+  e = hipMalloc(&p, 4 * arr + temp);
+  if (e != hipSuccess) return e;
+  unsigned char *b = static_cast<unsigned char *>(p);
+  ws.base = p;
+  ws.keys_in = reinterpret_cast<uint32_t *>(b);
+  ws.keys_out = reinterpret_cast<uint32_t *>(b + arr);
+  ws.vals_in = reinterpret_cast<uint32_t *>(b + 2 * arr);
+  ws.vals_out = reinterpret_cast<uint32_t *>(b + 3 * arr);
+  ws.temp = b + 4 * arr;
+  ws.temp_bytes = temp;
+  ws.capacity = n;
+  return hipSuccess;
+}
+
+hipError_t sort_rays(const RaysView &rays, uint64_t n, const double lo[3], const double hi[3], uint32_t mode,
+                     SortWorkspace &ws, hipStream_t stream, const uint32_t **order) {
+  *order = nullptr;
+  if (n == 0) return hipSuccess;
+  if (ws.capacity < n) return hipErrorInvalidValue;
+  double sc[3];
+  for (int k = 0; k < 3; k++) {
+    const double ext = hi[k] - lo[k];
+    sc[k] = (ext > 0.0 && ext == ext) ? 512.0 / ext : 0.0;
+  }
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_sort_keys, dim3((uint32_t)blocks), dim3(256), 0, stream, rays, n, lo[0], lo[1], lo[2], sc[0],
+                     sc[1], sc[2], mode, ws.keys_in, ws.vals_in);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  size_t temp = ws.temp_bytes;
+  e = hipcub::DeviceRadixSort::SortPairs(ws.temp, temp, ws.keys_in, ws.keys_out, ws.vals_in, ws.vals_out, (int)n, 0,
+                                         30, stream);
+  if (e != hipSuccess) return e;
+  *order = ws.vals_out;
+  return hipSuccess;
+}
+
+hipError_t launch_trace_batch(GeomType type, const BatchJob *jobs, uint32_t njobs, uint64_t total,
+                              const TraceLaunch &cfg, hipStream_t stream) {
+  const bool split = cfg.precision != PREC_FP64;
+  switch (type) {
+  case TRIANGLES:
+    return split ? launch_batch_kernel<TriRec, true>(jobs, njobs, total, cfg, stream)
+                 : launch_batch_kernel<TriRec, false>(jobs, njobs, total, cfg, stream);
+  case SPHERES:
+    return split ? launch_batch_kernel<SphRec, true>(jobs, njobs, total, cfg, stream)
+                 : launch_batch_kernel<SphRec, false>(jobs, njobs, total, cfg, stream);
+  case PLANES:
+    return split ? launch_batch_kernel<PlaRec, true>(jobs, njobs, total, cfg, stream)
+                 : launch_batch_kernel<PlaRec, false>(jobs, njobs, total, cfg, stream);
+  default:
+    return split ? launch_batch_kernel<CylRec, true>(jobs, njobs, total, cfg, stream)
+                 : launch_batch_kernel<CylRec, false>(jobs, njobs, total, cfg, stream);
   }
 }
 

@@ -573,8 +573,19 @@ struct DeviceScene {
   BvhView views[NUM_TYPES]{};
   unsigned long long *counters = nullptr;  // ray counters of the persistent schedule, one per launch in flight
   std::atomic<uint32_t> next_counter{0};
+  std::mutex sort_mu;
+  std::vector<std::pair<hipStream_t, std::unique_ptr<SortWorkspace>>> sorts;  // one ray-sort workspace per stream
   ~DeviceScene() {
+    for (auto &w : sorts)
+      if (w.second->base) (void)hipFree(w.second->base);
     for (void *p : allocs) (void)hipFree(p);
+  }
+  SortWorkspace *sort_workspace(hipStream_t st) {
+    std::lock_guard<std::mutex> lk(sort_mu);
+    for (auto &w : sorts)
+      if (w.first == st) return w.second.get();
+    sorts.emplace_back(st, std::make_unique<SortWorkspace>());
+    return sorts.back().second.get();
   }
   template <class T>
   hipError_t put(const std::vector<T> &v, const T **dst) {
@@ -816,20 +827,38 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
   if (L.precision > BRT_PRECISION_FP32_TRAV_FP64_HIT) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown precision");
   if (L.schedule > BRT_SCHEDULE_PER_THREAD) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown schedule");
   if (L.refill > 64) return fail(BRT_ERR_INVALID_ARGUMENT, "refill must be 0..64");
-  if (L.block_size && (L.block_size % 64 || L.block_size > 1024))
-    return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 up to 1024");
+  if (L.sort > BRT_SORT_MORTON) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown sort mode");
+  if (L.block_size && (L.block_size % 64 || L.block_size > 256))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
   int rc = upload(s);
   if (rc) return rc;
   RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
   HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz, nullptr};
   hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint32_t *order = nullptr;
+  if (L.sort != BRT_SORT_NONE) {
+    if (n > 0x7fffffffull) return fail(BRT_ERR_UNSUPPORTED, "sorted launches are limited to 2^31-1 rays");
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (uint32_t t = 0; t < NUM_TYPES; t++) {
+      const Geometry &g = s->geo[t];
+      if (!g.present || g.boxes.n == 0) continue;
+      for (int k = 0; k < 3; k++) {
+        lo[k] = std::min(lo[k], g.view.root_min[k]);
+        hi[k] = std::max(hi[k], g.view.root_max[k]);
+      }
+    }
+    SortWorkspace *ws = s->dev->sort_workspace(st);
+    hipError_t e = sort_workspace_reserve(*ws, n);
+    if (e == hipSuccess) e = sort_rays(rv, n, lo, hi, L.sort, *ws, st, &order);
+    if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("ray sort failed: ") + hipGetErrorString(e));
+  }
   bool first = true;
   for (uint32_t t = 0; t < NUM_TYPES; t++) {
     const Geometry &g = s->geo[t];
     if (!g.present || g.boxes.n == 0) continue;
     if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT && !g.extent_ok)
       return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
-    TraceLaunch cfg{L.precision, L.flags, L.block_size, first, L.schedule, L.refill, nullptr};
+    TraceLaunch cfg{L.precision, L.flags, L.block_size, first, L.schedule, L.refill, nullptr, order};
     if (L.schedule == BRT_SCHEDULE_PERSISTENT)
       cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
     SplitParams sp{g.extent};
@@ -869,7 +898,8 @@ int brt_intersect1(const brt_scene *s, const double org[3], const double dir[3],
     for (uint32_t t = 0; t < NUM_TYPES && e == hipSuccess; t++) {
       const Geometry &g = ms->geo[t];
       if (!g.present || g.boxes.n == 0) continue;
-      TraceLaunch cfg{BRT_PRECISION_FP64, flags | BRT_FLAG_HITS_INOUT, 64, false, SCHEDULE_PER_THREAD, 0, nullptr};
+      TraceLaunch cfg{BRT_PRECISION_FP64, flags | BRT_FLAG_HITS_INOUT, 64, false, SCHEDULE_PER_THREAD, 0, nullptr,
+                      nullptr};
       e = launch_trace((GeomType)t, ms->dev->views[t], rv, hv, 1, cfg, SplitParams{g.extent}, nullptr);
     }
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -890,6 +920,163 @@ int brt_intersect1(const brt_scene *s, const double org[3], const double dir[3],
   (void)hipFree(d);
   if (rc != BRT_OK) return rc;
   if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("intersect1: ") + hipGetErrorString(e));
+  return BRT_OK;
+}
+
+// ---------------------------------------------------------------------------------------------- batches
+}  // extern "C"
+
+struct brt_batch {
+  int device = -1;
+  brt_launch L{};
+  std::vector<brt_batch_item> items;
+  void *d_jobs[NUM_TYPES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t njobs[NUM_TYPES] = {0, 0, 0, 0};
+  uint64_t total[NUM_TYPES] = {0, 0, 0, 0};
+  unsigned long long *counters = nullptr;
+  std::vector<size_t> empty_items;  // items whose scene holds no primitive: their hits are initialised
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~brt_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void *p : d_jobs)
+      if (p) (void)hipFree(p);
+    if (counters) (void)hipFree(counters);
+  }
+};
+
+namespace {
+
+int batch_enqueue(brt_batch *b, hipStream_t st) {
+  for (uint32_t t = 0; t < NUM_TYPES; t++) {
+    if (!b->njobs[t]) continue;
+    TraceLaunch cfg{b->L.precision, b->L.flags, b->L.block_size, false, SCHEDULE_PERSISTENT, b->L.refill,
+                    b->counters + t, nullptr};
+    hipError_t e = launch_trace_batch((GeomType)t, static_cast<const BatchJob *>(b->d_jobs[t]), b->njobs[t],
+                                      b->total[t], cfg, st);
+    if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("batch launch failed: ") + hipGetErrorString(e));
+  }
+  if (!(b->L.flags & BRT_FLAG_HITS_INOUT))
+    for (size_t i : b->empty_items) {
+      const brt_hits &h = b->items[i].hits;
+      HitsView hv{h.t, h.u, h.v, h.prim_id, h.geom_id, h.nx, h.ny, h.nz, nullptr};
+      hipError_t e = launch_init_hits(hv, b->items[i].n, st);
+      if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+    }
+  return BRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int brt_batch_create(const brt_batch_item *items, uint32_t count, const brt_launch *launch, brt_batch **out) {
+  if (!out || (count && !items)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  auto b = std::make_unique<brt_batch>();
+  b->L.precision = BRT_PRECISION_FP64;
+  if (launch) b->L = *launch;
+  if (b->L.precision > BRT_PRECISION_FP32_TRAV_FP64_HIT) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (b->L.sort != BRT_SORT_NONE) return fail(BRT_ERR_UNSUPPORTED, "batches trace rays in input order");
+  if (b->L.refill > 64) return fail(BRT_ERR_INVALID_ARGUMENT, "refill must be 0..64");
+  if (b->L.block_size && (b->L.block_size % 64 || b->L.block_size > 256))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
+  if (hipGetDevice(&b->device) != hipSuccess) return fail(BRT_ERR_NO_DEVICE, "no HIP device");
+  b->items.assign(items, items + count);
+  std::vector<BatchJob> jobs[NUM_TYPES];
+  for (uint32_t i = 0; i < count; i++) {
+    const brt_batch_item &it = items[i];
+    brt_scene *s = const_cast<brt_scene *>(it.scene);
+    if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "item without a scene");
+    if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+    if (it.n && (!it.rays.ox || !it.rays.oy || !it.rays.oz || !it.rays.dx || !it.rays.dy || !it.rays.dz ||
+                 !it.hits.t || !it.hits.prim_id || !it.hits.geom_id))
+      return fail(BRT_ERR_INVALID_ARGUMENT, "item without ray or hit arrays");
+    if ((it.hits.nx != nullptr) != (it.hits.ny != nullptr) || (it.hits.nx != nullptr) != (it.hits.nz != nullptr))
+      return fail(BRT_ERR_INVALID_ARGUMENT, "normal arrays must be all set or all NULL");
+    int rc = upload(s);
+    if (rc) return rc;
+    if (it.n == 0) continue;
+    bool first = true;
+    for (uint32_t t = 0; t < NUM_TYPES; t++) {
+      const Geometry &g = s->geo[t];
+      if (!g.present || g.boxes.n == 0) continue;
+      if (b->L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT && !g.extent_ok)
+        return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
+      BatchJob j{};
+      j.bvh = s->dev->views[t];
+      j.rays = RaysView{it.rays.ox, it.rays.oy, it.rays.oz, it.rays.dx, it.rays.dy, it.rays.dz, it.rays.tmin,
+                        it.rays.tmax};
+      j.hits = HitsView{it.hits.t,  it.hits.u,  it.hits.v,  it.hits.prim_id, it.hits.geom_id,
+                        it.hits.nx, it.hits.ny, it.hits.nz, nullptr};
+      j.n = it.n;
+      j.offset = b->total[t];
+      j.first = first ? 1u : 0u;
+      j.is_tri = t == TRIANGLES ? 1u : 0u;
+      j.extent = g.extent;
+      j.order = nullptr;
+      b->total[t] += it.n;
+      jobs[t].push_back(j);
+      first = false;
+    }
+    if (first) b->empty_items.push_back(i);
+  }
+  for (uint32_t t = 0; t < NUM_TYPES; t++) {
+    b->njobs[t] = (uint32_t)jobs[t].size();
+    if (jobs[t].empty()) continue;
+    hipError_t e = hipMalloc(&b->d_jobs[t], jobs[t].size() * sizeof(BatchJob));
+    if (e == hipSuccess)
+      e = hipMemcpy(b->d_jobs[t], jobs[t].data(), jobs[t].size() * sizeof(BatchJob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("batch table: ") + hipGetErrorString(e));
+  }
+  if (hipMalloc(reinterpret_cast<void **>(&b->counters), NUM_TYPES * sizeof(unsigned long long)) != hipSuccess)
+    return fail(BRT_ERR_OUT_OF_MEMORY, "batch counters");
+  *out = b.release();
+  return BRT_OK;
+}
+
+int brt_batch_launch(brt_batch *b, void *stream) {
+  if (!b) return fail(BRT_ERR_INVALID_ARGUMENT, "batch is NULL");
+  return batch_enqueue(b, static_cast<hipStream_t>(stream));
+}
+
+int brt_batch_capture(brt_batch *b, void *stream) {
+  if (!b || !stream) return fail(BRT_ERR_INVALID_ARGUMENT, "batch and a non-default stream are required");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("begin capture: ") + hipGetErrorString(e));
+  const int rc = batch_enqueue(b, st);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(st, &g);
+  if (rc != BRT_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("end capture: ") + hipGetErrorString(e));
+  b->graph = g;
+  e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+  return BRT_OK;
+}
+
+int brt_batch_replay(brt_batch *b, void *stream) {
+  if (!b || !b->exec) return fail(BRT_ERR_STATE, "batch has no captured graph");
+  hipError_t e = hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
+  return BRT_OK;
+}
+
+int brt_batch_destroy(brt_batch *b) {
+  delete b;
   return BRT_OK;
 }
 

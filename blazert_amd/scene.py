@@ -123,7 +123,8 @@ class Scene:
     # -------------------------------------------------------------------------------------------- batched
     def intersect(self, org, dirs, tmin=None, tmax=None, *, precision: int = capi.PRECISION_FP64,
                   any_hit: bool = False, normals: bool = False, hits: dict | None = None, block_size: int = 0,
-                  schedule: int = capi.SCHEDULE_PERSISTENT, refill: int = 0, stream=None) -> dict:
+                  schedule: int = capi.SCHEDULE_PERSISTENT, refill: int = 0, sort: int = capi.SORT_NONE,
+                  stream=None) -> dict:
         """Batched intersect1 over n rays. org / dirs: (n,3) float64 CUDA tensors or tuples of three columns.
 
         Returns a dict of CUDA tensors t, u, v (float64), prim_id, geom_id (int32 holding uint32 bits) and,
@@ -153,11 +154,89 @@ class Scene:
         H = capi.Hits(hits["t"].data_ptr(), hits["u"].data_ptr(), hits["v"].data_ptr(), hits["prim_id"].data_ptr(),
                       hits["geom_id"].data_ptr(), *([nrm[k].data_ptr() for k in range(3)] if nrm is not None
                                                     else [None, None, None]))
-        L = capi.Launch(precision, flags, block_size, schedule, refill)
+        L = capi.Launch(precision, flags, block_size, schedule, refill, sort)
         st = stream if stream is not None else torch.cuda.current_stream(dev)
         keep = (cols_o, cols_d, tmn, tmx)  # noqa: F841 (inputs alive until the launch is enqueued)
         check(lib().brt_intersect(self._h, C.byref(rays), C.byref(H), n, C.byref(L), C.c_void_p(st.cuda_stream)))
         return hits
+
+
+def _alloc_hits(n, dev, normals):
+    import torch
+    hits = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in ("t", "u", "v")}
+    hits["prim_id"] = torch.empty(n, dtype=torch.int32, device=dev)
+    hits["geom_id"] = torch.empty(n, dtype=torch.int32, device=dev)
+    if normals:
+        hits["normal"] = torch.empty((3, n), dtype=torch.float64, device=dev)
+    return hits
+
+
+def _hits_struct(hits):
+    nrm = hits.get("normal")
+    return capi.Hits(hits["t"].data_ptr(), hits["u"].data_ptr(), hits["v"].data_ptr(), hits["prim_id"].data_ptr(),
+                     hits["geom_id"].data_ptr(), *([nrm[k].data_ptr() for k in range(3)] if nrm is not None
+                                                   else [None, None, None]))
+
+
+class Batch:
+    """Many (scene, rays) items traced together: one persistent launch per geometry type over all items
+    (brt_batch_* in the C ABI). ``items`` is a list of (Scene, origins, directions[, tmin, tmax]); origins and
+    directions as for ``Scene.intersect``. ``hits[i]`` holds item i's results after a launch or replay."""
+
+    def __init__(self, items, *, precision: int = capi.PRECISION_FP64, any_hit: bool = False,
+                 normals: bool = False, block_size: int = 0, refill: int = 0):
+        self._keep = []
+        self.scenes = []
+        self.hits = []
+        arr = (capi.BatchItem * max(1, len(items)))()
+        for i, it in enumerate(items):
+            scene, org, dirs = it[0], it[1], it[2]
+            co, cd = _columns(org), _columns(dirs)
+            n = co[0].numel()
+            dev = co[0].device
+            tmn = _scalar_col(it[3], n, dev) if len(it) > 3 and it[3] is not None else None
+            tmx = _scalar_col(it[4], n, dev) if len(it) > 4 and it[4] is not None else None
+            h = _alloc_hits(n, dev, normals)
+            self._keep.extend([co, cd, tmn, tmx])
+            self.scenes.append(scene)
+            self.hits.append(h)
+            arr[i].scene = scene.handle
+            arr[i].rays = capi.Rays(*[c.data_ptr() for c in co], *[c.data_ptr() for c in cd],
+                                    tmn.data_ptr() if tmn is not None else None,
+                                    tmx.data_ptr() if tmx is not None else None)
+            arr[i].hits = _hits_struct(h)
+            arr[i].n = n
+        L = capi.Launch(precision, capi.FLAG_ANY_HIT if any_hit else 0, block_size, capi.SCHEDULE_PERSISTENT, refill,
+                        capi.SORT_NONE)
+        h = C.c_void_p()
+        check(lib().brt_batch_create(arr, len(items), C.byref(L), C.byref(h)))
+        self._h = h
+
+    def _stream(self, stream):
+        import torch
+        return stream if stream is not None else torch.cuda.current_stream()
+
+    def launch(self, stream=None):
+        check(lib().brt_batch_launch(self._h, C.c_void_p(self._stream(stream).cuda_stream)))
+
+    def capture(self, stream):
+        """Records the launch sequence on `stream` (not the default stream) into a hipGraph."""
+        check(lib().brt_batch_capture(self._h, C.c_void_p(stream.cuda_stream)))
+
+    def replay(self, stream=None):
+        check(lib().brt_batch_replay(self._h, C.c_void_p(self._stream(stream).cuda_stream)))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h:
+            try:
+                lib().brt_batch_destroy(h)
+            except Exception:
+                pass
+
+    def __del__(self):
+        self.close()
 
 
 def _columns(x):

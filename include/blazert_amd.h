@@ -99,13 +99,21 @@ typedef struct brt_hits {
 #define BRT_SCHEDULE_PERSISTENT 0u /* persistent waves; idle lanes pull new rays from a device counter */
 #define BRT_SCHEDULE_PER_THREAD 1u /* one ray per thread */
 
+/* ray order of a launch: sorting groups rays that take the same child order (direction octant) and start close
+ * together (Morton code of the origin in the scene box). Uses a per-(scene, stream) device workspace that is grown
+ * with hipMalloc on the first sorted launch of a given size, so that first launch must not be graph-captured. */
+#define BRT_SORT_NONE 0u
+#define BRT_SORT_OCTANT_MORTON 1u
+#define BRT_SORT_MORTON 2u
+
 typedef struct brt_launch {
   uint32_t precision;  /* BRT_PRECISION_* */
   uint32_t flags;      /* BRT_FLAG_* */
-  uint32_t block_size; /* threads per block, multiple of 64 up to 1024; 0 = library default (256) */
+  uint32_t block_size; /* threads per block: 64, 128, 192 or 256; 0 = library default (256) */
   uint32_t schedule;   /* BRT_SCHEDULE_* */
   uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default */
-  uint32_t reserved[3];
+  uint32_t sort;       /* BRT_SORT_*: trace rays in a sorted order (results are written in input order) */
+  uint32_t reserved[2];
 } brt_launch;
 
 /* One ray / hit in host memory, for the single-ray drop-in (RayHit<T>, blazert/ray.h:72-85). */
@@ -162,6 +170,28 @@ int brt_scene_export_tree(const brt_scene *scene, uint32_t geom_type, brt_tree_n
  * 256 launches of one scene in flight at once on different streams would share a counter. */
 int brt_intersect(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,
                   const brt_launch *launch, void *stream);
+
+/* ---- many scenes in one launch ------------------------------------------------------------------------------
+ * A batch binds (scene, device rays, device hits, n) items; every geometry type present in any scene is traced by
+ * one persistent launch over the concatenated rays of all items, with each hit merged into its own item exactly as
+ * brt_intersect would. The batch keeps the device pointers: they must stay valid while it is launched.
+ * brt_batch_capture records the launch sequence on `stream` (a non-default stream) into a hipGraph owned by the
+ * batch; brt_batch_replay launches that graph. No allocation happens after brt_batch_create. */
+typedef struct brt_batch brt_batch;
+
+typedef struct brt_batch_item {
+  const brt_scene *scene;
+  brt_rays rays;
+  brt_hits hits;
+  uint64_t n;
+} brt_batch_item;
+
+/* launch may be NULL; its sort field must be BRT_SORT_NONE. Scenes are uploaded to the current device. */
+int brt_batch_create(const brt_batch_item *items, uint32_t count, const brt_launch *launch, brt_batch **out);
+int brt_batch_launch(brt_batch *batch, void *stream);
+int brt_batch_capture(brt_batch *batch, void *stream);
+int brt_batch_replay(brt_batch *batch, void *stream);
+int brt_batch_destroy(brt_batch *batch);
 
 /* Single-ray drop-in for intersect1(scene, Ray<T>{org, dir, tmin, tmax}, rayhit): host memory in/out, blocking.
  * hit is in/out exactly like RayHit<T>; *hit_out receives the bool result. */

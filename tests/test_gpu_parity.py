@@ -29,7 +29,7 @@ def dev():
     return torch.device("cuda:0")
 
 
-def gpu_intersect(scene, o, d, dev, tmin=None, tmax=None, precision=0, any_hit=False, normals=True):
+def gpu_intersect(scene, o, d, dev, tmin=None, tmax=None, precision=0, any_hit=False, normals=True, **launch):
     O = torch.from_numpy(np.ascontiguousarray(o)).to(dev)
     D = torch.from_numpy(np.ascontiguousarray(d)).to(dev)
     kw = {}
@@ -37,7 +37,7 @@ def gpu_intersect(scene, o, d, dev, tmin=None, tmax=None, precision=0, any_hit=F
         kw["tmin"] = torch.from_numpy(np.ascontiguousarray(tmin, dtype=np.float64)).to(dev)
     if tmax is not None:
         kw["tmax"] = torch.from_numpy(np.ascontiguousarray(tmax, dtype=np.float64)).to(dev)
-    h = scene.intersect(O, D, precision=precision, any_hit=any_hit, normals=normals, **kw)
+    h = scene.intersect(O, D, precision=precision, any_hit=any_hit, normals=normals, **kw, **launch)
     torch.cuda.synchronize()
     out = {k: h[k].cpu().numpy() for k in ("t", "u", "v")}
     out["prim_id"] = h["prim_id"].cpu().numpy().view(np.uint32)
@@ -153,6 +153,26 @@ def test_icosphere8_incoherent_subset(dev, ico8):
     ok = ref["trace"]["overflow"] == 0
     assert ok.mean() > 0.999
     assert_bit_exact(g, ref, ok)
+
+
+@pytest.mark.parametrize("sort", [1, 2])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_icosphere8_sorted_launch(dev, ico8, sort, precision):
+    """Sorted launches trace rays in another order but write each hit to its own ray: same bits as the oracle."""
+    p, o = ico8
+    org, dirs = W.icosphere_incoherent_rays(1 << 18)
+    g = gpu_intersect(p, org, dirs, dev, precision=precision, sort=sort)
+    ref = o.intersect_batch(org, dirs, trace=True)
+    ok = (ref["trace"]["overflow"] == 0) & ((ref["trace"]["tie"] == 0) | (precision == 0))
+    assert_bit_exact(g, ref, ok)
+
+
+@pytest.mark.parametrize("schedule,refill,block", [(1, 0, 64), (0, 1, 64), (0, 64, 128), (0, 16, 192)])
+def test_launch_configurations_identical(dev, ico8, schedule, refill, block):
+    p, o = ico8
+    org, dirs = W.icosphere_incoherent_rays(1 << 16, seed=21)
+    g = gpu_intersect(p, org, dirs, dev, schedule=schedule, refill=refill, block_size=block)
+    assert_bit_exact(g, o.intersect_batch(org, dirs))
 
 
 def test_icosphere8_coherent_subset(dev, ico8):
@@ -306,3 +326,81 @@ def test_empty_and_single_prim_scenes(dev, oracle_mod):
     assert (g["geom_id"] == INVALID).all() and (g["t"] == np.finfo(np.float64).max).all()
     p, o = make_pair(oracle_mod, lambda x: x.add_spheres(np.zeros((1, 3)), [1.0]))
     assert_bit_exact(gpu_intersect(p, org, dirs, dev), o.intersect_batch(org, dirs))
+
+
+# ------------------------------------------------------------------------------------------------ batches
+@pytest.fixture(scope="module")
+def batch_set(oracle_mod):
+    """Eight uneven scenes (icospheres, sphere fields and a mixed scene) with uneven ray batches."""
+    from blazert_amd import Scene
+    rng = W.Rng(77)
+    items = []
+    for i in range(8):
+        kind = i % 3
+        if kind == 0:
+            v, f = W.icosphere(3 + i % 4, seed=i)
+            fill = (lambda v, f: lambda s: s.add_mesh(v * 2.0, f))(v, f)
+        elif kind == 1:
+            c, r = W.sphere_field(200 + 300 * i, seed=i)
+            fill = (lambda c, r: lambda s: s.add_spheres(c * 0.2, r * 0.2))(c, r)
+        else:
+            m = W.mixed_scene(n_tri_target=3000, n_sph=500, n_cyl=100, n_pla=10, seed=i)
+            fill = (lambda m: lambda s: (s.add_mesh(*m["mesh"]), s.add_spheres(*m["spheres"]),
+                                         s.add_cylinders(*m["cylinders"]), s.add_planes(*m["planes"])))(m)
+        p = Scene()
+        fill(p)
+        p.commit()
+        o = oracle_mod.Scene()
+        fill(o)
+        o.commit()
+        n = int(2 ** (10 + 6 * rng.uniform(1)[0]))
+        org, dirs = W.incoherent_rays(n, seed=100 + i, box=4.0 if kind != 2 else 22.0)
+        items.append((p, o, org, dirs))
+    empty = Scene()
+    empty.commit()
+    org, dirs = W.incoherent_rays(777, seed=5, box=2.0)
+    items.append((empty, None, org, dirs))
+    return items
+
+
+def _batch_check(batch, items, precision):
+    for (p, o, org, dirs), h in zip(items, batch.hits):
+        g = {k: h[k].cpu().numpy() for k in ("t", "u", "v")}
+        g["prim_id"] = h["prim_id"].cpu().numpy().view(np.uint32)
+        g["geom_id"] = h["geom_id"].cpu().numpy().view(np.uint32)
+        if o is None:
+            assert (g["geom_id"] == INVALID).all()
+            continue
+        ref = o.intersect_batch(org, dirs, trace=True)
+        tr = ref["trace"]
+        ok = tr["overflow"] == 0
+        if precision:
+            ok &= (tr["tie"] == 0) & (tr["multi_cyl"] == 0)
+        assert_bit_exact(g, ref, ok, normals=False)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_launch_matches_oracle(dev, batch_set, precision):
+    from blazert_amd import Batch
+    items = [(p, torch.from_numpy(org).to(dev), torch.from_numpy(dirs).to(dev)) for p, o, org, dirs in batch_set]
+    b = Batch(items, precision=precision)
+    b.launch()
+    torch.cuda.synchronize()
+    _batch_check(b, batch_set, precision)
+
+
+def test_batch_graph_replays(dev, batch_set):
+    """A captured hipGraph replayed several times, with eager work in between, gives the eager result each time."""
+    from blazert_amd import Batch
+    items = [(p, torch.from_numpy(org).to(dev), torch.from_numpy(dirs).to(dev)) for p, o, org, dirs in batch_set]
+    b = Batch(items, precision=1)
+    s = torch.cuda.Stream()
+    b.capture(s)
+    for rep in range(3):
+        for h in b.hits:
+            h["t"].fill_(-1.0)
+            h["prim_id"].fill_(-7)
+        torch.cuda.synchronize()
+        b.replay(s)
+        s.synchronize()
+        _batch_check(b, batch_set, 1)

@@ -163,3 +163,17 @@ def test_tree_invariants():
         assert np.array_equal(lo, nodes["min"][i]) and np.array_equal(hi, nodes["max"][i])
     leaves = nodes[nodes["leaf"] == 1]
     assert (leaves["prim_count"] <= 4).all() or p.build_stats(0)["max_tree_depth"] >= 28
+
+
+def test_launch_options_validated_on_host():
+    """Invalid launch options are rejected before any device work (block sizes above the kernels' bound)."""
+    import ctypes as C
+    s = Scene()
+    s.add_spheres(np.zeros((1, 3)), [1.0])
+    s.commit()
+    rays = capi.Rays(1, 1, 1, 1, 1, 1, None, None)  # never dereferenced: validation fails first
+    hits = capi.Hits(1, 1, 1, 1, 1, None, None, None)
+    for L in (capi.Launch(0, 0, 512, 0, 0, 0), capi.Launch(0, 0, 96, 0, 0, 0), capi.Launch(7, 0, 0, 0, 0, 0),
+              capi.Launch(0, 0, 0, 5, 0, 0), capi.Launch(0, 0, 0, 0, 65, 0), capi.Launch(0, 0, 0, 0, 0, 9)):
+        rc = capi.lib().brt_intersect(s.handle, C.byref(rays), C.byref(hits), 10, C.byref(L), None)
+        assert rc == capi.ERR_INVALID_ARGUMENT, (L.block_size, rc)
