@@ -167,8 +167,12 @@ def main():
     exact = bool(np.array_equal(g_sub_t[ok].view(np.uint64), ref["t"][ok].view(np.uint64))
                  and np.array_equal(g_sub_p[ok], ref["prim_id"][ok]))
     achieved = bpr64 * n / (ms64 * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_fp64_icosphere.json")
+    if os.path.exists(tfile):  # rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel, copy-calibrated (see the file)
+        traffic = json.load(open(tfile)).get("traffic_bytes")
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                        "bytes_per_ray": round(bpr64, 1), "node_visits_per_ray": round(visits, 2),
                        "tri_tests_per_ray": round(ptests, 2)}
     out["parity"] = {"sample_rays": ns, "fp64_bit_exact_vs_oracle": exact,

@@ -978,7 +978,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysView &rays, const HitsVie
     const uint64_t need = (n + block - 1) / block;
     if (blocks > need) blocks = need;
     if (blocks == 0) return hipSuccess;
-    const uint32_t refill = cfg.refill ? cfg.refill : (SPLIT ? 16u : 8u);
+    const uint32_t refill = cfg.refill ? cfg.refill : 16u;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
     hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
                        hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill, cfg.order);
@@ -1009,7 +1009,7 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
   const uint64_t need = (total + block - 1) / block;
   if (blocks > need) blocks = need;
   if (blocks == 0) return hipSuccess;
-  const uint32_t refill = cfg.refill ? cfg.refill : (SPLIT ? 16u : 8u);
+  const uint32_t refill = cfg.refill ? cfg.refill : 16u;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
   hipLaunchKernelGGL((k_trace_batch<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, jobs, njobs, total,
                      cfg.flags, cfg.counter, refill);
