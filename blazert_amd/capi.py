@@ -72,6 +72,15 @@ class Hit1(C.Structure):
                 ("prim_id", C.c_uint32), ("geom_id", C.c_uint32)]
 
 
+class MirrorOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("t", "u", "v", "prim_id", "px", "py", "pz", "rx", "ry", "rz")]
+
+
+class MirrorGrads(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("gt", "gpx", "gpy", "gpz", "vertices", "gox", "goy", "goz", "gdx", "gdy",
+                                          "gdz")]
+
+
 # every entry point declared in include/blazert_amd.h: name -> (restype, argtypes)
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -98,6 +107,10 @@ SIGNATURES = {
     "brt_batch_capture": (C.c_int, [_P, _P]),
     "brt_batch_replay": (C.c_int, [_P, _P]),
     "brt_batch_destroy": (C.c_int, [_P]),
+    "brt_mirror_trace": (C.c_int, [_P, C.POINTER(Rays), C.c_uint64, C.c_uint32, C.c_double, C.POINTER(Launch),
+                                   C.POINTER(MirrorOut), C.c_void_p]),
+    "brt_mirror_adjoint": (C.c_int, [_P, C.POINTER(Rays), C.c_uint64, C.c_uint32, C.c_void_p, C.POINTER(MirrorGrads),
+                                     C.c_void_p]),
 }
 
 _lib = None

@@ -62,6 +62,27 @@ struct SplitParams {
 hipError_t launch_trace(GeomType type, const BvhView &bvh, const RaysView &rays, const HitsView &hits, uint64_t n,
                         const TraceLaunch &cfg, const SplitParams &split, hipStream_t stream);
 
+// Mirror tracing: the next bounce's rays from this bounce's hits (positions o + t normalize(d), mirror directions;
+// NaN for misses). pos / dir are x, y, z arrays of n doubles.
+hipError_t launch_mirror_step(const RaysView &rays, const HitsView &hits, const double *vertices,
+                              const uint32_t *faces, uint64_t n, double *const pos[3], double *const dir[3],
+                              hipStream_t stream);
+
+// Upstream gradients in, parameter gradients out, of the mirror-tracing adjoint (any pointer may be NULL).
+struct MirrorGrads {
+  const double *gt;              // dL/dt, bounces * n
+  const double *gpx, *gpy, *gpz;  // dL/d(hit position), bounces * n each
+  double *gv;                    // dL/dvertex, 3 per vertex, accumulated
+  double *gox, *goy, *goz;       // dL/d(ray origin), n each, written
+  double *gdx, *gdy, *gdz;       // dL/d(raw ray direction), n each, written
+};
+
+// Adjoint of `bounces` (1..8) mirror bounces recomputed from the stored face ids prims[b * n + i]; a ray's chain
+// ends at its first id >= num_faces (a miss).
+hipError_t launch_mirror_adjoint(const RaysView &rays, const uint32_t *prims, uint32_t bounces,
+                                 const double *vertices, const uint32_t *faces, uint32_t num_faces, uint64_t n,
+                                 const MirrorGrads &g, hipStream_t stream);
+
 // Writes the default RayHit (t = DBL_MAX, ids invalid, u = v = 0, normal 0) for n rays.
 hipError_t launch_init_hits(const HitsView &hits, uint64_t n, hipStream_t stream);
 

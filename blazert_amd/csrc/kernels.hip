@@ -67,8 +67,8 @@ __device__ __forceinline__ void make_ray(const RaysView &R, uint64_t i, Ray &r) 
     r.inv[k] = 1.0 / raw[k];
     r.sign |= (raw[k] < 0.0 ? 1u : 0u) << k;
   }
-  r.tmin = R.tmin ? R.tmin[i] : 0.0;
-  r.tmax = R.tmax ? R.tmax[i] : DBL_MAX;
+  r.tmin = R.tmin ? R.tmin[i] : R.tmin0;
+  r.tmax = R.tmax ? R.tmax[i] : R.tmax0;
   // With a finite origin, finite non-zero direction components (so 1/d is finite and non-zero), a non-NaN tmin and
   // finite node boxes (checked on the host), (b - o) * inv is never NaN: the slab chains reduce to plain max/min.
   bool ok = r.tmin == r.tmin;
@@ -715,7 +715,7 @@ __device__ __forceinline__ void finish_ray(const BvhView &bvh, const HitsView &h
     if (hits.u) hits.u[i] = s.u;
     if (hits.v) hits.v[i] = s.v;
     hits.prim_id[i] = pid;
-    hits.geom_id[i] = bvh.geom_id;
+    if (hits.geom_id) hits.geom_id[i] = bvh.geom_id;
     if (hits.nx) {
       double nrm[3];
       post_normal(reinterpret_cast<const Rec *>(bvh.prims), bvh, r, s, pid, nrm);
@@ -728,7 +728,7 @@ __device__ __forceinline__ void finish_ray(const BvhView &bvh, const HitsView &h
     if (hits.u) hits.u[i] = 0.0;
     if (hits.v) hits.v[i] = 0.0;
     hits.prim_id[i] = INVALID_ID;
-    hits.geom_id[i] = INVALID_ID;
+    if (hits.geom_id) hits.geom_id[i] = INVALID_ID;
     if (hits.nx) {
       hits.nx[i] = 0.0;
       hits.ny[i] = 0.0;
@@ -950,7 +950,7 @@ __global__ void k_init_hits(HitsView hits, uint64_t n) {
     if (hits.u) hits.u[i] = 0.0;
     if (hits.v) hits.v[i] = 0.0;
     hits.prim_id[i] = INVALID_ID;
-    hits.geom_id[i] = INVALID_ID;
+    if (hits.geom_id) hits.geom_id[i] = INVALID_ID;
     if (hits.nx) {
       hits.nx[i] = 0.0;
       hits.ny[i] = 0.0;

@@ -75,6 +75,7 @@ struct BvhView {
   const uint32_t *prim_ids;   // leaf order -> user primitive id
   const uint32_t *faces;      // triangles only: user faces (3 per primitive id)
   const double *vnormals;     // triangles only: vertex normals (3 per vertex)
+  const double *vertices;     // triangles only: user vertices (3 per vertex), for mirror tracing and its adjoint
   double root_min[3], root_max[3];
   float root_min32[3], root_max32[3];
   uint32_t root_ref, root_cnt;
@@ -85,9 +86,12 @@ struct BvhView {
   uint32_t nan_boxes;         // some node box coordinate is NaN (disables the NaN-free slab path)
 };
 
-// Structure-of-arrays rays / hits in device memory (mirrors brt_rays / brt_hits of the C ABI).
+// Structure-of-arrays rays / hits in device memory (mirrors brt_rays / brt_hits of the C ABI). tmin / tmax may be
+// NULL: every ray then uses tmin0 / tmax0.
 struct RaysView {
   const double *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+  double tmin0 = 0.0;
+  double tmax0 = 1.7976931348623157e308;
 };
 struct HitsView {
   double *t, *u, *v;

@@ -561,7 +561,7 @@ void build_geometry(Geometry &g, GeomType type, const brt_build_options &o) {
   flatten(g, type);
   s.build_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   s.device_bytes = g.n64.size() * sizeof(Node64) + g.n32.size() * sizeof(Node32) + g.recs.size() +
-                   g.order.size() * 4 + g.faces.size() * 4 + g.vnormals.size() * 8;
+                   g.order.size() * 4 + g.faces.size() * 4 + g.vnormals.size() * 8 + g.vertices.size() * 8;
 }
 
 // ------------------------------------------------------------------------------------------------ device
@@ -663,6 +663,7 @@ int upload(brt_scene *s) {
     if (t == TRIANGLES) {
       if (e == hipSuccess) e = d->put(g.faces, &v.faces);
       if (e == hipSuccess) e = d->put(g.vnormals, &v.vnormals);
+      if (e == hipSuccess) e = d->put(g.vertices, &v.vertices);
     }
     if (e != hipSuccess)
       return fail(e == hipErrorOutOfMemory ? BRT_ERR_OUT_OF_MEMORY : BRT_ERR_DEVICE,
@@ -672,6 +673,19 @@ int upload(brt_scene *s) {
   s->dev = std::move(d);
   return BRT_OK;
 }
+
+int mirror_check(brt_scene *s, uint64_t n, uint32_t bounces) {
+  if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  if (bounces < 1 || bounces > 8) return fail(BRT_ERR_INVALID_ARGUMENT, "bounces must be 1..8");
+  const Geometry &tri = s->geo[TRIANGLES];
+  if (!tri.present || tri.boxes.n == 0) return fail(BRT_ERR_UNSUPPORTED, "mirror tracing needs a triangle mesh");
+  for (uint32_t t = SPHERES; t < NUM_TYPES; t++)
+    if (s->geo[t].present && s->geo[t].boxes.n) return fail(BRT_ERR_UNSUPPORTED, "mirror tracing supports triangle meshes only");
+  if (n > 0xFFFFFFFFull / 8) return fail(BRT_ERR_UNSUPPORTED, "too many rays");
+  return upload(s);
+}
+
 
 }  // namespace
 
@@ -1077,6 +1091,79 @@ int brt_batch_replay(brt_batch *b, void *stream) {
 
 int brt_batch_destroy(brt_batch *b) {
   delete b;
+  return BRT_OK;
+}
+
+// ------------------------------------------------------------------------------------------------ mirror tracing
+int brt_mirror_trace(const brt_scene *cs, const brt_rays *rays, uint64_t n, uint32_t bounces, double bounce_tmin,
+                     const brt_launch *launch, const brt_mirror_out *out, void *stream) {
+  brt_scene *s = const_cast<brt_scene *>(cs);
+  if (!rays || !out) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "ray origin / direction arrays are required");
+  if (!out->t || !out->prim_id || !out->px || !out->py || !out->pz || !out->rx || !out->ry || !out->rz)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "t, prim_id, position and direction arrays are required");
+  if ((out->u != nullptr) != (out->v != nullptr)) return fail(BRT_ERR_INVALID_ARGUMENT, "u and v must both be set or NULL");
+  if (!(bounce_tmin >= 0.0)) return fail(BRT_ERR_INVALID_ARGUMENT, "bounce_tmin must be >= 0");
+  brt_launch L{};
+  if (launch) L = *launch;
+  if (L.precision != BRT_PRECISION_FP64) return fail(BRT_ERR_UNSUPPORTED, "mirror tracing runs the FP64 path");
+  if (L.flags) return fail(BRT_ERR_INVALID_ARGUMENT, "mirror tracing takes no launch flags");
+  if (L.schedule > BRT_SCHEDULE_PER_THREAD) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown schedule");
+  if (L.refill > 64) return fail(BRT_ERR_INVALID_ARGUMENT, "refill must be 0..64");
+  if (L.sort != BRT_SORT_NONE) return fail(BRT_ERR_UNSUPPORTED, "mirror tracing does not sort rays");
+  if (L.block_size && (L.block_size % 64 || L.block_size > 256))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
+  int rc = mirror_check(s, n, bounces);
+  if (rc) return rc;
+  if (n == 0) return BRT_OK;
+  const BvhView &bvh = s->dev->views[TRIANGLES];
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (uint32_t b = 0; b < bounces; b++) {
+    const uint64_t off = (uint64_t)b * n;
+    RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    if (b > 0) {
+      const uint64_t prev = off - n;
+      rv = RaysView{out->px + prev, out->py + prev, out->pz + prev, out->rx + prev, out->ry + prev, out->rz + prev,
+                    nullptr, nullptr};
+      rv.tmin0 = bounce_tmin;
+    }
+    HitsView hv{out->t + off, out->u ? out->u + off : nullptr, out->v ? out->v + off : nullptr, out->prim_id + off,
+                nullptr, nullptr, nullptr, nullptr, nullptr};
+    TraceLaunch cfg{L.precision, 0u, L.block_size, true, L.schedule, L.refill, nullptr, nullptr};
+    if (L.schedule == BRT_SCHEDULE_PERSISTENT)
+      cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
+    SplitParams sp{s->geo[TRIANGLES].extent};
+    hipError_t e = launch_trace(TRIANGLES, bvh, rv, hv, n, cfg, sp, st);
+    double *pos[3] = {out->px + off, out->py + off, out->pz + off};
+    double *dir[3] = {out->rx + off, out->ry + off, out->rz + off};
+    if (e == hipSuccess) e = launch_mirror_step(rv, hv, bvh.vertices, bvh.faces, n, pos, dir, st);
+    if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+  }
+  return BRT_OK;
+}
+
+int brt_mirror_adjoint(const brt_scene *cs, const brt_rays *rays, uint64_t n, uint32_t bounces,
+                       const uint32_t *prim_id, const brt_mirror_grads *grads, void *stream) {
+  brt_scene *s = const_cast<brt_scene *>(cs);
+  if (!rays || !prim_id || !grads) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "ray origin / direction arrays are required");
+  const bool gp = grads->gpx || grads->gpy || grads->gpz, go = grads->gox || grads->goy || grads->goz,
+             gd = grads->gdx || grads->gdy || grads->gdz;
+  if ((gp && !(grads->gpx && grads->gpy && grads->gpz)) || (go && !(grads->gox && grads->goy && grads->goz)) ||
+      (gd && !(grads->gdx && grads->gdy && grads->gdz)))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "x, y, z gradient arrays must be all set or all NULL");
+  int rc = mirror_check(s, n, bounces);
+  if (rc) return rc;
+  if (n == 0) return BRT_OK;
+  const BvhView &bvh = s->dev->views[TRIANGLES];
+  RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, nullptr, nullptr};
+  MirrorGrads g{grads->gt,  grads->gpx, grads->gpy, grads->gpz, grads->vertices, grads->gox,
+                grads->goy, grads->goz, grads->gdx, grads->gdy, grads->gdz};
+  hipError_t e = launch_mirror_adjoint(rv, prim_id, bounces, bvh.vertices, bvh.faces, bvh.n_prims, n, g,
+                                       static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
   return BRT_OK;
 }
 

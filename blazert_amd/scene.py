@@ -41,6 +41,7 @@ class Scene:
         check(lib().brt_scene_create(C.byref(opts), C.byref(h)))
         self._h = h
         self.has_been_committed = False
+        self.num_vertices = 0
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -69,7 +70,10 @@ class Scene:
     def add_mesh(self, vertices, triangles) -> int:
         v = _f64(vertices, 3)
         f = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1, 3)
-        return self._add(lib().brt_scene_add_mesh, _dp(v), len(v), _up(f), len(f))
+        gid = self._add(lib().brt_scene_add_mesh, _dp(v), len(v), _up(f), len(f))
+        if gid != INVALID_ID:
+            self.num_vertices = len(v)
+        return gid
 
     def add_spheres(self, centers, radii) -> int:
         c = _f64(centers, 3)

@@ -208,6 +208,35 @@ def mixed_rays(n: int = 8 * 1024 * 1024, seed: int = 13):
     return incoherent_rays(n, seed, 22.0)
 
 
+def box_mesh(half: float):
+    """Closed axis-aligned cube [-half, half]^3 as 8 vertices and 12 triangles."""
+    v = np.array([[x, y, z] for x in (-half, half) for y in (-half, half) for z in (-half, half)], dtype=np.float64)
+    f = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5],   # x = -h, x = +h
+                  [0, 4, 5], [0, 5, 1], [2, 3, 7], [2, 7, 6],   # y = -h, y = +h
+                  [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]],  # z = -h, z = +h
+                 dtype=np.uint32)
+    return v, f
+
+
+def mirror_box_scene(level: int = 8, half: float = 3.0, seed: int = 7):
+    """Displaced icosphere inside a 12-triangle box, merged into one mesh (box vertices and faces last)."""
+    sv, sf = icosphere(level, seed=seed)
+    bv, bf = box_mesh(half)
+    v = np.concatenate([sv, bv])
+    f = np.concatenate([sf, bf + len(sv)]).astype(np.uint32)
+    return np.ascontiguousarray(v), np.ascontiguousarray(f)
+
+
+def mirror_rays(n: int = 4 * 1024 * 1024, seed: int = 21):
+    """Incoherent rays for mirror tracing: origins uniform in [-2, 2]^3, directions uniform on the sphere."""
+    return incoherent_rays(n, seed, 2.0)
+
+
+def mirror_weights(n: int, bounces: int = 3, seed: int = 29):
+    """Per-ray, per-bounce loss weights w ~ U(0, 1), shape (bounces, n)."""
+    return Rng(seed).uniform(bounces * n).reshape(bounces, n)
+
+
 def scene_primitives():
     """examples/scene_primitives/main.cc:47-81 (two cylinders, then two spheres)."""
     rot = np.array([0, 0, 1, 0, 1, 0, -1, 0, 0], float)

@@ -193,6 +193,43 @@ int brt_batch_capture(brt_batch *batch, void *stream);
 int brt_batch_replay(brt_batch *batch, void *stream);
 int brt_batch_destroy(brt_batch *batch);
 
+/* ---- differentiable mirror tracing ----------------------------------------------------------------------------
+ * No counterpart in the reference: an extension of intersect1 (blazert/scene.h:161-211) on a scene whose only
+ * geometry is one triangle mesh. Bounce 0 traces `rays`; bounce b >= 1 starts at bounce b-1's hit position and
+ * follows the mirror direction of normalize(d) about the unit face normal normalize(cross(v1 - v0, v2 - v0))
+ * (blazert/primitives/trimesh.h:125-129), with tmin = bounce_tmin and tmax = DBL_MAX. A miss ends the ray: t =
+ * DBL_MAX, prim_id = BRT_INVALID_ID, position and direction NaN for that and every later bounce.
+ * Every array is bounces * n long, indexed [b * n + i], device memory. The traversal is the exact FP64 path, so
+ * each bounce's (t, u, v, prim_id) equals intersect1 on that bounce's ray. */
+typedef struct brt_mirror_out {
+  double *t;             /* hit distance along the unit direction */
+  double *u, *v;         /* barycentric hit coordinates; both NULL or both set */
+  uint32_t *prim_id;     /* face id */
+  double *px, *py, *pz;  /* hit position: the next bounce's origin */
+  double *rx, *ry, *rz;  /* mirror direction: the next bounce's direction */
+} brt_mirror_out;
+
+/* launch may be NULL; precision must be FP64, flags 0, sort NONE. Asynchronous on `stream`. */
+int brt_mirror_trace(const brt_scene *scene, const brt_rays *rays, uint64_t n, uint32_t bounces, double bounce_tmin,
+                     const brt_launch *launch, const brt_mirror_out *out, void *stream);
+
+/* Gradients of a loss L(t, positions) of brt_mirror_trace's outputs. In: dL/dt and dL/dposition per bounce (either
+ * may be NULL = 0). Out: dL/dvertex (3 per mesh vertex, ADDED to `vertices`, so clear it first), dL/d(origin) and
+ * dL/d(raw direction) per ray (written; NULL = not wanted). FP64 throughout; vertex gradients are scatter-added with
+ * atomics, so their last bits depend on the summation order. */
+typedef struct brt_mirror_grads {
+  const double *gt;
+  const double *gpx, *gpy, *gpz;
+  double *vertices;
+  double *gox, *goy, *goz;
+  double *gdx, *gdy, *gdz;
+} brt_mirror_grads;
+
+/* Recomputes every bounce of every ray from the stored face ids (prim_id from brt_mirror_trace, bounces * n) and the
+ * committed vertices; no other intermediate of the forward pass is needed. Asynchronous on `stream`. */
+int brt_mirror_adjoint(const brt_scene *scene, const brt_rays *rays, uint64_t n, uint32_t bounces,
+                       const uint32_t *prim_id, const brt_mirror_grads *grads, void *stream);
+
 /* Single-ray drop-in for intersect1(scene, Ray<T>{org, dir, tmin, tmax}, rayhit): host memory in/out, blocking.
  * hit is in/out exactly like RayHit<T>; *hit_out receives the bool result. */
 int brt_intersect1(const brt_scene *scene, const double org[3], const double dir[3], double tmin, double tmax,

@@ -118,6 +118,25 @@ uint64_t orc_intersect_batch(const orc_scene *s, uint64_t n, const double *ox, c
 /* brute-force closest candidate per type (no BVH): every primitive tested from a fresh intersector */
 int orc_bruteforce_type(const orc_scene *s, int type, const orc_ray *ray, orc_hit *hit);
 
+/* ---- multi-bounce mirror tracing (mirror_oracle.c; no counterpart in the reference) ----
+ * Arrays of per-bounce outputs and upstream gradients are bounces * n long, indexed [b * n + i]; per-ray gradient
+ * outputs go / gd are 3 * n long, [c * n + i]. gv (3 per vertex) is accumulated. */
+void orc_mirror_trace(const orc_scene *s, const double *V, const uint32_t *F, uint64_t n, const double *ox,
+                      const double *oy, const double *oz, const double *dx, const double *dy, const double *dz,
+                      const double *tmin, const double *tmax, uint32_t bounces, double bounce_tmin, double *t,
+                      double *u, double *v, uint32_t *prim, double *px, double *py, double *pz, double *rx,
+                      double *ry, double *rz, int nthreads);
+/* per-ray loss sum_b gt.t_b + gp.x_b of the chain with fixed face ids (t from the face plane) */
+void orc_mirror_loss(const double *V, const uint32_t *F, uint64_t n, const double *ox, const double *oy,
+                     const double *oz, const double *dx, const double *dy, const double *dz, uint32_t bounces,
+                     const uint32_t *prims, const double *gt, const double *gpx, const double *gpy,
+                     const double *gpz, double *loss, int nthreads);
+/* analytic gradient of the summed orc_mirror_loss (explicit Jacobians, barycentric dt/dvertex) */
+void orc_mirror_grad(const double *V, const uint32_t *F, uint64_t n, const double *ox, const double *oy,
+                     const double *oz, const double *dx, const double *dy, const double *dz, uint32_t bounces,
+                     const uint32_t *prims, const double *gt, const double *gpx, const double *gpy,
+                     const double *gpz, double *gv, double *go, double *gd, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,13 @@ def lib():
         L.orc_intersect_batch.argtypes = [P, C.c_uint64] + [dp] * 8 + [dp, dp, dp, up, up, dp, dp, dp, C.c_void_p,
                                                                         C.c_int, C.c_int]
         L.orc_bruteforce_type.argtypes = [P, C.c_int, C.POINTER(Ray), C.POINTER(Hit)]
+        L.orc_mirror_trace.restype = None
+        L.orc_mirror_trace.argtypes = [P, dp, up, C.c_uint64] + [dp] * 8 + [C.c_uint32, C.c_double, dp, dp, dp, up] + \
+            [dp] * 6 + [C.c_int]
+        L.orc_mirror_loss.restype = None
+        L.orc_mirror_loss.argtypes = [dp, up, C.c_uint64] + [dp] * 6 + [C.c_uint32, up] + [dp] * 5 + [C.c_int]
+        L.orc_mirror_grad.restype = None
+        L.orc_mirror_grad.argtypes = [dp, up, C.c_uint64] + [dp] * 6 + [C.c_uint32, up] + [dp] * 7 + [C.c_int]
         _lib = L
     return _lib
 
@@ -254,3 +261,67 @@ class Scene:
         if trace:
             out["trace"] = tr
         return out
+
+
+# ---------------------------------------------------------------------------------------------- mirror tracing
+def _cols(a, n):
+    a = np.asarray(a, dtype=np.float64).reshape(n, 3)
+    return [np.ascontiguousarray(a[:, k]) for k in range(3)]
+
+
+def _opt(a, shape):
+    return None if a is None else np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float64)
+
+
+def mirror_trace(scene: Scene, vertices, faces, org, dirs, bounces, bounce_tmin, tmin=None, tmax=None, nthreads=0):
+    """Multi-bounce mirror tracing on the CPU (mirror_oracle.c). Returns t, u, v, prim_id (bounces, n) and
+    position, direction (bounces, n, 3)."""
+    V = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+    F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+    n = len(np.asarray(org).reshape(-1, 3))
+    cols = _cols(org, n) + _cols(dirs, n)
+    tmn, tmx = _opt(tmin, (n,)), _opt(tmax, (n,))
+    B = int(bounces)
+    t, u, v = (np.zeros(B * n) for _ in range(3))
+    prim = np.zeros(B * n, dtype=np.uint32)
+    pr = [np.zeros(B * n) for _ in range(6)]
+    lib().orc_mirror_trace(scene.h, _dp(V), _up(F), n, *[_dp(c) for c in cols], _dp(tmn), _dp(tmx), B,
+                           float(bounce_tmin), _dp(t), _dp(u), _dp(v), _up(prim), *[_dp(c) for c in pr],
+                           int(nthreads))
+    return {"t": t.reshape(B, n), "u": u.reshape(B, n), "v": v.reshape(B, n), "prim_id": prim.reshape(B, n),
+            "position": np.stack(pr[:3], axis=-1).reshape(B, n, 3),
+            "direction": np.stack(pr[3:], axis=-1).reshape(B, n, 3)}
+
+
+def _grad_inputs(org, dirs, prim_id, gt, gp):
+    prim = np.ascontiguousarray(prim_id, dtype=np.uint32)
+    B, n = prim.shape
+    cols = _cols(org, n) + _cols(dirs, n)
+    g_t = _opt(gt, (B, n))
+    gps = [None, None, None] if gp is None else [np.ascontiguousarray(np.asarray(gp, dtype=np.float64)
+                                                                      .reshape(B, n, 3)[..., k]) for k in range(3)]
+    return prim, B, n, cols, g_t, gps
+
+
+def mirror_loss(vertices, faces, org, dirs, prim_id, gt=None, gp=None, nthreads=0):
+    """Per-ray loss sum_b gt[b]*t_b + gp[b].x_b of the bounce chain with the given (fixed) face ids."""
+    V = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+    F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+    prim, B, n, cols, g_t, gps = _grad_inputs(org, dirs, prim_id, gt, gp)
+    loss = np.zeros(n)
+    lib().orc_mirror_loss(_dp(V), _up(F), n, *[_dp(c) for c in cols], B, _up(prim), _dp(g_t),
+                          *[_dp(c) for c in gps], _dp(loss), int(nthreads))
+    return loss
+
+
+def mirror_grad(vertices, faces, org, dirs, prim_id, gt=None, gp=None, nthreads=0):
+    """Analytic FP64 gradient of sum(mirror_loss): (dL/dvertex (nv,3), dL/dorigin (n,3), dL/ddirection (n,3))."""
+    V = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+    F = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+    prim, B, n, cols, g_t, gps = _grad_inputs(org, dirs, prim_id, gt, gp)
+    gv = np.zeros(V.size)
+    go = np.zeros(3 * n)
+    gd = np.zeros(3 * n)
+    lib().orc_mirror_grad(_dp(V), _up(F), n, *[_dp(c) for c in cols], B, _up(prim), _dp(g_t),
+                          *[_dp(c) for c in gps], _dp(gv), _dp(go), _dp(gd), int(nthreads))
+    return gv.reshape(-1, 3), go.reshape(3, n).T.copy(), gd.reshape(3, n).T.copy()
