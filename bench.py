@@ -50,6 +50,80 @@ def time_launch(fn, steps, warmup):
     return float(np.mean(ms)), wall
 
 
+def classify_disagreements(osc, org, dirs, idx) -> dict:
+    """Re-runs the oracle (with its traversal trace) on the rays where the two precisions disagree and counts how
+    many are ties (a runner-up within 1e-12 relative), order-dependent cylinder rays or reference stack overflows."""
+    if len(idx) == 0:
+        return {"disagree": 0, "tie": 0, "cylinder": 0, "overflow": 0, "unexplained": 0}
+    ref = osc.intersect_batch(org[idx], dirs[idx], normals=False, trace=True)
+    tr = ref["trace"]
+    tie = tr["tie"] != 0
+    cyl = (tr["multi_cyl"] != 0) | (ref["geom_id"] == 3)
+    ovf = tr["overflow"] != 0
+    return {"disagree": int(len(idx)), "tie": int(tie.sum()), "cylinder": int((cyl & ~tie).sum()),
+            "overflow": int((ovf & ~tie & ~cyl).sum()), "unexplained": int((~tie & ~cyl & ~ovf).sum())}
+
+
+def split_agreement(h64, h32) -> tuple:
+    same = (h64["prim_id"] == h32["prim_id"]) & (h64["geom_id"] == h32["geom_id"])
+    hit = same & (h64["geom_id"] != -1)
+    rel = ((h64["t"] - h32["t"]).abs() / h64["t"].abs())[hit]
+    max_rel = float(rel.max().item()) if rel.numel() else 0.0
+    idx = (~same).nonzero().flatten().cpu().numpy()
+    return float(same.double().mean().item()), max_rel, idx
+
+
+def mixed_leg(args, dev) -> dict:
+    """Mixed-primitive scene (200k triangles as displaced icosphere patches merged into one mesh, 50k spheres, 10k
+    cylinders, 100 planes) x 8,388,608 incoherent rays: FP64 and FP32-traversal/FP64-hit throughput, hit-id
+    agreement with every disagreement classified by the oracle, and FP64 bit-exactness on a sample."""
+    import torch
+
+    from blazert_amd import PRECISION_FP32_TRAV_FP64_HIT, PRECISION_FP64, Scene, workloads as W
+    from oracle import oracle as OR
+
+    m = W.mixed_scene()
+    org, dirs = W.mixed_rays()
+    n = len(org)
+    scene = Scene()
+    osc = OR.Scene()
+    for s in (scene, osc):
+        s.add_mesh(*m["mesh"])
+        s.add_spheres(*m["spheres"])
+        s.add_cylinders(*m["cylinders"])
+        s.add_planes(*m["planes"])
+    t0 = time.perf_counter()
+    scene.commit()
+    scene.upload()
+    build_s = time.perf_counter() - t0
+    osc.commit()
+    O = tuple(torch.from_numpy(np.ascontiguousarray(org[:, k])).to(dev) for k in range(3))
+    D = tuple(torch.from_numpy(np.ascontiguousarray(dirs[:, k])).to(dev) for k in range(3))
+    steps = max(1, args.steps // 2)
+    ms64, _ = time_launch(lambda: scene.intersect(O, D, precision=PRECISION_FP64), steps, args.warmup)
+    ms32, _ = time_launch(lambda: scene.intersect(O, D, precision=PRECISION_FP32_TRAV_FP64_HIT), steps, args.warmup)
+    h64 = scene.intersect(O, D, precision=PRECISION_FP64)
+    h32 = scene.intersect(O, D, precision=PRECISION_FP32_TRAV_FP64_HIT)
+    torch.cuda.synchronize()
+    agree, max_rel, idx = split_agreement(h64, h32)
+    log(f"mixed: fp64 {n / ms64 / 1e3:.1f} / fp32-trav {n / ms32 / 1e3:.1f} Mrays/s, agreement {agree:.7f}")
+    ns = 1 << 18
+    ref = osc.intersect_batch(org[:ns], dirs[:ns], normals=False, trace=True,
+                              nthreads=len(os.sched_getaffinity(0)))
+    ok = ref["trace"]["overflow"] == 0
+    gt = h64["t"][:ns].cpu().numpy()
+    gp = h64["prim_id"][:ns].cpu().numpy().view(np.uint32)
+    gg = h64["geom_id"][:ns].cpu().numpy().view(np.uint32)
+    exact = bool(np.array_equal(gt[ok].view(np.uint64), ref["t"][ok].view(np.uint64))
+                 and np.array_equal(gp[ok], ref["prim_id"][ok]) and np.array_equal(gg[ok], ref["geom_id"][ok]))
+    return {"workload": f"{len(m['mesh'][1])} tris + 50,000 spheres + 10,000 cylinders + 100 planes x {n} "
+                        "incoherent rays (origins U[-22,22]^3)",
+            "fp64_mrays_s": round(n / ms64 / 1e3, 3), "fp32_trav_mrays_s": round(n / ms32 / 1e3, 3),
+            "speedup": round(ms64 / ms32, 3), "hit_id_agreement": agree, "max_rel_dt_agreeing": max_rel,
+            "disagreements": classify_disagreements(osc, org, dirs, idx), "commit_s": round(build_s, 3),
+            "fp64_bit_exact_vs_oracle_sample": exact, "sample_rays": ns}
+
+
 def adjoint_leg(args, dev) -> dict:
     """Differentiable 3-bounce mirror tracing: the 1,310,720-triangle icosphere inside a 12-triangle box (one merged
     mesh), 4,194,304 rays, loss = sum over bounces of w * t with w ~ U(0,1). Times the forward (3 traced bounces) and
@@ -190,6 +264,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=1 << 22)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the mirror-tracing adjoint leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-primitive split-precision leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -250,11 +325,7 @@ def main():
     h64 = scene.intersect(O, D, precision=PRECISION_FP64)
     h32 = scene.intersect(O, D, precision=PRECISION_FP32_TRAV_FP64_HIT)
     torch.cuda.synchronize()
-    same = (h64["prim_id"] == h32["prim_id"]) & (h64["geom_id"] == h32["geom_id"])
-    agree = float(same.double().mean().item())
-    hit = same & (h64["geom_id"] != -1)
-    rel = ((h64["t"] - h32["t"]).abs() / h64["t"].abs())[hit]
-    max_rel = float(rel.max().item()) if rel.numel() else 0.0
+    agree, max_rel, dis_idx = split_agreement(h64, h32)
 
     out = {
         "metric": METRIC,
@@ -310,8 +381,11 @@ def main():
                        "tri_tests_per_ray": round(ptests, 2)}
     out["parity"] = {"sample_rays": ns, "fp64_bit_exact_vs_oracle": exact,
                      "reference_stack_overflow_rays": int((~ok).sum())}
+    out["split_precision"]["disagreements"] = classify_disagreements(osc, org, dirs, dis_idx)
+    del h64, h32, O, D, CO, CD
+    if not args.no_mixed:
+        out["mixed_split_precision"] = mixed_leg(args, dev)
     if not args.no_adjoint:
-        del h64, h32, O, D, CO, CD
         out["adjoint"] = adjoint_leg(args, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = {"value": round(ns / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,

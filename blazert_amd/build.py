@@ -1,6 +1,7 @@
 """Builds libblazert_amd.so in-tree with hipcc for gfx950.
 
-Sources: csrc/kernels.hip and csrc/mirror.hip (device code), csrc/scene.cpp (host builder + C ABI). Both are compiled with
+Sources: csrc/kernels.hip and csrc/mirror.hip (device code), csrc/scene.cpp (host builder + C ABI) and
+csrc/autotune.cpp (launch / leaf-size autotuner). Both are compiled with
 -ffp-contract=off so that no multiply-add is fused (the results must round exactly like the reference's
 separate multiply and add), and device code with packed-FP32 instructions disabled.
 """
@@ -48,9 +49,11 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
             obj = os.path.join(tmp, src.replace(".hip", ".o"))
             _run([cc, *COMMON, *DEVICE, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj])
             objs.append(obj)
-        s_obj = os.path.join(tmp, "scene.o")
-        _run([cc, *COMMON, "-x", "c++", "-pthread", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-c", os.path.join(CSRC, "scene.cpp"), "-o", s_obj])
-        objs.append(s_obj)
+        for src in ("scene.cpp", "autotune.cpp"):
+            obj = os.path.join(tmp, src.replace(".cpp", ".o"))
+            _run([cc, *COMMON, "-x", "c++", "-pthread", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-c",
+                  os.path.join(CSRC, src), "-o", obj])
+            objs.append(obj)
         out_tmp = out + ".tmp"
         _run([cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-pthread", *objs, "-o", out_tmp])
         os.replace(out_tmp, out)

@@ -60,7 +60,8 @@ SORT_MORTON = 2
 
 class Launch(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("flags", C.c_uint32), ("block_size", C.c_uint32),
-                ("schedule", C.c_uint32), ("refill", C.c_uint32), ("sort", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("schedule", C.c_uint32), ("refill", C.c_uint32), ("sort", C.c_uint32), ("stack_ring", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class BatchItem(C.Structure):
@@ -79,6 +80,11 @@ class MirrorOut(C.Structure):
 class MirrorGrads(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("gt", "gpx", "gpy", "gpz", "vertices", "gox", "goy", "goz", "gdx", "gdy",
                                           "gdz")]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("launch", Launch), ("min_leaf_primitives", C.c_uint32), ("candidates", C.c_uint32),
+                ("best_ms", C.c_double), ("default_ms", C.c_double)]
 
 
 # every entry point declared in include/blazert_amd.h: name -> (restype, argtypes)
@@ -107,6 +113,11 @@ SIGNATURES = {
     "brt_batch_capture": (C.c_int, [_P, _P]),
     "brt_batch_replay": (C.c_int, [_P, _P]),
     "brt_batch_destroy": (C.c_int, [_P]),
+    "brt_autotune_launch": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_uint64, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, C.POINTER(TuneResult), C.c_void_p]),
+    "brt_batch_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(TuneResult), C.c_void_p]),
+    "brt_scene_tune_leaf": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_uint64, C.c_uint32, _U, C.c_uint32,
+                                      C.c_uint32, C.POINTER(TuneResult), C.c_void_p]),
     "brt_mirror_trace": (C.c_int, [_P, C.POINTER(Rays), C.c_uint64, C.c_uint32, C.c_double, C.POINTER(Launch),
                                    C.POINTER(MirrorOut), C.c_void_p]),
     "brt_mirror_adjoint": (C.c_int, [_P, C.POINTER(Rays), C.c_uint64, C.c_uint32, C.c_void_p, C.POINTER(MirrorGrads),

@@ -29,7 +29,11 @@ constexpr double L1 = 1.0 + 4.0 * DBL_EPSILON;  // far-plane widening of blazert
 constexpr uint32_t PREC_FP64 = 0;
 constexpr uint32_t FLAG_INOUT = 1u;
 constexpr uint32_t FLAG_ANY_HIT = 2u;
-constexpr int STACK_CAP = 64;
+// Deepest stack a traversal can build: the binary FP64 DFS pushes at most one entry per level (trees are limited
+// to depth 63 at upload); the 4-wide FP32 DFS at most three per Node4 level, and a Node4 level spans two binary
+// levels, so at most 3 * 32 = 96.
+template <bool SPLIT>
+constexpr int stack_cap() { return SPLIT ? 128 : 64; }
 
 __device__ __forceinline__ double std_min(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double std_max(double a, double b) { return (a < b) ? b : a; }
@@ -521,16 +525,17 @@ struct Trav {
   float h32;          // split precision: hit distance rounded up to FP32
 };
 
-// Traversal stack: the top RING entries of each lane live in LDS (slot-major, lane-minor, so a wave's accesses
-// spread over all banks); older entries spill to private memory only when a ray goes deeper than RING, and come
+// Traversal stack: the top `ring` entries of each lane live in LDS (slot-major, lane-minor, so a wave's accesses
+// spread over all banks); older entries spill to private memory only when a ray goes deeper than the ring, and come
 // back when the ring drains. Invariant: entries [lo, sp) are in LDS, [0, lo) in the private spill area.
-constexpr int RING = 8;
+// The ring depth is a launch parameter (a power of two, 4..16): deeper rings spill less but take more LDS per block.
+constexpr uint32_t RING_DEFAULT = 8;
 
 template <bool SPLIT>
 struct SpillArea {  // private (scratch) memory for entries pushed deeper than the ring
   using Key = typename std::conditional<SPLIT, float, double>::type;
-  Key key[STACK_CAP];
-  uint32_t ref[STACK_CAP], cnt[STACK_CAP];
+  Key key[stack_cap<SPLIT>()];
+  uint32_t ref[stack_cap<SPLIT>()], cnt[stack_cap<SPLIT>()];
 };
 
 // The counters and pointers are kept apart from the spill arrays so that they stay in registers.
@@ -541,20 +546,21 @@ struct RingStack {
   uint32_t *lref, *lcnt;
   SpillArea<SPLIT> *spill;
   int sp, lo;
+  int ring;  // LDS slots per lane (power of two)
 
   __device__ __forceinline__ void reset() {
     sp = 0;
     lo = 0;
   }
   __device__ __forceinline__ void push(uint32_t r, uint32_t c, Key k) {
-    if (sp - lo == RING) {
-      const int j = (lo & (RING - 1)) * 64;
+    if (sp - lo == ring) {
+      const int j = (lo & (ring - 1)) * 64;
       spill->key[lo] = lkey[j];
       spill->ref[lo] = lref[j];
       spill->cnt[lo] = lcnt[j];
       lo++;
     }
-    const int j = (sp & (RING - 1)) * 64;
+    const int j = (sp & (ring - 1)) * 64;
     lkey[j] = k;
     lref[j] = r;
     lcnt[j] = c;
@@ -569,7 +575,7 @@ struct RingStack {
       c = spill->cnt[sp];
       k = spill->key[sp];
     } else {
-      const int j = (sp & (RING - 1)) * 64;
+      const int j = (sp & (ring - 1)) * 64;
       r = lref[j];
       c = lcnt[j];
       k = lkey[j];
@@ -579,28 +585,52 @@ struct RingStack {
 
 // LDS bytes of the ring stacks of one block
 template <bool SPLIT>
-constexpr size_t ring_lds_bytes(uint32_t block) {
-  return (size_t)(block / 64) * RING * 64 * (sizeof(typename RingStack<SPLIT>::Key) + 8);
+constexpr size_t ring_lds_bytes(uint32_t block, uint32_t ring) {
+  return (size_t)(block / 64) * ring * 64 * (sizeof(typename RingStack<SPLIT>::Key) + 8);
 }
 
 template <bool SPLIT>
-__device__ __forceinline__ void ring_bind(RingStack<SPLIT> &st, unsigned char *smem, SpillArea<SPLIT> *spill) {
+__device__ __forceinline__ void ring_bind(RingStack<SPLIT> &st, unsigned char *smem, SpillArea<SPLIT> *spill,
+                                          uint32_t ring) {
   using Key = typename RingStack<SPLIT>::Key;
   const uint32_t waves = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   Key *keys = reinterpret_cast<Key *>(smem);
-  uint32_t *refs = reinterpret_cast<uint32_t *>(keys + (size_t)waves * RING * 64);
-  uint32_t *cnts = refs + (size_t)waves * RING * 64;
-  st.lkey = keys + (size_t)wave * RING * 64 + lane;
-  st.lref = refs + (size_t)wave * RING * 64 + lane;
-  st.lcnt = cnts + (size_t)wave * RING * 64 + lane;
+  uint32_t *refs = reinterpret_cast<uint32_t *>(keys + (size_t)waves * ring * 64);
+  uint32_t *cnts = refs + (size_t)waves * ring * 64;
+  st.lkey = keys + (size_t)wave * ring * 64 + lane;
+  st.lref = refs + (size_t)wave * ring * 64 + lane;
+  st.lcnt = cnts + (size_t)wave * ring * 64 + lane;
+  st.ring = (int)ring;
   st.spill = spill;
   st.reset();
+}
+
+// Ascending sort of four (key, ref, cnt) triples: compare-exchange network (0,1)(2,3)(0,2)(1,3)(1,2), every index
+// a compile-time constant so the arrays stay in registers.
+template <int I, int J>
+__device__ __forceinline__ void cmpx(float (&k)[4], uint32_t (&r)[4], uint32_t (&c)[4]) {
+  const bool sw = k[J] < k[I];
+  const float ki = k[I], kj = k[J];
+  const uint32_t ri = r[I], rj = r[J], ci = c[I], cj = c[J];
+  k[I] = sw ? kj : ki;
+  k[J] = sw ? ki : kj;
+  r[I] = sw ? rj : ri;
+  r[J] = sw ? ri : rj;
+  c[I] = sw ? cj : ci;
+  c[J] = sw ? ci : cj;
+}
+__device__ __forceinline__ void sort4(float (&k)[4], uint32_t (&r)[4], uint32_t (&c)[4]) {
+  cmpx<0, 1>(k, r, c);
+  cmpx<2, 3>(k, r, c);
+  cmpx<0, 2>(k, r, c);
+  cmpx<1, 3>(k, r, c);
+  cmpx<1, 2>(k, r, c);
 }
 
 // Root test (the reference pushes node 0 and slab-tests it on the first pop with the ray's tmax).
 template <bool SPLIT>
 __device__ __forceinline__ void trav_begin(const BvhView &bvh, const Ray &r, const Ray32 &q, const Hit &s, Trav &t) {
-  t.cur = bvh.root_ref;
+  t.cur = SPLIT ? bvh.root_ref4 : bvh.root_ref;
   t.cnt = bvh.root_cnt;
   if (SPLIT) {
     float k32;
@@ -627,22 +657,38 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const Ray &r, cons
     uint32_t nref, ncnt, fref, fcnt;
     typename std::conditional<SPLIT, float, double>::type fkey;
     if constexpr (SPLIT) {
-      const Node32 nd = bvh.nodes32[t.cur];
-      const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
-      const uint32_t nc = (q.sign >> axis) & 1u;
-      float k0, k1;
-      const bool p0 = slab32(nd.b, q, k0);
-      const bool p1 = slab32(nd.b + 6, q, k1);
-      near_ok = (nc ? p1 : p0) && !(t.h32 < (nc ? k1 : k0));
-      far_ok = nc ? p0 : p1;
-      fkey = nc ? k0 : k1;
-      // both fields read into registers and selected: a runtime index into the local record would put the
-      // whole record in scratch memory
-      const uint32_t r0 = nd.ref[0], r1 = nd.ref[1], c0 = nd.cnt[0] & CNT_MASK, c1 = nd.cnt[1] & CNT_MASK;
-      nref = nc ? r1 : r0;
-      ncnt = nc ? c1 : c0;
-      fref = nc ? r0 : r1;
-      fcnt = nc ? c0 : c1;
+      // up to four children per record, visited nearest entry first (the FP32 path needs no particular order:
+      // it only has to test every primitive the FP64 path tests)
+      const Node4 nd = bvh.nodes4[t.cur];
+      float k[4];
+      uint32_t rf[4], cn[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        float A = q.tmin, M = t.h32;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          const bool sg = (q.sign >> a) & 1u;
+          const float lo = nd.lo[a][c], hi = nd.hi[a][c];
+          A = fmaxf(A, __builtin_fmaf(sg ? hi : lo, q.inv[a], q.offn[a]));
+          M = fminf(M, __builtin_fmaf(sg ? lo : hi, q.inv[a], q.offf[a]));
+        }
+        k[c] = (nd.cnt[c] != EMPTY_SLOT && A <= M) ? A : __builtin_inff();
+        rf[c] = nd.ref[c];
+        cn[c] = nd.cnt[c];
+      }
+      sort4(k, rf, cn);
+      const float INF = __builtin_inff();
+      if (k[3] < INF) st.push(rf[3], cn[3], k[3]);
+      if (k[2] < INF) st.push(rf[2], cn[2], k[2]);
+      if (k[1] < INF) st.push(rf[1], cn[1], k[1]);
+      if (k[0] < INF) {
+        t.cur = rf[0];
+        t.cnt = cn[0];
+        return;
+      }
+      near_ok = far_ok = false;
+      nref = ncnt = fref = fcnt = 0;
+      fkey = 0.0f;
     } else {
       const Node64 nd = bvh.nodes64[t.cur];
       const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
@@ -752,13 +798,14 @@ __device__ __forceinline__ void start_ray(const BvhView &bvh, const RaysView &ra
 // One ray per thread (grid-stride).
 template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysView rays, HitsView hits, uint64_t n, uint32_t flags,
-                                               uint32_t first, uint32_t is_tri, double S, const uint32_t *order) {
+                                               uint32_t first, uint32_t is_tri, double S, const uint32_t *order,
+                                               uint32_t ring) {
   extern __shared__ unsigned char smem[];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
   SpillArea<SPLIT> spill;
   RingStack<SPLIT> st;
-  ring_bind<SPLIT>(st, smem, &spill);
+  ring_bind<SPLIT>(st, smem, &spill, ring);
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     const uint64_t i = order ? (uint64_t)order[k] : k;
     Ray r;
@@ -788,13 +835,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WA
 k_trace_persistent(BvhView bvh, RaysView rays, HitsView hits, uint64_t n,
                                                           uint32_t flags, uint32_t first, uint32_t is_tri, double S,
                                                           unsigned long long *counter, uint32_t refill,
-                                                          const uint32_t *order) {
+                                                          const uint32_t *order, uint32_t ring) {
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
   const uint32_t lane = __lane_id();
   SpillArea<SPLIT> spill;
   RingStack<SPLIT> st;
-  ring_bind<SPLIT>(st, smem, &spill);
+  ring_bind<SPLIT>(st, smem, &spill, ring);
   Ray r;
   Ray32 q;
   Hit s;
@@ -856,13 +903,13 @@ __device__ __forceinline__ uint32_t find_job(const BatchJob *jobs, uint32_t njob
 template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
 k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total, uint32_t flags,
-              unsigned long long *counter, uint32_t refill) {
+              unsigned long long *counter, uint32_t refill, uint32_t ring) {
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
   const uint32_t lane = __lane_id();
   SpillArea<SPLIT> spill;
   RingStack<SPLIT> st;
-  ring_bind<SPLIT>(st, smem, &spill);
+  ring_bind<SPLIT>(st, smem, &spill, ring);
   Ray r;
   Ray32 q;
   Hit s;
@@ -964,7 +1011,8 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysView &rays, const HitsVie
                          const TraceLaunch &cfg, const SplitParams &sp, uint32_t is_tri, hipStream_t stream) {
   const uint32_t block = cfg.block ? cfg.block : 256;
   const uint32_t first = cfg.first ? 1u : 0u;
-  const size_t lds = ring_lds_bytes<SPLIT>(block);
+  const uint32_t ring = cfg.ring ? cfg.ring : RING_DEFAULT;
+  const size_t lds = ring_lds_bytes<SPLIT>(block, ring);
   if (cfg.schedule == SCHEDULE_PERSISTENT) {
     if (!cfg.counter) return hipErrorInvalidValue;
     int dev = 0, cus = 0, per_cu = 0;
@@ -981,14 +1029,14 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysView &rays, const HitsVie
     const uint32_t refill = cfg.refill ? cfg.refill : 16u;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
     hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
-                       hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill, cfg.order);
+                       hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill, cfg.order, ring);
     return hipGetLastError();
   }
   uint64_t blocks = (n + block - 1) / block;
   if (blocks > (1ull << 30)) blocks = 1ull << 30;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((k_trace<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays, hits, n,
-                     cfg.flags, first, is_tri, sp.scene_extent, cfg.order);
+                     cfg.flags, first, is_tri, sp.scene_extent, cfg.order, ring);
   return hipGetLastError();
 }
 
@@ -996,7 +1044,8 @@ template <class Rec, bool SPLIT>
 hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t total, const TraceLaunch &cfg,
                                hipStream_t stream) {
   const uint32_t block = cfg.block ? cfg.block : 256;
-  const size_t lds = ring_lds_bytes<SPLIT>(block);
+  const uint32_t ring = cfg.ring ? cfg.ring : RING_DEFAULT;
+  const size_t lds = ring_lds_bytes<SPLIT>(block, ring);
   if (!cfg.counter || njobs == 0) return hipErrorInvalidValue;
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -1012,7 +1061,7 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
   const uint32_t refill = cfg.refill ? cfg.refill : 16u;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
   hipLaunchKernelGGL((k_trace_batch<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, jobs, njobs, total,
-                     cfg.flags, cfg.counter, refill);
+                     cfg.flags, cfg.counter, refill, ring);
   return hipGetLastError();
 }
 

@@ -26,13 +26,18 @@ struct alignas(128) Node64 {
 };
 static_assert(sizeof(Node64) == 128, "Node64 must be one 128-byte line");
 
-// FP32 internal node for split-precision traversal: child boxes rounded outward (min down, max up).
-struct alignas(64) Node32 {
-  float b[12];
-  uint32_t ref[2];
-  uint32_t cnt[2];
+// FP32 4-wide node for split-precision traversal, 128 B: the binary tree collapsed two levels at a time (an
+// internal node's internal children are replaced by their own two children), so one record holds up to four
+// child boxes, rounded outward (min down, max up). Planes are child-minor: lo[axis][child]. A child reference is a
+// Node4 index or LEAF_BIT | first primitive with the leaf count beside it; unused slots have cnt = EMPTY_SLOT.
+constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
+struct alignas(128) Node4 {
+  float lo[3][4];
+  float hi[3][4];
+  uint32_t ref[4];
+  uint32_t cnt[4];
 };
-static_assert(sizeof(Node32) == 64, "Node32 must be 64 bytes");
+static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 
 // Triangle record in leaf order: v0, e1 = v0 - v1, e2 = v2 - v0 (blazert/primitives/trimesh.h:26-27).
 struct alignas(16) TriRec {
@@ -70,7 +75,7 @@ static_assert(sizeof(CylRec) == 128, "CylRec size");
 // Kernel-side view of one uploaded geometry BVH (passed by value).
 struct BvhView {
   const Node64 *nodes64;
-  const Node32 *nodes32;
+  const Node4 *nodes4;
   const void *prims;          // TriRec / SphRec / PlaRec / CylRec in leaf order
   const uint32_t *prim_ids;   // leaf order -> user primitive id
   const uint32_t *faces;      // triangles only: user faces (3 per primitive id)
@@ -79,6 +84,7 @@ struct BvhView {
   double root_min[3], root_max[3];
   float root_min32[3], root_max32[3];
   uint32_t root_ref, root_cnt;
+  uint32_t root_ref4;         // root as a Node4 reference (index 0, or the same leaf as root_ref)
   uint32_t geom_id;
   uint32_t n_prims;
   uint32_t n_nodes;

@@ -24,10 +24,10 @@
 
 #include "../../include/blazert_amd.h"
 #include "engine.h"
+#include "internal.h"
 #include "layout.h"
 
 namespace brt {
-namespace {
 
 thread_local std::string g_error;
 
@@ -35,6 +35,8 @@ int fail(int code, const std::string &msg) {
   g_error = msg;
   return code;
 }
+
+namespace {
 
 // ------------------------------------------------------------------------------------------------ vec helpers
 // blaze conventions: dot = (a0 b0 + a1 b1) + a2 b2, normalize = v * (len != 0 ? 1/len : 0),
@@ -94,7 +96,7 @@ struct Geometry {
   brt_build_stats stats{};
   // flattened
   std::vector<Node64> n64;
-  std::vector<Node32> n32;
+  std::vector<Node4> n4;
   std::vector<unsigned char> recs;
   size_t rec_size = 0;
   BvhView view{};
@@ -415,7 +417,6 @@ void flatten(Geometry &g, GeomType type) {
   for (size_t i = 0; i < T.size(); i++)
     if (!T[i].leaf) internal[i] = ni++;
   g.n64.assign(ni, Node64{});
-  g.n32.assign(ni, Node32{});
   double extent = 0.0;
   bool ok = true, nan_boxes = false;
   auto note = [&](double v) {
@@ -435,23 +436,70 @@ void flatten(Geometry &g, GeomType type) {
   for (size_t i = 0; i < T.size(); i++) {
     if (T[i].leaf) continue;
     Node64 &a = g.n64[internal[i]];
-    Node32 &b = g.n32[internal[i]];
     for (int c = 0; c < 2; c++) {
       const brt_tree_node &ch = T[T[i].children[c]];
       for (int k = 0; k < 3; k++) {
         a.b[6 * c + k] = ch.min[k];
         a.b[6 * c + 3 + k] = ch.max[k];
-        b.b[6 * c + k] = f32_down(ch.min[k]);
-        b.b[6 * c + 3 + k] = f32_up(ch.max[k]);
         note(ch.min[k]);
         note(ch.max[k]);
       }
       child_ref(T[i].children[c], a.ref[c], a.cnt[c]);
-      b.ref[c] = a.ref[c];
-      b.cnt[c] = a.cnt[c];
     }
     a.cnt[1] |= T[i].axis << AXIS_SHIFT;
-    b.cnt[1] |= T[i].axis << AXIS_SHIFT;
+  }
+  // 4-wide FP32 records: every Node4 stands for one binary internal node; its slots are the node's children, with
+  // each internal child replaced by that child's two children. Depth-first allocation keeps subtrees together.
+  g.n4.clear();
+  uint32_t root4 = INVALID_ID;
+  if (!T.empty() && !T[0].leaf) {
+    std::vector<std::pair<uint32_t, uint32_t>> work;  // (binary internal node, its Node4 index)
+    g.n4.push_back(Node4{});
+    root4 = 0;
+    work.emplace_back(0u, 0u);
+    while (!work.empty()) {
+      const auto [bi, k4] = work.back();
+      work.pop_back();
+      uint32_t slot[4];
+      int m = 0;
+      for (int c = 0; c < 2; c++) {
+        const uint32_t ch = T[bi].children[c];
+        if (T[ch].leaf) {
+          slot[m++] = ch;
+        } else {
+          slot[m++] = T[ch].children[0];
+          slot[m++] = T[ch].children[1];
+        }
+      }
+      Node4 nd{};
+      for (int c = 0; c < 4; c++) {
+        for (int k = 0; k < 3; k++) {
+          nd.lo[k][c] = __builtin_inff();
+          nd.hi[k][c] = -__builtin_inff();
+        }
+        nd.ref[c] = 0;
+        nd.cnt[c] = EMPTY_SLOT;
+      }
+      std::vector<std::pair<uint32_t, uint32_t>> children;
+      for (int c = 0; c < m; c++) {
+        const brt_tree_node &sn = T[slot[c]];
+        for (int k = 0; k < 3; k++) {
+          nd.lo[k][c] = f32_down(sn.min[k]);
+          nd.hi[k][c] = f32_up(sn.max[k]);
+        }
+        if (sn.leaf) {
+          nd.ref[c] = LEAF_BIT | sn.prim_begin;
+          nd.cnt[c] = sn.prim_count;
+        } else {
+          nd.ref[c] = (uint32_t)g.n4.size();
+          nd.cnt[c] = 0;
+          g.n4.push_back(Node4{});
+          children.emplace_back(slot[c], nd.ref[c]);
+        }
+      }
+      g.n4[k4] = nd;
+      for (auto it = children.rbegin(); it != children.rend(); ++it) work.push_back(*it);
+    }
   }
   BvhView &v = g.view;
   v = BvhView{};
@@ -464,6 +512,7 @@ void flatten(Geometry &g, GeomType type) {
     note(T[0].max[k]);
   }
   child_ref(0, v.root_ref, v.root_cnt);
+  v.root_ref4 = (root4 == INVALID_ID) ? v.root_ref : root4;
   v.geom_id = g.geom_id;
   v.n_prims = g.boxes.n;
   v.n_nodes = ni;
@@ -560,7 +609,7 @@ void build_geometry(Geometry &g, GeomType type, const brt_build_options &o) {
   s.geom_id = g.geom_id;
   flatten(g, type);
   s.build_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  s.device_bytes = g.n64.size() * sizeof(Node64) + g.n32.size() * sizeof(Node32) + g.recs.size() +
+  s.device_bytes = g.n64.size() * sizeof(Node64) + g.n4.size() * sizeof(Node4) + g.recs.size() +
                    g.order.size() * 4 + g.faces.size() * 4 + g.vnormals.size() * 8 + g.vertices.size() * 8;
 }
 
@@ -602,6 +651,22 @@ struct DeviceScene {
 };
 
 }  // namespace
+}  // namespace brt
+
+namespace brt {
+
+int check_launch(const brt_launch &L) {
+  if (L.precision > BRT_PRECISION_FP32_TRAV_FP64_HIT) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (L.schedule > BRT_SCHEDULE_PER_THREAD) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown schedule");
+  if (L.refill > 64) return fail(BRT_ERR_INVALID_ARGUMENT, "refill must be 0..64");
+  if (L.sort > BRT_SORT_MORTON) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown sort mode");
+  if (L.block_size && (L.block_size % 64 || L.block_size > 256))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
+  if (L.stack_ring && L.stack_ring != 4 && L.stack_ring != 8 && L.stack_ring != 16)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "stack_ring must be 0, 4, 8 or 16");
+  return BRT_OK;
+}
+
 }  // namespace brt
 
 using namespace brt;
@@ -655,7 +720,7 @@ int upload(brt_scene *s) {
     if (g.stats.max_tree_depth >= 64) return fail(BRT_ERR_UNSUPPORTED, "tree deeper than the 64-entry device stack");
     BvhView v = g.view;
     hipError_t e = d->put(g.n64, &v.nodes64);
-    if (e == hipSuccess) e = d->put(g.n32, &v.nodes32);
+    if (e == hipSuccess) e = d->put(g.n4, &v.nodes4);
     const unsigned char *recs = nullptr;
     if (e == hipSuccess) e = d->put(g.recs, &recs);
     v.prims = recs;
@@ -838,13 +903,9 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
   L.precision = BRT_PRECISION_FP64;
   L.schedule = BRT_SCHEDULE_PERSISTENT;
   if (launch) L = *launch;
-  if (L.precision > BRT_PRECISION_FP32_TRAV_FP64_HIT) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown precision");
-  if (L.schedule > BRT_SCHEDULE_PER_THREAD) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown schedule");
-  if (L.refill > 64) return fail(BRT_ERR_INVALID_ARGUMENT, "refill must be 0..64");
-  if (L.sort > BRT_SORT_MORTON) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown sort mode");
-  if (L.block_size && (L.block_size % 64 || L.block_size > 256))
-    return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
-  int rc = upload(s);
+  int rc = check_launch(L);
+  if (rc) return rc;
+  rc = upload(s);
   if (rc) return rc;
   RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
   HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz, nullptr};
@@ -873,6 +934,7 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
     if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT && !g.extent_ok)
       return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
     TraceLaunch cfg{L.precision, L.flags, L.block_size, first, L.schedule, L.refill, nullptr, order};
+    cfg.ring = L.stack_ring;
     if (L.schedule == BRT_SCHEDULE_PERSISTENT)
       cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
     SplitParams sp{g.extent};
@@ -967,6 +1029,7 @@ int batch_enqueue(brt_batch *b, hipStream_t st) {
     if (!b->njobs[t]) continue;
     TraceLaunch cfg{b->L.precision, b->L.flags, b->L.block_size, false, SCHEDULE_PERSISTENT, b->L.refill,
                     b->counters + t, nullptr};
+    cfg.ring = b->L.stack_ring;
     hipError_t e = launch_trace_batch((GeomType)t, static_cast<const BatchJob *>(b->d_jobs[t]), b->njobs[t],
                                       b->total[t], cfg, st);
     if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("batch launch failed: ") + hipGetErrorString(e));
@@ -991,11 +1054,8 @@ int brt_batch_create(const brt_batch_item *items, uint32_t count, const brt_laun
   auto b = std::make_unique<brt_batch>();
   b->L.precision = BRT_PRECISION_FP64;
   if (launch) b->L = *launch;
-  if (b->L.precision > BRT_PRECISION_FP32_TRAV_FP64_HIT) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (int rc = check_launch(b->L)) return rc;
   if (b->L.sort != BRT_SORT_NONE) return fail(BRT_ERR_UNSUPPORTED, "batches trace rays in input order");
-  if (b->L.refill > 64) return fail(BRT_ERR_INVALID_ARGUMENT, "refill must be 0..64");
-  if (b->L.block_size && (b->L.block_size % 64 || b->L.block_size > 256))
-    return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
   if (hipGetDevice(&b->device) != hipSuccess) return fail(BRT_ERR_NO_DEVICE, "no HIP device");
   b->items.assign(items, items + count);
   std::vector<BatchJob> jobs[NUM_TYPES];
@@ -1107,13 +1167,10 @@ int brt_mirror_trace(const brt_scene *cs, const brt_rays *rays, uint64_t n, uint
   if (!(bounce_tmin >= 0.0)) return fail(BRT_ERR_INVALID_ARGUMENT, "bounce_tmin must be >= 0");
   brt_launch L{};
   if (launch) L = *launch;
+  if (int rc = check_launch(L)) return rc;
   if (L.precision != BRT_PRECISION_FP64) return fail(BRT_ERR_UNSUPPORTED, "mirror tracing runs the FP64 path");
   if (L.flags) return fail(BRT_ERR_INVALID_ARGUMENT, "mirror tracing takes no launch flags");
-  if (L.schedule > BRT_SCHEDULE_PER_THREAD) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown schedule");
-  if (L.refill > 64) return fail(BRT_ERR_INVALID_ARGUMENT, "refill must be 0..64");
   if (L.sort != BRT_SORT_NONE) return fail(BRT_ERR_UNSUPPORTED, "mirror tracing does not sort rays");
-  if (L.block_size && (L.block_size % 64 || L.block_size > 256))
-    return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
   int rc = mirror_check(s, n, bounces);
   if (rc) return rc;
   if (n == 0) return BRT_OK;
@@ -1131,6 +1188,7 @@ int brt_mirror_trace(const brt_scene *cs, const brt_rays *rays, uint64_t n, uint
     HitsView hv{out->t + off, out->u ? out->u + off : nullptr, out->v ? out->v + off : nullptr, out->prim_id + off,
                 nullptr, nullptr, nullptr, nullptr, nullptr};
     TraceLaunch cfg{L.precision, 0u, L.block_size, true, L.schedule, L.refill, nullptr, nullptr};
+    cfg.ring = L.stack_ring;
     if (L.schedule == BRT_SCHEDULE_PERSISTENT)
       cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
     SplitParams sp{s->geo[TRIANGLES].extent};
@@ -1168,3 +1226,35 @@ int brt_mirror_adjoint(const brt_scene *cs, const brt_rays *rays, uint64_t n, ui
 }
 
 }  // extern "C"
+
+namespace brt {
+
+int scene_rebuild(brt_scene *s, const brt_build_options &opts) {
+  if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (opts.min_leaf_primitives == 0 || opts.bin_size == 0)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "leaf size and bin count must be positive");
+  std::lock_guard<std::mutex> lk(s->upload_mu);
+  s->dev.reset();
+  s->opts = opts;
+  for (uint32_t t = 0; t < NUM_TYPES; t++)
+    if (s->geo[t].present) build_geometry(s->geo[t], (GeomType)t, s->opts);
+  s->committed = true;
+  return BRT_OK;
+}
+
+brt_build_options scene_options(const brt_scene *s) { return s->opts; }
+
+brt_launch batch_launch_config(const brt_batch *b) { return b->L; }
+
+int batch_set_launch_config(brt_batch *b, const brt_launch &L) {
+  if (b->exec) return fail(BRT_ERR_STATE, "the batch has been captured; tune it before brt_batch_capture");
+  if (int rc = check_launch(L)) return rc;
+  if (L.sort != BRT_SORT_NONE) return fail(BRT_ERR_UNSUPPORTED, "batches trace rays in input order");
+  const uint32_t precision = b->L.precision, flags = b->L.flags;
+  b->L = L;
+  b->L.precision = precision;
+  b->L.flags = flags;
+  return BRT_OK;
+}
+
+}  // namespace brt

@@ -128,7 +128,7 @@ class Scene:
     def intersect(self, org, dirs, tmin=None, tmax=None, *, precision: int = capi.PRECISION_FP64,
                   any_hit: bool = False, normals: bool = False, hits: dict | None = None, block_size: int = 0,
                   schedule: int = capi.SCHEDULE_PERSISTENT, refill: int = 0, sort: int = capi.SORT_NONE,
-                  stream=None) -> dict:
+                  stack_ring: int = 0, stream=None) -> dict:
         """Batched intersect1 over n rays. org / dirs: (n,3) float64 CUDA tensors or tuples of three columns.
 
         Returns a dict of CUDA tensors t, u, v (float64), prim_id, geom_id (int32 holding uint32 bits) and,
@@ -158,7 +158,7 @@ class Scene:
         H = capi.Hits(hits["t"].data_ptr(), hits["u"].data_ptr(), hits["v"].data_ptr(), hits["prim_id"].data_ptr(),
                       hits["geom_id"].data_ptr(), *([nrm[k].data_ptr() for k in range(3)] if nrm is not None
                                                     else [None, None, None]))
-        L = capi.Launch(precision, flags, block_size, schedule, refill, sort)
+        L = capi.Launch(precision, flags, block_size, schedule, refill, sort, stack_ring)
         st = stream if stream is not None else torch.cuda.current_stream(dev)
         keep = (cols_o, cols_d, tmn, tmx)  # noqa: F841 (inputs alive until the launch is enqueued)
         check(lib().brt_intersect(self._h, C.byref(rays), C.byref(H), n, C.byref(L), C.c_void_p(st.cuda_stream)))
@@ -188,7 +188,7 @@ class Batch:
     directions as for ``Scene.intersect``. ``hits[i]`` holds item i's results after a launch or replay."""
 
     def __init__(self, items, *, precision: int = capi.PRECISION_FP64, any_hit: bool = False,
-                 normals: bool = False, block_size: int = 0, refill: int = 0):
+                 normals: bool = False, block_size: int = 0, refill: int = 0, stack_ring: int = 0):
         self._keep = []
         self.scenes = []
         self.hits = []
@@ -211,7 +211,7 @@ class Batch:
             arr[i].hits = _hits_struct(h)
             arr[i].n = n
         L = capi.Launch(precision, capi.FLAG_ANY_HIT if any_hit else 0, block_size, capi.SCHEDULE_PERSISTENT, refill,
-                        capi.SORT_NONE)
+                        capi.SORT_NONE, stack_ring)
         h = C.c_void_p()
         check(lib().brt_batch_create(arr, len(items), C.byref(L), C.byref(h)))
         self._h = h
@@ -229,6 +229,10 @@ class Batch:
 
     def replay(self, stream=None):
         check(lib().brt_batch_replay(self._h, C.c_void_p(self._stream(stream).cuda_stream)))
+
+    @property
+    def handle(self):
+        return self._h
 
     def close(self):
         h = getattr(self, "_h", None)

@@ -208,6 +208,50 @@ def mixed_rays(n: int = 8 * 1024 * 1024, seed: int = 13):
     return incoherent_rays(n, seed, 22.0)
 
 
+def torus(n_tri_target: int, seed: int = 0, R: float = 1.0, r: float = 0.35, displacement: float = 0.05):
+    """Closed torus mesh with about n_tri_target triangles (an nu x nv grid, 2 triangles per cell), each vertex
+    moved along its tube normal by displacement * r * N(0,1)."""
+    cells = max(8, n_tri_target // 2)
+    nv = max(3, int(round(math.sqrt(cells * r / R))))
+    nu = max(3, int(round(cells / nv)))
+    u = 2 * math.pi * np.arange(nu) / nu
+    v = 2 * math.pi * np.arange(nv) / nv
+    U, Vv = np.meshgrid(u, v, indexing="ij")
+    g = Rng(seed).normal(nu * nv).reshape(nu, nv)
+    rr = r * (1.0 + displacement * g)
+    x = (R + rr * np.cos(Vv)) * np.cos(U)
+    y = (R + rr * np.cos(Vv)) * np.sin(U)
+    z = rr * np.sin(Vv)
+    verts = np.stack([x, y, z], axis=-1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    d = i * nv + (j + 1) % nv
+    faces = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)])
+    return np.ascontiguousarray(verts), np.ascontiguousarray(faces.astype(np.uint32))
+
+
+def uneven_batch(count: int = 32, seed: int = 41, tri_range=(1_000, 500_000), ray_range=(16_384, 2_097_152)):
+    """Batched uneven workload: `count` triangle scenes (displaced icospheres and tori alternating, sizes
+    log-uniform in tri_range) each with its own incoherent rays (count log-uniform in ray_range, origins in the
+    scene's bounding cube scaled by 1.5). Returns a list of dicts with vertices, faces, origins, directions."""
+    rng = Rng(seed)
+    lt = np.exp(rng.uniform(count, math.log(tri_range[0]), math.log(tri_range[1])))
+    lr = np.exp(rng.uniform(count, math.log(ray_range[0]), math.log(ray_range[1])))
+    out = []
+    for k in range(count):
+        target = int(lt[k])
+        if k % 2 == 0:
+            level = max(1, int(round(math.log(max(target, 20) / 20.0, 4))))
+            V, F = icosphere(level, seed=seed + k)
+        else:
+            V, F = torus(target, seed=seed + k)
+        o, d = incoherent_rays(int(lr[k]), seed=seed + 1000 + k, box=1.5)
+        out.append({"vertices": V, "faces": F, "origins": o, "directions": d})
+    return out
+
+
 def box_mesh(half: float):
     """Closed axis-aligned cube [-half, half]^3 as 8 vertices and 12 triangles."""
     v = np.array([[x, y, z] for x in (-half, half) for y in (-half, half) for z in (-half, half)], dtype=np.float64)

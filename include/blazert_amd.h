@@ -113,7 +113,9 @@ typedef struct brt_launch {
   uint32_t schedule;   /* BRT_SCHEDULE_* */
   uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default */
   uint32_t sort;       /* BRT_SORT_*: trace rays in a sorted order (results are written in input order) */
-  uint32_t reserved[2];
+  uint32_t stack_ring; /* traversal-stack entries per lane kept in LDS: 4, 8 or 16; 0 = 8. Deeper entries spill to
+                          private memory; a deeper ring spills less but takes more LDS per block */
+  uint32_t reserved;
 } brt_launch;
 
 /* One ray / hit in host memory, for the single-ray drop-in (RayHit<T>, blazert/ray.h:72-85). */
@@ -229,6 +231,33 @@ typedef struct brt_mirror_grads {
  * committed vertices; no other intermediate of the forward pass is needed. Asynchronous on `stream`. */
 int brt_mirror_adjoint(const brt_scene *scene, const brt_rays *rays, uint64_t n, uint32_t bounces,
                        const uint32_t *prim_id, const brt_mirror_grads *grads, void *stream);
+
+/* ---- autotuning ---------------------------------------------------------------------------------------------
+ * No counterpart in the reference. Times launch configurations (and BVH leaf sizes) on the caller's device rays with
+ * HIP events and reports the fastest. Every candidate computes the same hits (the FP64 path bit for bit, split
+ * precision on every ray that is not a tie), so tuning changes speed only. `hits` is scratch output. */
+typedef struct brt_tune_result {
+  brt_launch launch;             /* fastest configuration (precision and flags as requested) */
+  uint32_t min_leaf_primitives;  /* brt_scene_tune_leaf: the leaf size chosen, else 0 */
+  uint32_t candidates;           /* configurations timed */
+  double best_ms;                /* mean launch time of the fastest configuration */
+  double default_ms;             /* mean launch time of the library defaults (leaf size 4 for the leaf tuner) */
+} brt_tune_result;
+
+/* block_size {64,128,192,256} x refill {8,16,32} x stack_ring {4,8,16} x sort {NONE, OCTANT_MORTON}, plus the
+ * defaults; `repeats` timed launches each (0 = 3) after one warm-up. Synchronizes `stream`. flags must not hold
+ * BRT_FLAG_HITS_INOUT. */
+int brt_autotune_launch(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,
+                        uint32_t precision, uint32_t flags, uint32_t repeats, brt_tune_result *out, void *stream);
+/* The same over a batch (block_size x refill x stack_ring; batches do not sort). The batch keeps the fastest
+ * configuration for later launches and captures, so tune before brt_batch_capture. */
+int brt_batch_autotune(brt_batch *batch, uint32_t repeats, brt_tune_result *out, void *stream);
+/* BVH leaf size (BVHBuildOptions::min_leaf_primitives, blazert/bvh/options.h:21-27): rebuilds the scene with each
+ * size in leaf_sizes (other build options kept), times a default launch of `precision` on the rays, and leaves the
+ * scene committed with the fastest. Rebuilding discards the scene's device copy. */
+int brt_scene_tune_leaf(brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n, uint32_t precision,
+                        const uint32_t *leaf_sizes, uint32_t count, uint32_t repeats, brt_tune_result *out,
+                        void *stream);
 
 /* Single-ray drop-in for intersect1(scene, Ray<T>{org, dir, tmin, tmax}, rayhit): host memory in/out, blocking.
  * hit is in/out exactly like RayHit<T>; *hit_out receives the bool result. */

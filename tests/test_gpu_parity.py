@@ -167,12 +167,17 @@ def test_icosphere8_sorted_launch(dev, ico8, sort, precision):
     assert_bit_exact(g, ref, ok)
 
 
-@pytest.mark.parametrize("schedule,refill,block", [(1, 0, 64), (0, 1, 64), (0, 64, 128), (0, 16, 192)])
-def test_launch_configurations_identical(dev, ico8, schedule, refill, block):
+@pytest.mark.parametrize("schedule,refill,block,ring", [(1, 0, 64, 0), (0, 1, 64, 4), (0, 64, 128, 16),
+                                                        (0, 16, 192, 8), (1, 0, 256, 4)])
+def test_launch_configurations_identical(dev, ico8, schedule, refill, block, ring):
     p, o = ico8
     org, dirs = W.icosphere_incoherent_rays(1 << 16, seed=21)
-    g = gpu_intersect(p, org, dirs, dev, schedule=schedule, refill=refill, block_size=block)
+    g = gpu_intersect(p, org, dirs, dev, schedule=schedule, refill=refill, block_size=block, stack_ring=ring)
     assert_bit_exact(g, o.intersect_batch(org, dirs))
+    g32 = gpu_intersect(p, org, dirs, dev, precision=1, schedule=schedule, refill=refill, block_size=block,
+                        stack_ring=ring)
+    base = gpu_intersect(p, org, dirs, dev, precision=1)
+    assert_bit_exact(g32, base)  # the split path is deterministic per ray, whatever the launch shape
 
 
 def test_icosphere8_coherent_subset(dev, ico8):
