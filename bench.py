@@ -50,15 +50,22 @@ def time_launch(fn, steps, warmup):
     return float(np.mean(ms)), wall
 
 
-def classify_disagreements(osc, org, dirs, idx) -> dict:
+def classify_disagreements(osc, org, dirs, idx, other=None) -> dict:
     """Re-runs the oracle (with its traversal trace) on the rays where the two precisions disagree and counts how
-    many are ties (a runner-up within 1e-12 relative), order-dependent cylinder rays or reference stack overflows."""
+    many are ties (a runner-up within 1e-12 relative in the reference traversal, or a split-precision hit at the
+    same distance to 1e-12), order-dependent cylinder rays (either side's hit is a cylinder: the reference accepts a
+    cylinder without a distance check, so the last one visited wins) or reference stack overflows. `other` holds
+    the split path's t and geom_id on those rays."""
     if len(idx) == 0:
         return {"disagree": 0, "tie": 0, "cylinder": 0, "overflow": 0, "unexplained": 0}
     ref = osc.intersect_batch(org[idx], dirs[idx], normals=False, trace=True)
     tr = ref["trace"]
     tie = tr["tie"] != 0
     cyl = (tr["multi_cyl"] != 0) | (ref["geom_id"] == 3)
+    if other is not None:
+        ot, og = other
+        tie |= np.abs(ot - ref["t"]) <= 1e-12 * np.abs(ref["t"])
+        cyl |= og == 3
     ovf = tr["overflow"] != 0
     return {"disagree": int(len(idx)), "tie": int(tie.sum()), "cylinder": int((cyl & ~tie).sum()),
             "overflow": int((ovf & ~tie & ~cyl).sum()), "unexplained": int((~tie & ~cyl & ~ovf).sum())}
@@ -69,8 +76,9 @@ def split_agreement(h64, h32) -> tuple:
     hit = same & (h64["geom_id"] != -1)
     rel = ((h64["t"] - h32["t"]).abs() / h64["t"].abs())[hit]
     max_rel = float(rel.max().item()) if rel.numel() else 0.0
-    idx = (~same).nonzero().flatten().cpu().numpy()
-    return float(same.double().mean().item()), max_rel, idx
+    idx = (~same).nonzero().flatten()
+    other = (h32["t"][idx].cpu().numpy(), h32["geom_id"][idx].cpu().numpy().view(np.uint32))
+    return float(same.double().mean().item()), max_rel, idx.cpu().numpy(), other
 
 
 def mixed_leg(args, dev) -> dict:
@@ -105,7 +113,7 @@ def mixed_leg(args, dev) -> dict:
     h64 = scene.intersect(O, D, precision=PRECISION_FP64)
     h32 = scene.intersect(O, D, precision=PRECISION_FP32_TRAV_FP64_HIT)
     torch.cuda.synchronize()
-    agree, max_rel, idx = split_agreement(h64, h32)
+    agree, max_rel, idx, other = split_agreement(h64, h32)
     log(f"mixed: fp64 {n / ms64 / 1e3:.1f} / fp32-trav {n / ms32 / 1e3:.1f} Mrays/s, agreement {agree:.7f}")
     ns = 1 << 18
     ref = osc.intersect_batch(org[:ns], dirs[:ns], normals=False, trace=True,
@@ -120,8 +128,80 @@ def mixed_leg(args, dev) -> dict:
                         "incoherent rays (origins U[-22,22]^3)",
             "fp64_mrays_s": round(n / ms64 / 1e3, 3), "fp32_trav_mrays_s": round(n / ms32 / 1e3, 3),
             "speedup": round(ms64 / ms32, 3), "hit_id_agreement": agree, "max_rel_dt_agreeing": max_rel,
-            "disagreements": classify_disagreements(osc, org, dirs, idx), "commit_s": round(build_s, 3),
+            "disagreements": classify_disagreements(osc, org, dirs, idx, other), "commit_s": round(build_s, 3),
             "fp64_bit_exact_vs_oracle_sample": exact, "sample_rays": ns}
+
+
+def batch_leg(args, dev) -> dict:
+    """32 triangle scenes (displaced icospheres and tori, 1k-500k triangles, log-uniform) each with its own
+    incoherent rays (16k-2M, log-uniform), split precision. Leaf size is tuned per scene and block size / refill /
+    LDS stack depth for the whole batch by the library's autotuner; the batch is captured once in a hipGraph and
+    replayed. Reports aggregate Mrays/s of the replay, the per-scene latency of tracing each scene alone, and the
+    agreement of every hit id with the FP64 path."""
+    import torch
+
+    from blazert_amd import PRECISION_FP32_TRAV_FP64_HIT, PRECISION_FP64, Batch, Scene, workloads as W
+    from blazert_amd.autotune import tune_batch, tune_leaf
+
+    specs = W.uneven_batch()
+    items, leaves = [], []
+    t0 = time.perf_counter()
+    for sp in specs:
+        s = Scene()
+        s.add_mesh(sp["vertices"], sp["faces"])
+        O = tuple(torch.from_numpy(np.ascontiguousarray(sp["origins"][:, k])).to(dev) for k in range(3))
+        D = tuple(torch.from_numpy(np.ascontiguousarray(sp["directions"][:, k])).to(dev) for k in range(3))
+        m = min(len(sp["origins"]), 1 << 15)
+        r = tune_leaf(s, tuple(c[:m] for c in O), tuple(c[:m] for c in D), precision=PRECISION_FP32_TRAV_FP64_HIT,
+                      leaf_sizes=(2, 4, 8), repeats=2)
+        leaves.append(r["min_leaf_primitives"])
+        items.append((s, O, D))
+    prep_s = time.perf_counter() - t0
+    n_total = sum(len(sp["origins"]) for sp in specs)
+    batch = Batch(items, precision=PRECISION_FP32_TRAV_FP64_HIT)
+    tuned = tune_batch(batch, repeats=2)
+    stream = torch.cuda.Stream()
+    batch.capture(stream)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        batch.replay(stream)
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms = []
+    for _ in range(args.steps):
+        e0.record(stream)
+        batch.replay(stream)
+        e1.record(stream)
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    replay_ms = float(np.mean(ms))
+    # each scene alone (same precision, default launch): its latency when not batched
+    alone = []
+    for s, O, D in items:
+        t, _ = time_launch(lambda: s.intersect(O, D, precision=PRECISION_FP32_TRAV_FP64_HIT), 3, 1)
+        alone.append(t)
+    agree_n, agree_same, max_rel = 0, 0, 0.0
+    for (s, O, D), h in zip(items, batch.hits):
+        r = s.intersect(O, D, precision=PRECISION_FP64)
+        same = (r["prim_id"] == h["prim_id"]) & (r["geom_id"] == h["geom_id"])
+        agree_n += same.numel()
+        agree_same += int(same.sum().item())
+        hit = same & (r["geom_id"] != -1)
+        if hit.any():
+            max_rel = max(max_rel, float(((r["t"] - h["t"]).abs() / r["t"].abs())[hit].max().item()))
+    log(f"batch: {len(specs)} scenes, {n_total} rays, replay {replay_ms:.3f} ms -> {n_total / replay_ms / 1e3:.1f} "
+        f"Mrays/s; scenes alone sum {sum(alone):.3f} ms")
+    return {"workload": f"{len(specs)} triangle scenes ({sum(len(sp['faces']) for sp in specs)} tris in total, "
+                        f"1k-500k each) x {n_total} incoherent rays (16k-2M per scene), split precision, one "
+                        "hipGraph replay",
+            "mrays_s": round(n_total / replay_ms / 1e3, 3), "replay_ms": round(replay_ms, 4),
+            "scenes_alone_sum_ms": round(float(sum(alone)), 4),
+            "per_scene_alone_ms": {"min": round(float(min(alone)), 4), "median": round(float(np.median(alone)), 4),
+                                   "max": round(float(max(alone)), 4)},
+            "tuned_launch": {k: tuned[k] for k in ("block_size", "refill", "stack_ring")},
+            "tuned_vs_default_ms": [round(tuned["best_ms"], 4), round(tuned["default_ms"], 4)],
+            "tuned_leaf_sizes": leaves, "prep_s": round(prep_s, 2),
+            "hit_id_agreement_vs_fp64": agree_same / max(agree_n, 1), "max_rel_dt_agreeing": max_rel}
 
 
 def adjoint_leg(args, dev) -> dict:
@@ -193,27 +273,38 @@ def adjoint_leg(args, dev) -> dict:
     an_err = max(float(np.abs(a.cpu().numpy() - b).max() / np.abs(b).max())
                  for a, b in ((gvs, rv), (gos, ro), (gds, rd)))
 
-    # central differences on 64 vertices with the face ids fixed (only rays touching the vertex change)
+    # central differences on 64 vertices with the face ids fixed (only rays touching the vertex change). Plain
+    # central differences at h = 1e-6 carry a truncation error of order h^2 times the third derivative, visible on
+    # rays with a grazing bounce; the Richardson combination of h and h/2 removes that term and is reported beside it.
     h = 1e-6
     rng = np.random.default_rng(5)
     valid = pid != OR.INVALID
     touched = np.unique(F[pid[valid]].reshape(-1))
     verts = np.concatenate([rng.choice(touched[touched < len(V) - 8], 60, replace=False), len(V) - 1 - np.arange(4)])
     scale = 1e-3 * np.abs(gv_h).max()
-    fd_err = 0.0
+    fd_err, fd_rich_err, fd_over = 0.0, 0.0, 0
+
+    def central(vi, k, hh, idx):
+        Vp, Vm = V.copy(), V.copy()
+        Vp[vi, k] += hh
+        Vm[vi, k] -= hh
+        lp = OR.mirror_loss(Vp, F, org[idx], dirs[idx], pid[:, idx], w[:, idx], nthreads=threads)
+        lm = OR.mirror_loss(Vm, F, org[idx], dirs[idx], pid[:, idx], w[:, idx], nthreads=threads)
+        return (lp - lm).sum() / (2 * hh)
+
     for vi in verts:
         k = int(rng.integers(3))
         faces_v = np.nonzero((F == vi).any(axis=1))[0]
         fmask = np.zeros(len(F) + 1, bool)
         fmask[faces_v] = True
         idx = np.nonzero(fmask[np.where(valid, pid, len(F))].any(axis=0))[0]
-        Vp, Vm = V.copy(), V.copy()
-        Vp[vi, k] += h
-        Vm[vi, k] -= h
-        lp = OR.mirror_loss(Vp, F, org[idx], dirs[idx], pid[:, idx], w[:, idx], nthreads=threads)
-        lm = OR.mirror_loss(Vm, F, org[idx], dirs[idx], pid[:, idx], w[:, idx], nthreads=threads)
-        fd = (lp - lm).sum() / (2 * h)
-        fd_err = max(fd_err, abs(fd - gv_h[vi, k]) / max(abs(gv_h[vi, k]), scale))
+        d1 = central(vi, k, h, idx)
+        d2 = central(vi, k, h / 2, idx)
+        den = max(abs(gv_h[vi, k]), scale)
+        e1 = abs(d1 - gv_h[vi, k]) / den
+        fd_over += int(e1 > 1e-6)
+        fd_err = max(fd_err, e1)
+        fd_rich_err = max(fd_rich_err, abs((4 * d2 - d1) / 3 - gv_h[vi, k]) / den)
 
     # central differences on 64 rays, every perturbed ray re-traced on the GPU
     ridx = np.arange(0, n, n // 64)[:64]
@@ -252,7 +343,8 @@ def adjoint_leg(args, dev) -> dict:
             "commit_s": round(build_s, 3),
             "forward_bit_exact_vs_oracle": fwd_exact, "forward_sample_rays": ns,
             "max_rel_err_vs_analytic_cpu": an_err, "analytic_sample_rays": na,
-            "fd_vertices_max_rel_err": fd_err, "fd_vertices": 64,
+            "fd_vertices_max_rel_err": fd_err, "fd_vertices": 64, "fd_vertices_above_1e-6": fd_over,
+            "fd_vertices_richardson_max_rel_err": fd_rich_err,
             "fd_rays_max_rel_err": ray_err, "fd_rays": 64, "fd_ray_perturbations_flipped": flipped}
 
 
@@ -265,6 +357,7 @@ def main():
     ap.add_argument("--cpu-rays", type=int, default=1 << 22)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the mirror-tracing adjoint leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-primitive split-precision leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven multi-scene batch leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -325,7 +418,7 @@ def main():
     h64 = scene.intersect(O, D, precision=PRECISION_FP64)
     h32 = scene.intersect(O, D, precision=PRECISION_FP32_TRAV_FP64_HIT)
     torch.cuda.synchronize()
-    agree, max_rel, dis_idx = split_agreement(h64, h32)
+    agree, max_rel, dis_idx, dis_other = split_agreement(h64, h32)
 
     out = {
         "metric": METRIC,
@@ -381,10 +474,12 @@ def main():
                        "tri_tests_per_ray": round(ptests, 2)}
     out["parity"] = {"sample_rays": ns, "fp64_bit_exact_vs_oracle": exact,
                      "reference_stack_overflow_rays": int((~ok).sum())}
-    out["split_precision"]["disagreements"] = classify_disagreements(osc, org, dirs, dis_idx)
+    out["split_precision"]["disagreements"] = classify_disagreements(osc, org, dirs, dis_idx, dis_other)
     del h64, h32, O, D, CO, CD
     if not args.no_mixed:
         out["mixed_split_precision"] = mixed_leg(args, dev)
+    if not args.no_batch:
+        out["batch_uneven"] = batch_leg(args, dev)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args, dev)
     if not args.no_cpu:

@@ -9,8 +9,8 @@
 // Adjoint: one thread per ray re-derives the whole bounce chain from the stored face ids alone (t from the plane
 // of the stored face, no intermediates kept between passes), then runs the chain backwards in FP64: dL/d(origin,
 // raw direction) per ray, and dL/dvertex scatter-added per face corner. The scatter goes through a per-block LDS
-// table (one probe, keyed by vertex) so that vertices hit by many rays of a block (large walls) cost one global
-// atomic per block instead of one per ray; vertices that lose the slot race add to global memory directly.
+// table (open addressing keyed by vertex) so that vertices hit by many rays of a block (large walls) cost one global
+// atomic per block instead of one per ray; vertices that find no slot add to global memory directly.
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <stdint.h>
@@ -125,10 +125,11 @@ __device__ __forceinline__ void bounce_backward(const Bounce &B, D3 p0, D3 p1, D
   gd = (1.0 / B.s) * (gdh - dot(gdh, B.dh) * B.dh);
 }
 
-constexpr uint32_t VTAB = 512;  // LDS vertex slots per block (power of two)
+constexpr uint32_t VTAB = 1024;  // LDS vertex slots per block (power of two)
 constexpr uint32_t VTAB_EMPTY = 0xFFFFFFFFu;
+constexpr int VTAB_PROBES = 8;   // linear probing: hot vertices find a slot even when their home slot is taken
 
-__device__ __forceinline__ uint32_t vtab_slot(uint32_t v) { return (v * 2654435761u) >> 23; }  // top 9 bits
+__device__ __forceinline__ uint32_t vtab_slot(uint32_t v) { return (v * 2654435761u) >> 22; }  // top 10 bits
 
 template <int NB>
 __global__ void __launch_bounds__(256) k_mirror_adjoint(RaysView rays, const uint32_t *__restrict__ prims,
@@ -141,23 +142,27 @@ __global__ void __launch_bounds__(256) k_mirror_adjoint(RaysView rays, const uin
     acc[3 * s] = acc[3 * s + 1] = acc[3 * s + 2] = 0.0;
   }
   __syncthreads();
-  // claims the vertex's LDS slot if it is free; adds there if the slot holds the vertex, else to global memory
+  // finds or claims the vertex's LDS slot (linear probing from its hash) and adds there; a vertex that finds no
+  // slot within VTAB_PROBES (the table is full of other vertices) adds to global memory directly
   auto scatter = [&](uint32_t v, D3 gvec) {
-    const uint32_t slot = vtab_slot(v);
-    uint32_t k = keys[slot];
-    if (k == VTAB_EMPTY) {
-      k = atomicCAS(&keys[slot], VTAB_EMPTY, v);
-      if (k == VTAB_EMPTY) k = v;
+    uint32_t slot = vtab_slot(v);
+    for (int p = 0; p < VTAB_PROBES; p++) {
+      uint32_t k = keys[slot];
+      if (k == VTAB_EMPTY) {
+        k = atomicCAS(&keys[slot], VTAB_EMPTY, v);
+        if (k == VTAB_EMPTY) k = v;
+      }
+      if (k == v) {
+        unsafeAtomicAdd(&acc[3 * slot], gvec.x);
+        unsafeAtomicAdd(&acc[3 * slot + 1], gvec.y);
+        unsafeAtomicAdd(&acc[3 * slot + 2], gvec.z);
+        return;
+      }
+      slot = (slot + 1) & (VTAB - 1);
     }
-    if (k == v) {
-      unsafeAtomicAdd(&acc[3 * slot], gvec.x);
-      unsafeAtomicAdd(&acc[3 * slot + 1], gvec.y);
-      unsafeAtomicAdd(&acc[3 * slot + 2], gvec.z);
-    } else {
-      unsafeAtomicAdd(&g.gv[3ull * v], gvec.x);
-      unsafeAtomicAdd(&g.gv[3ull * v + 1], gvec.y);
-      unsafeAtomicAdd(&g.gv[3ull * v + 2], gvec.z);
-    }
+    unsafeAtomicAdd(&g.gv[3ull * v], gvec.x);
+    unsafeAtomicAdd(&g.gv[3ull * v + 1], gvec.y);
+    unsafeAtomicAdd(&g.gv[3ull * v + 2], gvec.z);
   };
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
