@@ -50,22 +50,26 @@ def time_launch(fn, steps, warmup):
     return float(np.mean(ms)), wall
 
 
-def classify_disagreements(osc, org, dirs, idx, other=None) -> dict:
+def classify_disagreements(osc, org, dirs, idx, other=None, cyl_geom=None) -> dict:
     """Re-runs the oracle (with its traversal trace) on the rays where the two precisions disagree and counts how
     many are ties (a runner-up within 1e-12 relative in the reference traversal, or a split-precision hit at the
     same distance to 1e-12), order-dependent cylinder rays (either side's hit is a cylinder: the reference accepts a
-    cylinder without a distance check, so the last one visited wins) or reference stack overflows. `other` holds
-    the split path's t and geom_id on those rays."""
+    cylinder without a distance check, so the last one visited wins, and bounds a rotated cylinder by a box that
+    can miss part of it) or reference stack overflows. `other` holds the split path's t and geom_id on those rays;
+    cyl_geom is the geometry id of the scene's cylinders (geometry ids follow the add order)."""
     if len(idx) == 0:
         return {"disagree": 0, "tie": 0, "cylinder": 0, "overflow": 0, "unexplained": 0}
     ref = osc.intersect_batch(org[idx], dirs[idx], normals=False, trace=True)
     tr = ref["trace"]
     tie = tr["tie"] != 0
-    cyl = (tr["multi_cyl"] != 0) | (ref["geom_id"] == 3)
+    cyl = tr["multi_cyl"] != 0
+    if cyl_geom is not None:
+        cyl |= ref["geom_id"] == cyl_geom
     if other is not None:
         ot, og = other
         tie |= np.abs(ot - ref["t"]) <= 1e-12 * np.abs(ref["t"])
-        cyl |= og == 3
+        if cyl_geom is not None:
+            cyl |= og == cyl_geom
     ovf = tr["overflow"] != 0
     return {"disagree": int(len(idx)), "tie": int(tie.sum()), "cylinder": int((cyl & ~tie).sum()),
             "overflow": int((ovf & ~tie & ~cyl).sum()), "unexplained": int((~tie & ~cyl & ~ovf).sum())}
@@ -98,7 +102,7 @@ def mixed_leg(args, dev) -> dict:
     for s in (scene, osc):
         s.add_mesh(*m["mesh"])
         s.add_spheres(*m["spheres"])
-        s.add_cylinders(*m["cylinders"])
+        cyl_geom = s.add_cylinders(*m["cylinders"])
         s.add_planes(*m["planes"])
     t0 = time.perf_counter()
     scene.commit()
@@ -128,7 +132,8 @@ def mixed_leg(args, dev) -> dict:
                         "incoherent rays (origins U[-22,22]^3)",
             "fp64_mrays_s": round(n / ms64 / 1e3, 3), "fp32_trav_mrays_s": round(n / ms32 / 1e3, 3),
             "speedup": round(ms64 / ms32, 3), "hit_id_agreement": agree, "max_rel_dt_agreeing": max_rel,
-            "disagreements": classify_disagreements(osc, org, dirs, idx, other), "commit_s": round(build_s, 3),
+            "disagreements": classify_disagreements(osc, org, dirs, idx, other, cyl_geom),
+            "commit_s": round(build_s, 3),
             "fp64_bit_exact_vs_oracle_sample": exact, "sample_rays": ns}
 
 

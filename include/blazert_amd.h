@@ -244,7 +244,7 @@ typedef struct brt_tune_result {
   double default_ms;             /* mean launch time of the library defaults (leaf size 4 for the leaf tuner) */
 } brt_tune_result;
 
-/* block_size {64,128,192,256} x refill {8,16,32} x stack_ring {4,8,16} x sort {NONE, OCTANT_MORTON}, plus the
+/* block_size {64,128,192,256} x refill {8,16,32,48} x stack_ring {4,8,16} x sort {NONE, OCTANT_MORTON}, plus the
  * defaults; `repeats` timed launches each (0 = 3) after one warm-up. Synchronizes `stream`. flags must not hold
  * BRT_FLAG_HITS_INOUT. */
 int brt_autotune_launch(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,

@@ -153,9 +153,13 @@ class Scene:
         self.h = lib().orc_scene_new(min_leaf_primitives, max_tree_depth, bin_size)
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().orc_scene_free(self.h)
-            self.h = None
+        h = getattr(self, "h", None)
+        self.h = None
+        if h:
+            try:
+                lib().orc_scene_free(h)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
 
     def _k(self, *arrs):
         self._keep.extend(arrs)
