@@ -19,6 +19,7 @@ struct TraceLaunch {
   unsigned long long *counter;  // persistent: device ray counter for this launch (zeroed on the stream)
   const uint32_t *order;        // optional ray permutation (sorted launches); kernels trace rays[order[k]]
   uint32_t ring = 0;            // LDS stack slots per lane: 4, 8 or 16 (0 = 8)
+  bool overwrite = false;       // every hit replaces the record (traverse semantics), not only closer ones
 };
 
 // One (scene, geometry type, ray batch) of a multi-scene launch; jobs are sorted by offset.

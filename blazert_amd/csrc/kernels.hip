@@ -1078,9 +1078,9 @@ hipError_t launch_trace(GeomType type, const BvhView &bvh, const RaysView &rays,
                         const TraceLaunch &cfg, const SplitParams &split, hipStream_t stream) {
   switch (type) {
   case TRIANGLES: return launch_typed<TriRec>(bvh, rays, hits, n, cfg, split, 1u, stream);
-  case SPHERES: return launch_typed<SphRec>(bvh, rays, hits, n, cfg, split, 0u, stream);
-  case PLANES: return launch_typed<PlaRec>(bvh, rays, hits, n, cfg, split, 0u, stream);
-  default: return launch_typed<CylRec>(bvh, rays, hits, n, cfg, split, 0u, stream);
+  case SPHERES: return launch_typed<SphRec>(bvh, rays, hits, n, cfg, split, cfg.overwrite ? 1u : 0u, stream);
+  case PLANES: return launch_typed<PlaRec>(bvh, rays, hits, n, cfg, split, cfg.overwrite ? 1u : 0u, stream);
+  default: return launch_typed<CylRec>(bvh, rays, hits, n, cfg, split, cfg.overwrite ? 1u : 0u, stream);
   }
 }
 

@@ -949,6 +949,45 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
   return BRT_OK;
 }
 
+int brt_traverse(const brt_scene *cs, uint32_t geom_type, const brt_rays *rays, const brt_hits *hits, uint64_t n,
+                 const brt_launch *launch, void *stream) {
+  brt_scene *s = const_cast<brt_scene *>(cs);
+  if (!s || !rays || !hits) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (geom_type >= NUM_TYPES) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown geometry type");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  if (!s->geo[geom_type].present) return fail(BRT_ERR_INVALID_ARGUMENT, "the scene holds no geometry of this type");
+  if (n == 0) return BRT_OK;
+  if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "ray origin / direction arrays are required");
+  if (!hits->t || !hits->prim_id || !hits->geom_id)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "hit t / prim_id / geom_id arrays are required");
+  if ((hits->nx != nullptr) != (hits->ny != nullptr) || (hits->nx != nullptr) != (hits->nz != nullptr))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "normal arrays must be all set or all NULL");
+  brt_launch L{};
+  if (launch) L = *launch;
+  if (int rc = check_launch(L)) return rc;
+  if (L.sort != BRT_SORT_NONE) return fail(BRT_ERR_UNSUPPORTED, "traverse keeps the input ray order");
+  if (L.flags & ~BRT_FLAG_ANY_HIT) return fail(BRT_ERR_INVALID_ARGUMENT, "traverse takes only BRT_FLAG_ANY_HIT");
+  const Geometry &g = s->geo[geom_type];
+  if (g.boxes.n == 0) return BRT_OK;
+  if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT && !g.extent_ok)
+    return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
+  if (int rc = upload(s)) return rc;
+  RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+  HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz, nullptr};
+  TraceLaunch cfg{L.precision, L.flags | BRT_FLAG_HITS_INOUT, L.block_size, true, L.schedule, L.refill, nullptr,
+                  nullptr};
+  cfg.ring = L.stack_ring;
+  cfg.overwrite = true;
+  if (L.schedule == BRT_SCHEDULE_PERSISTENT)
+    cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
+  SplitParams sp{g.extent};
+  hipError_t e = launch_trace((GeomType)geom_type, s->dev->views[geom_type], rv, hv, n, cfg, sp,
+                              static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+  return BRT_OK;
+}
+
 int brt_intersect1(const brt_scene *s, const double org[3], const double dir[3], double tmin, double tmax,
                    uint32_t flags, brt_hit1 *hit, int *hit_out) {
   if (!s || !org || !dir || !hit) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");

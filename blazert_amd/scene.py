@@ -164,6 +164,30 @@ class Scene:
         check(lib().brt_intersect(self._h, C.byref(rays), C.byref(H), n, C.byref(L), C.c_void_p(st.cuda_stream)))
         return hits
 
+    def traverse(self, geom_type: int, org, dirs, hits: dict, tmin=None, tmax=None, **kw) -> dict:
+        """Low-level traverse of one geometry type (blazert/bvh/accel.h:60-120) over device rays: rays that hit get
+        their record in `hits` (same layout as intersect's) overwritten, rays that miss keep it."""
+        return _traverse(self, geom_type, org, dirs, hits, tmin, tmax, **kw)
+
+
+def _traverse(scene, geom_type, org, dirs, hits, tmin=None, tmax=None, *, precision=capi.PRECISION_FP64,
+              any_hit=False, stream=None):
+    import torch
+
+    cols_o, cols_d = _columns(org), _columns(dirs)
+    n = cols_o[0].numel()
+    dev = cols_o[0].device
+    tmn = None if tmin is None else _scalar_col(tmin, n, dev)
+    tmx = None if tmax is None else _scalar_col(tmax, n, dev)
+    rays = capi.Rays(*[c.data_ptr() for c in cols_o], *[c.data_ptr() for c in cols_d],
+                     tmn.data_ptr() if tmn is not None else None, tmx.data_ptr() if tmx is not None else None)
+    L = capi.Launch(precision, capi.FLAG_ANY_HIT if any_hit else 0, 0, capi.SCHEDULE_PERSISTENT, 0, capi.SORT_NONE)
+    st = stream if stream is not None else torch.cuda.current_stream(dev)
+    keep = (cols_o, cols_d, tmn, tmx)  # noqa: F841
+    check(lib().brt_traverse(scene.handle, geom_type, C.byref(rays), C.byref(_hits_struct(hits)), n, C.byref(L),
+                             C.c_void_p(st.cuda_stream)))
+    return hits
+
 
 def _alloc_hits(n, dev, normals):
     import torch

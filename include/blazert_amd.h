@@ -173,6 +173,13 @@ int brt_scene_export_tree(const brt_scene *scene, uint32_t geom_type, brt_tree_n
 int brt_intersect(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,
                   const brt_launch *launch, void *stream);
 
+/* Low-level traverse of one geometry type's BVH (blazert/bvh/accel.h:60-120): like the reference, a ray that hits
+ * gets its record overwritten by post_traversal (t, u, v, prim_id, normal; geom_id is set to that geometry's id,
+ * which the reference leaves to intersect1), and a ray that misses keeps the record the caller passed in. launch may be
+ * NULL; flags may hold BRT_FLAG_ANY_HIT only; sort must be BRT_SORT_NONE. Asynchronous on `stream`. */
+int brt_traverse(const brt_scene *scene, uint32_t geom_type, const brt_rays *rays, const brt_hits *hits, uint64_t n,
+                 const brt_launch *launch, void *stream);
+
 /* ---- many scenes in one launch ------------------------------------------------------------------------------
  * A batch binds (scene, device rays, device hits, n) items; every geometry type present in any scene is traced by
  * one persistent launch over the concatenated rays of all items, with each hit merged into its own item exactly as

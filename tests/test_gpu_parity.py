@@ -409,3 +409,41 @@ def test_batch_graph_replays(dev, batch_set):
         b.replay(s)
         s.synchronize()
         _batch_check(b, batch_set, 1)
+
+
+def test_low_level_traverse_per_type(dev, mixed, oracle_mod):
+    """brt_traverse (blazert/bvh/accel.h:60-120) on each geometry type: hit rays get the type's post_traversal
+    record bit for bit, missed rays keep the record passed in."""
+    p, o = mixed
+    org, dirs = W.mixed_rays(4096, seed=9)
+    O, D = torch.from_numpy(org).to(dev), torch.from_numpy(dirs).to(dev)
+    geom_of_type = {0: 0, 1: 1, 3: 2, 2: 3}  # add order of the fixture: mesh, spheres, cylinders, planes
+    for typ in range(4):
+        n = len(org)
+        hits = {"t": torch.full((n,), -1.0, dtype=torch.float64, device=dev),
+                "u": torch.full((n,), -2.0, dtype=torch.float64, device=dev),
+                "v": torch.full((n,), -2.0, dtype=torch.float64, device=dev),
+                "prim_id": torch.full((n,), 123456, dtype=torch.int32, device=dev),
+                "geom_id": torch.full((n,), 7, dtype=torch.int32, device=dev),
+                "normal": torch.full((3, n), -3.0, dtype=torch.float64, device=dev)}
+        p.traverse(typ, O, D, hits)
+        torch.cuda.synchronize()
+        g = {k: hits[k].cpu().numpy() for k in ("t", "u", "v")}
+        gp = hits["prim_id"].cpu().numpy().view(np.uint32)
+        gg = hits["geom_id"].cpu().numpy().view(np.uint32)
+        gn = hits["normal"].cpu().numpy().T
+        nhit = 0
+        for i in range(n):
+            ok, h, tr = o.traverse(typ, oracle_mod.make_ray(org[i], dirs[i]))
+            if tr.overflow:
+                continue
+            if ok:
+                nhit += 1
+                assert np.float64(g["t"][i]).view(np.uint64) == np.float64(h.t).view(np.uint64), (typ, i)
+                assert np.float64(g["u"][i]).view(np.uint64) == np.float64(h.u).view(np.uint64), (typ, i)
+                assert np.float64(g["v"][i]).view(np.uint64) == np.float64(h.v).view(np.uint64), (typ, i)
+                assert gp[i] == h.prim_id and gg[i] == geom_of_type[typ], (typ, i)
+                assert np.array_equal(gn[i].view(np.uint64), np.array(h.normal[:]).view(np.uint64)), (typ, i)
+            else:
+                assert g["t"][i] == -1.0 and gp[i] == 123456 and gg[i] == 7 and (gn[i] == -3.0).all(), (typ, i)
+        assert nhit > 0, typ
