@@ -18,7 +18,7 @@ using brt::fail;
 namespace {
 
 constexpr uint32_t BLOCKS[] = {64, 128, 192, 256};
-constexpr uint32_t REFILLS[] = {8, 16, 32, 48};
+constexpr uint32_t REFILLS[] = {4, 8, 16, 32};
 constexpr uint32_t RINGS[] = {4, 8, 16};
 constexpr uint32_t SORTS[] = {BRT_SORT_NONE, BRT_SORT_OCTANT_MORTON};
 

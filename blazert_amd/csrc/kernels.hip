@@ -820,11 +820,49 @@ __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysView rays, HitsV
   }
 }
 
-// Persistent waves: every lane owns one ray at a time; when at least `refill` lanes of a wave are idle, one lane
-// claims that many ray indices from the launch's counter with a single atomic and the idle lanes start them.
-// Skewed per-ray work (many rays miss the root box, a few visit hundreds of nodes) then no longer idles the
-// wave until its slowest ray ends. Every lane leaves the loop once the counter has passed n and its own ray is
-// done, so the grid always drains.
+// Persistent waves: every lane owns one ray at a time; when at least `refill` lanes of a wave are idle, they start
+// the next indices of the wave's pool. Skewed per-ray work (many rays miss the root box, a few visit hundreds of
+// nodes) then no longer idles the wave until its slowest ray ends. The pool is a range of POOL_CHUNK indices (or
+// the number of idle lanes, if larger) claimed from the launch's counter with one atomic: every claim on that one
+// address serialises across the GPU, and claiming per refill made the counter a bottleneck at small refill
+// thresholds. Every lane leaves the loop once the counter has passed n, the pool is empty and its own ray is done,
+// so the grid always drains; at the end a wave holds at most POOL_CHUNK unstarted rays (one per lane).
+constexpr uint32_t POOL_CHUNK = 64;
+
+// Wave-uniform pool of claimed, not yet started ray indices [next, end).
+struct WavePool {
+  uint64_t next = 0, end = 0;
+  bool last = false;  // the claim that reached n has been made
+};
+
+// Called by every lane of a wave with the same pool state when the wave refills. An idle lane handed an index gets
+// it in `idx` and `got`; `exhausted` is set once nothing is left to hand out.
+__device__ __forceinline__ void pool_refill(WavePool &P, unsigned long long *counter, uint64_t n,
+                                            unsigned long long idle, uint32_t nidle, uint32_t lane, bool &got,
+                                            uint64_t &idx, bool &exhausted) {
+  got = false;
+  if (P.next == P.end && !P.last) {
+    const uint32_t want = nidle > POOL_CHUNK ? nidle : POOL_CHUNK;
+    unsigned long long base = 0;
+    const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)want);
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)base, (int)leader);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(base >> 32), (int)leader);
+    base = ((unsigned long long)hi << 32) | lo;
+    P.next = base < n ? base : n;
+    P.end = base + want < n ? base + want : n;
+    if (base + want >= n) P.last = true;
+  }
+  const uint64_t avail = P.end - P.next;
+  const uint32_t take = avail < nidle ? (uint32_t)avail : nidle;
+  const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+  if (((idle >> lane) & 1ull) && rank < take) {
+    idx = P.next + rank;
+    got = true;
+  }
+  P.next += take;
+  if (P.last && P.next == P.end) exhausted = true;
+}
 // Occupancy target of the persistent kernels (waves per SIMD the register allocation must allow).
 #ifndef BRT_WAVES_PER_EU
 #define BRT_WAVES_PER_EU 4
@@ -850,28 +888,21 @@ k_trace_persistent(BvhView bvh, RaysView rays, HitsView hits, uint64_t n,
   bool has_ray = false;
   bool exhausted = false;  // wave-uniform
   uint64_t idx = 0;
+  WavePool pool;
   for (;;) {
     if (!exhausted) {
       const unsigned long long idle = __ballot(!has_ray);
       const uint32_t nidle = (uint32_t)__popcll(idle);
       if (nidle >= refill || nidle == 64u) {
-        unsigned long long base = 0;
-        const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
-        if (lane == leader) base = atomicAdd(counter, (unsigned long long)nidle);
-        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)base, (int)leader);
-        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(base >> 32), (int)leader);
-        base = ((unsigned long long)hi << 32) | lo;
-        if (base + nidle >= n) exhausted = true;
-        if (!has_ray) {
-          const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-          const uint64_t cand = base + rank;
-          if (cand < n) {
-            idx = order ? (uint64_t)order[cand] : cand;
-            has_ray = true;
-            start_ray<SPLIT>(bvh, rays, idx, S, r, q, s);
-            trav_begin<SPLIT>(bvh, r, q, s, t);
-            st.reset();
-          }
+        bool got;
+        uint64_t cand = 0;
+        pool_refill(pool, counter, n, idle, nidle, lane, got, cand, exhausted);
+        if (got) {
+          idx = order ? (uint64_t)order[cand] : cand;
+          has_ray = true;
+          start_ray<SPLIT>(bvh, rays, idx, S, r, q, s);
+          trav_begin<SPLIT>(bvh, r, q, s, t);
+          st.reset();
         }
       }
     }
@@ -919,31 +950,24 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
   bool exhausted = false;  // wave-uniform
   uint64_t idx = 0;
   uint32_t j = 0;
+  WavePool pool;
   for (;;) {
     if (!exhausted) {
       const unsigned long long idle = __ballot(!has_ray);
       const uint32_t nidle = (uint32_t)__popcll(idle);
       if (nidle >= refill || nidle == 64u) {
-        unsigned long long base = 0;
-        const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
-        if (lane == leader) base = atomicAdd(counter, (unsigned long long)nidle);
-        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)base, (int)leader);
-        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(base >> 32), (int)leader);
-        base = ((unsigned long long)hi << 32) | lo;
-        if (base + nidle >= total) exhausted = true;
-        if (!has_ray) {
-          const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-          const uint64_t g = base + rank;
-          if (g < total) {
-            j = find_job(jobs, njobs, g);
-            const BatchJob &J = jobs[j];
-            const uint64_t local = g - J.offset;
-            idx = J.order ? (uint64_t)J.order[local] : local;
-            has_ray = true;
-            start_ray<SPLIT>(J.bvh, J.rays, idx, J.extent, r, q, s);
-            trav_begin<SPLIT>(J.bvh, r, q, s, t);
-            st.reset();
-          }
+        bool got;
+        uint64_t g = 0;
+        pool_refill(pool, counter, total, idle, nidle, lane, got, g, exhausted);
+        if (got) {
+          j = find_job(jobs, njobs, g);
+          const BatchJob &J = jobs[j];
+          const uint64_t local = g - J.offset;
+          idx = J.order ? (uint64_t)J.order[local] : local;
+          has_ray = true;
+          start_ray<SPLIT>(J.bvh, J.rays, idx, J.extent, r, q, s);
+          trav_begin<SPLIT>(J.bvh, r, q, s, t);
+          st.reset();
         }
       }
     }
@@ -1026,7 +1050,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysView &rays, const HitsVie
     const uint64_t need = (n + block - 1) / block;
     if (blocks > need) blocks = need;
     if (blocks == 0) return hipSuccess;
-    const uint32_t refill = cfg.refill ? cfg.refill : 16u;
+    const uint32_t refill = cfg.refill ? cfg.refill : 8u;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
     hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
                        hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill, cfg.order, ring);
@@ -1058,7 +1082,7 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
   const uint64_t need = (total + block - 1) / block;
   if (blocks > need) blocks = need;
   if (blocks == 0) return hipSuccess;
-  const uint32_t refill = cfg.refill ? cfg.refill : 16u;
+  const uint32_t refill = cfg.refill ? cfg.refill : 8u;
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
   hipLaunchKernelGGL((k_trace_batch<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, jobs, njobs, total,
                      cfg.flags, cfg.counter, refill, ring);

@@ -111,7 +111,7 @@ typedef struct brt_launch {
   uint32_t flags;      /* BRT_FLAG_* */
   uint32_t block_size; /* threads per block: 64, 128, 192 or 256; 0 = library default (256) */
   uint32_t schedule;   /* BRT_SCHEDULE_* */
-  uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default */
+  uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default (8) */
   uint32_t sort;       /* BRT_SORT_*: trace rays in a sorted order (results are written in input order) */
   uint32_t stack_ring; /* traversal-stack entries per lane kept in LDS: 4, 8 or 16; 0 = 8. Deeper entries spill to
                           private memory; a deeper ring spills less but takes more LDS per block */
@@ -251,7 +251,7 @@ typedef struct brt_tune_result {
   double default_ms;             /* mean launch time of the library defaults (leaf size 4 for the leaf tuner) */
 } brt_tune_result;
 
-/* block_size {64,128,192,256} x refill {8,16,32,48} x stack_ring {4,8,16} x sort {NONE, OCTANT_MORTON}, plus the
+/* block_size {64,128,192,256} x refill {4,8,16,32} x stack_ring {4,8,16} x sort {NONE, OCTANT_MORTON}, plus the
  * defaults; `repeats` timed launches each (0 = 3) after one warm-up. Synchronizes `stream`. flags must not hold
  * BRT_FLAG_HITS_INOUT. */
 int brt_autotune_launch(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,
