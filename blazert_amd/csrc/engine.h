@@ -18,7 +18,7 @@ struct TraceLaunch {
   uint32_t refill;     // persistent: refill a wave once this many lanes are idle (1..64)
   unsigned long long *counter;  // persistent: device ray counter for this launch (zeroed on the stream)
   const uint32_t *order;        // optional ray permutation (sorted launches); kernels trace rays[order[k]]
-  uint32_t ring = 0;            // LDS stack slots per lane: 4, 8 or 16 (0 = 8)
+  uint32_t ring = 0;            // LDS stack slots per lane: 4, 8 or 16 (0 = default of the precision)
   bool overwrite = false;       // every hit replaces the record (traverse semantics), not only closer ones
 };
 
