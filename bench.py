@@ -368,7 +368,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(rays are independent: the multi-GPU plan replicates the scene and shards rays, see DESIGN.md)",
+              "(rays are independent: a multi-GPU run would replicate the scene on every GPU and shard the rays)",
               file=sys.stderr)
         sys.exit(2)
 
