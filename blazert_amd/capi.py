@@ -35,7 +35,7 @@ class BuildStats(C.Structure):
     _fields_ = [("max_tree_depth", C.c_uint32), ("leaf_nodes", C.c_uint32), ("max_num_prims_per_leaf", C.c_uint32),
                 ("min_num_prims_per_leaf", C.c_uint32), ("branch_nodes", C.c_uint32), ("bad_splits", C.c_uint32),
                 ("build_time_s", C.c_double), ("num_primitives", C.c_uint32), ("geom_id", C.c_uint32),
-                ("device_bytes", C.c_uint64)]
+                ("device_bytes", C.c_uint64), ("built_on_device", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class TreeNode(C.Structure):
@@ -56,6 +56,10 @@ SCHEDULE_PER_THREAD = 1
 SORT_NONE = 0
 SORT_OCTANT_MORTON = 1
 SORT_MORTON = 2
+ARRAY_NODES64 = 0
+ARRAY_NODES4 = 1
+ARRAY_PRIMITIVES = 2
+ARRAY_PRIM_IDS = 3
 
 
 class Launch(C.Structure):
@@ -103,6 +107,9 @@ SIGNATURES = {
     "brt_scene_add_cylinders": (C.c_int, [_P, _D, _D, _D, _D, _D, C.c_uint32, _U]),
     "brt_scene_commit": (C.c_int, [_P]),
     "brt_scene_upload": (C.c_int, [_P]),
+    "brt_scene_commit_device": (C.c_int, [_P, _P]),
+    "brt_scene_read_device_array": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64,
+                                              C.POINTER(C.c_uint64)]),
     "brt_scene_build_stats": (C.c_int, [_P, C.c_uint32, C.POINTER(BuildStats)]),
     "brt_scene_export_tree": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32, _U, _U]),
     "brt_intersect": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_uint64, C.POINTER(Launch), C.c_void_p]),

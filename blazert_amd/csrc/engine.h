@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../../include/blazert_amd.h"
 #include "layout.h"
 
 namespace brt {
@@ -87,5 +88,36 @@ hipError_t launch_mirror_adjoint(const RaysView &rays, const uint32_t *prims, ui
 
 // Writes the default RayHit (t = DBL_MAX, ids invalid, u = v = 0, normal 0) for n rays.
 hipError_t launch_init_hits(const HitsView &hits, uint64_t n, hipStream_t stream);
+
+// Device build of one geometry type's BVH (build.hip): the host builder's tree, primitive order and flattened
+// records, bit for bit, provided no primitive box coordinate is NaN or -0.0 (the caller checks) and
+// device_build_supported() holds for the options.
+struct DeviceBuildIn {
+  GeomType type;
+  uint32_t n;  // primitives (> 0)
+  uint32_t min_leaf, max_depth, nbins;
+  const double *vertices;  // triangles: device vertices and faces (the boxes are computed on the device)
+  const uint32_t *faces;
+  const double *box_lo, *box_hi, *box_ctr;  // other types: host boxes and centers, 3 per primitive
+  const double *centers, *radii, *dxs, *dys, *rot, *sa, *sb, *heights;  // other types: host record inputs
+};
+// Results are device allocations owned by the caller afterwards.
+struct DeviceBuildOut {
+  brt_tree_node *tree;  // reference-form nodes in pre-order
+  uint32_t num_nodes;
+  uint32_t *order;      // leaf order -> primitive id
+  Node64 *n64;
+  uint32_t num_n64;
+  Node4 *n4;
+  uint32_t num_n4;
+  void *recs;           // primitive records in leaf order
+  size_t rec_size;
+  brt_tree_node root;   // host copy of node 0
+  uint32_t max_depth, leaves, branches, bad_splits, max_prims, min_prims;
+  uint32_t failed_line;  // on error: source line (build.hip) of the failing call
+};
+bool device_build_supported(uint32_t nbins, uint32_t max_depth);
+// Builds on `stream` and waits for it.
+hipError_t device_build(const DeviceBuildIn &in, DeviceBuildOut &out, hipStream_t stream);
 
 }  // namespace brt

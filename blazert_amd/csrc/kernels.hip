@@ -20,6 +20,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "engine.h"
+#include "f32_round.h"
 
 namespace brt {
 
@@ -138,27 +139,6 @@ struct Ray32 {
   float tmin;     // rounded down
   uint32_t sign;
 };
-
-// Adjacent float toward -inf / +inf (finite or infinite input, never NaN here).
-__device__ __forceinline__ float float_below(float f) {
-  if (f == 0.0f) return __int_as_float((int)0x80000001);
-  const int b = __float_as_int(f);
-  return __int_as_float(f > 0.0f ? b - 1 : b + 1);
-}
-__device__ __forceinline__ float float_above(float f) {
-  if (f == 0.0f) return __int_as_float(1);
-  const int b = __float_as_int(f);
-  return __int_as_float(f > 0.0f ? b + 1 : b - 1);
-}
-// FP64 -> FP32 rounded toward -inf / +inf.
-__device__ __forceinline__ float f32_down(double x) {
-  const float f = (float)x;
-  return ((double)f > x) ? float_below(f) : f;
-}
-__device__ __forceinline__ float f32_up(double x) {
-  const float f = (float)x;
-  return ((double)f < x) ? float_above(f) : f;
-}
 
 // E_i = 5 * 2^-24 * |inv_i| * (S + |o_i|) bounds |fma(b32, inv32, off32) - (b - o) inv| for every |b| <= S,
 // including the FP64 reference's own rounding and its 1 + 4 eps far-plane factor.

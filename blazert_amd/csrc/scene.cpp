@@ -73,11 +73,18 @@ inline double ref_max(double a, double b) { return (a < b) ? b : a; }  // std::m
 struct Boxes {
   std::vector<double> lo, hi, ctr;  // 3 per primitive
   uint32_t n = 0;
+  // every box coordinate is finite and none is -0.0: then every min / max the builder takes has one result
+  // whatever the evaluation order, which the device builder relies on
+  bool device_buildable = false;
   void resize(uint32_t m) {
     n = m;
     lo.assign(3ull * m, 0.0);
     hi.assign(3ull * m, 0.0);
     ctr.assign(3ull * m, 0.0);
+  }
+  void classify() {
+    auto plain = [](double x) { return std::isfinite(x) && !(x == 0.0 && std::signbit(x)); };
+    device_buildable = std::all_of(lo.begin(), lo.end(), plain) && std::all_of(hi.begin(), hi.end(), plain);
   }
 };
 
@@ -99,9 +106,14 @@ struct Geometry {
   std::vector<Node4> n4;
   std::vector<unsigned char> recs;
   size_t rec_size = 0;
+  uint32_t num_n4 = 0;
   BvhView view{};
   double extent = 0.0;  // max finite |coordinate| of all boxes (split-precision error bound)
   bool extent_ok = true;
+  // device build: the reference-form tree stays on the device (downloaded on export)
+  bool built_on_device = false;
+  const brt_tree_node *dtree = nullptr;
+  uint32_t dtree_nodes = 0;
 };
 
 void mesh_precompute(Geometry &g, uint32_t nv) {
@@ -501,6 +513,7 @@ void flatten(Geometry &g, GeomType type) {
       for (auto it = children.rbegin(); it != children.rend(); ++it) work.push_back(*it);
     }
   }
+  g.num_n4 = (uint32_t)g.n4.size();
   BvhView &v = g.view;
   v = BvhView{};
   for (int k = 0; k < 3; k++) {
@@ -588,6 +601,9 @@ unsigned hw_threads() {
 
 void build_geometry(Geometry &g, GeomType type, const brt_build_options &o) {
   const auto t0 = std::chrono::steady_clock::now();
+  g.built_on_device = false;
+  g.dtree = nullptr;
+  g.dtree_nodes = 0;
   const uint32_t n = g.boxes.n;
   g.order.resize(n);
   std::iota(g.order.begin(), g.order.end(), 0u);
@@ -607,6 +623,7 @@ void build_geometry(Geometry &g, GeomType type, const brt_build_options &o) {
   s.min_num_prims_per_leaf = b.cnt.min_prims;
   s.num_primitives = n;
   s.geom_id = g.geom_id;
+  s.built_on_device = 0;
   flatten(g, type);
   s.build_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   s.device_bytes = g.n64.size() * sizeof(Node64) + g.n4.size() * sizeof(Node4) + g.recs.size() +
@@ -674,6 +691,7 @@ using namespace brt;
 struct brt_scene {
   brt_build_options opts{4, 28, 64};
   bool committed = false;
+  bool device_commit = false;  // committed with brt_scene_commit_device (rebuilds build on the device too)
   uint32_t geometries = 0;
   Geometry geo[NUM_TYPES];
   std::unique_ptr<DeviceScene> dev;
@@ -697,6 +715,43 @@ void finish_add(brt_scene *s, uint32_t type, uint32_t *geom_id) {
   if (geom_id) *geom_id = g.geom_id;
 }
 
+hipError_t new_device_scene(int dev, std::unique_ptr<DeviceScene> &out) {
+  auto d = std::make_unique<DeviceScene>();
+  d->device = dev;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, NUM_COUNTERS * sizeof(unsigned long long));
+  if (e != hipSuccess) return e;
+  d->allocs.push_back(p);
+  d->counters = static_cast<unsigned long long *>(p);
+  out = std::move(d);
+  return hipSuccess;
+}
+
+int device_error(hipError_t e, const char *what) {
+  return fail(e == hipErrorOutOfMemory ? BRT_ERR_OUT_OF_MEMORY : BRT_ERR_DEVICE,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Copies one host-built geometry to the device.
+int put_geometry(DeviceScene &d, Geometry &g, uint32_t t) {
+  if (g.stats.max_tree_depth >= 64) return fail(BRT_ERR_UNSUPPORTED, "tree deeper than the 64-entry device stack");
+  BvhView v = g.view;
+  hipError_t e = d.put(g.n64, &v.nodes64);
+  if (e == hipSuccess) e = d.put(g.n4, &v.nodes4);
+  const unsigned char *recs = nullptr;
+  if (e == hipSuccess) e = d.put(g.recs, &recs);
+  v.prims = recs;
+  if (e == hipSuccess) e = d.put(g.order, &v.prim_ids);
+  if (t == TRIANGLES) {
+    if (e == hipSuccess) e = d.put(g.faces, &v.faces);
+    if (e == hipSuccess) e = d.put(g.vnormals, &v.vnormals);
+    if (e == hipSuccess) e = d.put(g.vertices, &v.vertices);
+  }
+  if (e != hipSuccess) return device_error(e, "upload failed");
+  d.views[t] = v;
+  return BRT_OK;
+}
+
 int upload(brt_scene *s) {
   std::lock_guard<std::mutex> lk(s->upload_mu);
   int dev = 0;
@@ -705,37 +760,143 @@ int upload(brt_scene *s) {
     if (s->dev->device != dev) return fail(BRT_ERR_STATE, "scene was uploaded to another device");
     return BRT_OK;
   }
-  auto d = std::make_unique<DeviceScene>();
-  d->device = dev;
-  {
-    void *p = nullptr;
-    hipError_t e = hipMalloc(&p, NUM_COUNTERS * sizeof(unsigned long long));
-    if (e != hipSuccess) return fail(BRT_ERR_OUT_OF_MEMORY, std::string("counter allocation: ") + hipGetErrorString(e));
-    d->allocs.push_back(p);
-    d->counters = static_cast<unsigned long long *>(p);
-  }
+  std::unique_ptr<DeviceScene> d;
+  if (hipError_t e = new_device_scene(dev, d)) return device_error(e, "counter allocation");
   for (uint32_t t = 0; t < NUM_TYPES; t++) {
     Geometry &g = s->geo[t];
     if (!g.present || g.boxes.n == 0) continue;
-    if (g.stats.max_tree_depth >= 64) return fail(BRT_ERR_UNSUPPORTED, "tree deeper than the 64-entry device stack");
-    BvhView v = g.view;
-    hipError_t e = d->put(g.n64, &v.nodes64);
-    if (e == hipSuccess) e = d->put(g.n4, &v.nodes4);
-    const unsigned char *recs = nullptr;
-    if (e == hipSuccess) e = d->put(g.recs, &recs);
-    v.prims = recs;
-    if (e == hipSuccess) e = d->put(g.order, &v.prim_ids);
-    if (t == TRIANGLES) {
-      if (e == hipSuccess) e = d->put(g.faces, &v.faces);
-      if (e == hipSuccess) e = d->put(g.vnormals, &v.vnormals);
-      if (e == hipSuccess) e = d->put(g.vertices, &v.vertices);
-    }
-    if (e != hipSuccess)
-      return fail(e == hipErrorOutOfMemory ? BRT_ERR_OUT_OF_MEMORY : BRT_ERR_DEVICE,
-                  std::string("upload failed: ") + hipGetErrorString(e));
-    d->views[t] = v;
+    if (int rc = put_geometry(*d, g, t)) return rc;
   }
   s->dev = std::move(d);
+  return BRT_OK;
+}
+
+// Builds every geometry type on the current device (brt_scene_commit_device); types whose boxes or build options
+// the device builder does not reproduce bit for bit are built on the host and uploaded. Leaves the scene uploaded.
+int commit_on_device(brt_scene *s, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(s->upload_mu);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(BRT_ERR_NO_DEVICE, "no HIP device");
+  std::unique_ptr<DeviceScene> d;
+  if (hipError_t e = new_device_scene(dev, d)) return device_error(e, "counter allocation");
+  const brt_build_options &o = s->opts;
+  for (uint32_t t = 0; t < NUM_TYPES; t++) {
+    Geometry &g = s->geo[t];
+    if (!g.present) continue;
+    g.built_on_device = false;
+    g.dtree = nullptr;
+    g.dtree_nodes = 0;
+    if (g.boxes.n == 0) {
+      build_geometry(g, (GeomType)t, o);
+      continue;
+    }
+    if (!g.boxes.device_buildable || !device_build_supported(o.bin_size, o.max_tree_depth)) {
+      build_geometry(g, (GeomType)t, o);
+      if (int rc = put_geometry(*d, g, t)) return rc;
+      continue;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    BvhView v{};
+    DeviceBuildIn in{};
+    in.type = (GeomType)t;
+    in.n = g.boxes.n;
+    in.min_leaf = o.min_leaf_primitives;
+    in.max_depth = o.max_tree_depth;
+    in.nbins = o.bin_size;
+    if (t == TRIANGLES) {
+      hipError_t e = d->put(g.faces, &v.faces);
+      if (e == hipSuccess) e = d->put(g.vnormals, &v.vnormals);
+      if (e == hipSuccess) e = d->put(g.vertices, &v.vertices);
+      if (e != hipSuccess) return device_error(e, "upload failed");
+      in.vertices = v.vertices;
+      in.faces = v.faces;
+    } else {
+      in.box_lo = g.boxes.lo.data();
+      in.box_hi = g.boxes.hi.data();
+      in.box_ctr = g.boxes.ctr.data();
+      in.centers = g.centers.data();
+      in.radii = g.radii.data();
+      in.dxs = g.dxs.data();
+      in.dys = g.dys.data();
+      in.rot = g.rot.data();
+      in.sa = g.sa.data();
+      in.sb = g.sb.data();
+      in.heights = g.heights.data();
+    }
+    DeviceBuildOut r{};
+    hipError_t e = device_build(in, r, st);
+    if (e != hipSuccess)
+      return device_error(e, ("device build failed (build.hip:" + std::to_string(r.failed_line) + ")").c_str());
+    for (void *p : {(void *)r.tree, (void *)r.order, (void *)r.n64, (void *)r.n4, r.recs}) d->allocs.push_back(p);
+    g.tree.clear();
+    g.order.clear();
+    g.n64.clear();
+    g.n4.clear();
+    g.recs.clear();
+    g.rec_size = r.rec_size;
+    g.num_n4 = r.num_n4;
+    g.built_on_device = true;
+    g.dtree = r.tree;
+    g.dtree_nodes = r.num_nodes;
+    brt_build_stats &bs = g.stats;
+    bs = brt_build_stats{};
+    bs.max_tree_depth = r.max_depth;
+    bs.leaf_nodes = r.leaves;
+    bs.branch_nodes = r.branches;
+    bs.bad_splits = r.bad_splits;
+    bs.max_num_prims_per_leaf = r.max_prims;
+    bs.min_num_prims_per_leaf = r.min_prims;
+    bs.num_primitives = g.boxes.n;
+    bs.geom_id = g.geom_id;
+    bs.built_on_device = 1;
+    // root descriptor as the host flatten writes it; all node boxes lie inside the (finite) root box
+    const brt_tree_node &root = r.root;
+    double extent = 0.0;
+    for (int k = 0; k < 3; k++) {
+      v.root_min[k] = root.min[k];
+      v.root_max[k] = root.max[k];
+      v.root_min32[k] = f32_down(root.min[k]);
+      v.root_max32[k] = f32_up(root.max[k]);
+      extent = std::max(extent, std::max(std::fabs(root.min[k]), std::fabs(root.max[k])));
+    }
+    v.root_ref = root.leaf ? (LEAF_BIT | root.prim_begin) : 0u;
+    v.root_cnt = root.leaf ? root.prim_count : 0u;
+    v.root_ref4 = v.root_ref;
+    v.geom_id = g.geom_id;
+    v.n_prims = g.boxes.n;
+    v.n_nodes = r.num_n64;
+    v.depth = r.max_depth;
+    v.nan_boxes = 0;
+    g.view = v;
+    g.extent = extent;
+    g.extent_ok = extent < 1e30;
+    if (r.max_depth >= 64) return fail(BRT_ERR_UNSUPPORTED, "tree deeper than the 64-entry device stack");
+    v.nodes64 = r.n64;
+    v.nodes4 = r.n4;
+    v.prims = r.recs;
+    v.prim_ids = r.order;
+    d->views[t] = v;
+    bs.build_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    bs.device_bytes = (uint64_t)r.num_n64 * sizeof(Node64) + (uint64_t)r.num_n4 * sizeof(Node4) +
+                      (uint64_t)g.boxes.n * (r.rec_size + 4) + g.faces.size() * 4 + g.vnormals.size() * 8 +
+                      g.vertices.size() * 8;
+  }
+  s->dev = std::move(d);
+  return BRT_OK;
+}
+
+// Reference-form tree and leaf order of a device-built geometry, fetched once for export.
+int download_tree(brt_scene *s, Geometry &g, uint32_t t) {
+  if (!g.built_on_device || !g.tree.empty()) return BRT_OK;
+  if (hipSetDevice(s->dev->device) != hipSuccess) return fail(BRT_ERR_DEVICE, "hipSetDevice failed");
+  std::vector<brt_tree_node> tree(g.dtree_nodes);
+  std::vector<uint32_t> order(g.boxes.n);
+  hipError_t e = hipMemcpy(tree.data(), g.dtree, tree.size() * sizeof(brt_tree_node), hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    e = hipMemcpy(order.data(), s->dev->views[t].prim_ids, order.size() * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return device_error(e, "tree download failed");
+  g.tree.swap(tree);
+  g.order.swap(order);
   return BRT_OK;
 }
 
@@ -798,6 +959,7 @@ int brt_scene_add_mesh(brt_scene *s, const double *vertices, uint32_t nv, const 
   g.vertices.assign(vertices, vertices + 3ull * nv);
   g.faces.assign(faces, faces + 3ull * nf);
   mesh_precompute(g, nv);
+  g.boxes.classify();
   finish_add(s, TRIANGLES, geom_id);
   return BRT_OK;
 }
@@ -811,6 +973,7 @@ int brt_scene_add_spheres(brt_scene *s, const double *centers, const double *rad
   g.centers.assign(centers, centers + 3ull * n);
   g.radii.assign(radii, radii + n);
   sphere_precompute(g);
+  g.boxes.classify();
   finish_add(s, SPHERES, geom_id);
   return BRT_OK;
 }
@@ -827,6 +990,7 @@ int brt_scene_add_planes(brt_scene *s, const double *centers, const double *dxs,
   g.dys.assign(dys, dys + n);
   g.rot.assign(rotations, rotations + 9ull * n);
   plane_precompute(g);
+  g.boxes.classify();
   finish_add(s, PLANES, geom_id);
   return BRT_OK;
 }
@@ -845,6 +1009,7 @@ int brt_scene_add_cylinders(brt_scene *s, const double *centers, const double *a
   g.heights.assign(heights, heights + n);
   g.rot.assign(rotations, rotations + 9ull * n);
   cylinder_precompute(g);
+  g.boxes.classify();
   finish_add(s, CYLINDERS, geom_id);
   return BRT_OK;
 }
@@ -855,6 +1020,42 @@ int brt_scene_commit(brt_scene *s) {
   for (uint32_t t = 0; t < NUM_TYPES; t++)
     if (s->geo[t].present) build_geometry(s->geo[t], (GeomType)t, s->opts);
   s->committed = true;
+  return BRT_OK;
+}
+
+int brt_scene_commit_device(brt_scene *s, void *stream) {
+  if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (s->committed) return BRT_OK;
+  if (int rc = commit_on_device(s, static_cast<hipStream_t>(stream))) return rc;
+  s->committed = true;
+  s->device_commit = true;
+  return BRT_OK;
+}
+
+int brt_scene_read_device_array(const brt_scene *cs, uint32_t type, uint32_t which, void *dst, uint64_t capacity,
+                                uint64_t *bytes) {
+  brt_scene *s = const_cast<brt_scene *>(cs);
+  if (!s || type >= NUM_TYPES || !bytes || which > BRT_ARRAY_PRIM_IDS)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  const Geometry &g = s->geo[type];
+  if (!g.present) return fail(BRT_ERR_INVALID_ARGUMENT, "geometry type not in scene");
+  *bytes = 0;
+  if (g.boxes.n == 0) return BRT_OK;
+  if (int rc = upload(s)) return rc;
+  const BvhView &v = s->dev->views[type];
+  const void *src = nullptr;
+  switch (which) {
+  case BRT_ARRAY_NODES64: src = v.nodes64; *bytes = (uint64_t)v.n_nodes * sizeof(Node64); break;
+  case BRT_ARRAY_NODES4: src = v.nodes4; *bytes = (uint64_t)g.num_n4 * sizeof(Node4); break;
+  case BRT_ARRAY_PRIMITIVES: src = v.prims; *bytes = (uint64_t)g.boxes.n * g.rec_size; break;
+  default: src = v.prim_ids; *bytes = (uint64_t)g.boxes.n * 4; break;
+  }
+  if (!dst || *bytes == 0) return BRT_OK;
+  if (capacity < *bytes) return fail(BRT_ERR_INVALID_ARGUMENT, "capacity too small");
+  if (hipSetDevice(s->dev->device) != hipSuccess) return fail(BRT_ERR_DEVICE, "hipSetDevice failed");
+  hipError_t e = hipMemcpy(dst, src, *bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return device_error(e, "read failed");
   return BRT_OK;
 }
 
@@ -876,8 +1077,9 @@ int brt_scene_export_tree(const brt_scene *s, uint32_t type, brt_tree_node *node
                           uint32_t *prim_order, uint32_t *num_nodes) {
   if (!s || type >= NUM_TYPES || !num_nodes) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
-  const Geometry &g = s->geo[type];
+  Geometry &g = const_cast<brt_scene *>(s)->geo[type];
   if (!g.present) return fail(BRT_ERR_INVALID_ARGUMENT, "geometry type not in scene");
+  if (int rc = download_tree(const_cast<brt_scene *>(s), g, type)) return rc;
   *num_nodes = (uint32_t)g.tree.size();
   if (nodes) {
     if (capacity < g.tree.size()) return fail(BRT_ERR_INVALID_ARGUMENT, "capacity too small");
@@ -1272,11 +1474,18 @@ int scene_rebuild(brt_scene *s, const brt_build_options &opts) {
   if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
   if (opts.min_leaf_primitives == 0 || opts.bin_size == 0)
     return fail(BRT_ERR_INVALID_ARGUMENT, "leaf size and bin count must be positive");
-  std::lock_guard<std::mutex> lk(s->upload_mu);
-  s->dev.reset();
-  s->opts = opts;
-  for (uint32_t t = 0; t < NUM_TYPES; t++)
-    if (s->geo[t].present) build_geometry(s->geo[t], (GeomType)t, s->opts);
+  {
+    std::lock_guard<std::mutex> lk(s->upload_mu);
+    s->dev.reset();
+    s->opts = opts;
+  }
+  if (s->device_commit) {
+    s->committed = false;
+    if (int rc = commit_on_device(s, nullptr)) return rc;
+  } else {
+    for (uint32_t t = 0; t < NUM_TYPES; t++)
+      if (s->geo[t].present) build_geometry(s->geo[t], (GeomType)t, s->opts);
+  }
   s->committed = true;
   return BRT_OK;
 }

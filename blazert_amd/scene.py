@@ -98,10 +98,27 @@ class Scene:
             raise ValueError("cylinder arrays must have the same length")
         return self._add(lib().brt_scene_add_cylinders, _dp(c), _dp(a), _dp(b), _dp(h), _dp(R), len(a))
 
-    def commit(self) -> bool:
-        check(lib().brt_scene_commit(self._h))
+    def commit(self, device: bool = False, stream=None) -> bool:
+        """Builds the BVHs (blazert/scene.h:91-115). device=True builds them on the current GPU
+        (brt_scene_commit_device: the same trees and records, bit for bit) and leaves the scene uploaded."""
+        if device:
+            import torch
+            st = torch.cuda.current_stream() if stream is None else stream
+            check(lib().brt_scene_commit_device(self._h, C.c_void_p(st.cuda_stream)))
+        else:
+            check(lib().brt_scene_commit(self._h))
         self.has_been_committed = True
         return True
+
+    def read_device_array(self, geom_type: int, which: int) -> np.ndarray:
+        """Raw bytes of one device array (capi.ARRAY_*) of a committed geometry type."""
+        nbytes = C.c_uint64(0)
+        check(lib().brt_scene_read_device_array(self._h, geom_type, which, None, 0, C.byref(nbytes)))
+        out = np.zeros(nbytes.value, dtype=np.uint8)
+        if nbytes.value:
+            check(lib().brt_scene_read_device_array(self._h, geom_type, which, out.ctypes.data_as(C.c_void_p),
+                                                    nbytes.value, C.byref(nbytes)))
+        return out
 
     def upload(self) -> None:
         check(lib().brt_scene_upload(self._h))

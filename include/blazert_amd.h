@@ -68,6 +68,8 @@ typedef struct brt_build_stats {
   uint32_t num_primitives;
   uint32_t geom_id;
   uint64_t device_bytes; /* nodes (FP64 + FP32) + primitive records + ids + post-processing data */
+  uint32_t built_on_device; /* 1: built by brt_scene_commit_device's GPU builder, 0: by the host builder */
+  uint32_t reserved;
 } brt_build_stats;
 
 /* Reference-form tree node (pre-order numbering as built by blazert/bvh/builder.h:84-115). */
@@ -157,6 +159,24 @@ int brt_scene_add_cylinders(brt_scene *scene, const double *centers, const doubl
 int brt_scene_commit(brt_scene *scene);
 /* Copies the committed scene to the current HIP device (idempotent). */
 int brt_scene_upload(brt_scene *scene);
+
+/* commit (blazert/scene.h:91-115) with the BVHs built on the current HIP device, enqueued on `stream` and waited
+ * for; leaves the scene uploaded. Extension: the reference builds on the host (blazert/bvh/builder.h:46-226). The
+ * trees, leaf order and device records are those of brt_scene_commit, bit for bit: a geometry type whose boxes
+ * hold a NaN, an infinity or a negative zero, or whose options have more than 64 bins or a depth limit above 62,
+ * is built by the host builder instead (brt_build_stats.built_on_device tells which). Rebuilds by the autotuner
+ * stay on the device. */
+int brt_scene_commit_device(brt_scene *scene, void *stream);
+
+/* Device arrays of one committed geometry type, for inspection (brt_scene_read_device_array). */
+#define BRT_ARRAY_NODES64 0u    /* 128-byte FP64 records, children in parent */
+#define BRT_ARRAY_NODES4 1u     /* 128-byte FP32 4-wide records (split precision) */
+#define BRT_ARRAY_PRIMITIVES 2u /* primitive records in leaf order */
+#define BRT_ARRAY_PRIM_IDS 3u   /* uint32 leaf order -> primitive id */
+/* Copies one device array to host memory dst (capacity bytes); *bytes receives its size (dst may be NULL to
+ * query it). Uploads a host-committed scene first. Extension (the reference keeps no device copy). */
+int brt_scene_read_device_array(const brt_scene *scene, uint32_t geom_type, uint32_t which, void *dst,
+                                uint64_t capacity, uint64_t *bytes);
 
 /* Statistics of one geometry type's build (BRT_ERR_INVALID_ARGUMENT if the type is absent). */
 int brt_scene_build_stats(const brt_scene *scene, uint32_t geom_type, brt_build_stats *stats);
