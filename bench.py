@@ -353,6 +353,32 @@ def adjoint_leg(args, dev) -> dict:
             "fd_rays_max_rel_err": ray_err, "fd_rays": 64, "fd_ray_perturbations_flipped": flipped}
 
 
+def device_commit_leg(v, f, host_scene, host_s) -> dict:
+    """commit() built on the GPU (brt_scene_commit_device) for the same mesh: one warm-up build, then the timed
+    one; the device arrays must equal the host-built scene's byte for byte."""
+    import torch
+
+    from blazert_amd import Scene, capi
+    warm = Scene()
+    warm.add_mesh(v, f)
+    warm.commit(device=True)
+    warm.close()
+    d = Scene()
+    d.add_mesh(v, f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.commit(device=True)
+    dev_s = time.perf_counter() - t0
+    same = all(np.array_equal(host_scene.read_device_array(0, a), d.read_device_array(0, a))
+               for a in (capi.ARRAY_NODES64, capi.ARRAY_NODES4, capi.ARRAY_PRIMITIVES, capi.ARRAY_PRIM_IDS))
+    same = same and d.build_stats(0)["built_on_device"] == 1
+    d.close()
+    log(f"device commit {dev_s * 1e3:.1f} ms (host {host_s * 1e3:.1f} ms), identical arrays: {same}")
+    return {"device_ms": round(dev_s * 1e3, 2), "host_ms": round(host_s * 1e3, 2),
+            "speedup": round(host_s / dev_s, 2), "identical_device_arrays": same,
+            "host_threads": len(os.sched_getaffinity(0))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -481,6 +507,7 @@ def main():
                      "reference_stack_overflow_rays": int((~ok).sum())}
     out["split_precision"]["disagreements"] = classify_disagreements(osc, org, dirs, dis_idx, dis_other)
     del h64, h32, O, D, CO, CD
+    out["device_commit"] = device_commit_leg(v, f, scene, build_s)
     if not args.no_mixed:
         out["mixed_split_precision"] = mixed_leg(args, dev)
     if not args.no_batch:
