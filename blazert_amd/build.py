@@ -1,6 +1,6 @@
 """Builds libblazert_amd.so in-tree with hipcc for gfx950.
 
-Sources: csrc/kernels.hip, csrc/mirror.hip and csrc/build.hip (device code), csrc/scene.cpp (host builder + C ABI) and
+Sources: csrc/kernels.hip, csrc/mirror.hip, csrc/build.hip and csrc/distance.hip (device code), csrc/scene.cpp (host builder + C ABI) and
 csrc/autotune.cpp (launch / leaf-size autotuner). Both are compiled with
 -ffp-contract=off so that no multiply-add is fused (the results must round exactly like the reference's
 separate multiply and add), and device code with packed-FP32 instructions disabled.
@@ -45,7 +45,7 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     tmp = tempfile.mkdtemp(prefix="brt_build_")
     try:
         objs = []
-        for src in ("kernels.hip", "mirror.hip", "build.hip"):
+        for src in ("kernels.hip", "mirror.hip", "build.hip", "distance.hip"):
             obj = os.path.join(tmp, src.replace(".hip", ".o"))
             _run([cc, *COMMON, *DEVICE, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj])
             objs.append(obj)

@@ -43,6 +43,10 @@ class TreeNode(C.Structure):
                 ("children", C.c_uint32 * 2), ("prim_begin", C.c_uint32), ("prim_count", C.c_uint32)]
 
 
+class Points(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p)]
+
+
 class Rays(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")]
 
@@ -108,6 +112,8 @@ SIGNATURES = {
     "brt_scene_commit": (C.c_int, [_P]),
     "brt_scene_upload": (C.c_int, [_P]),
     "brt_scene_commit_device": (C.c_int, [_P, _P]),
+    "brt_distance_to_surface": (C.c_int, [_P, C.c_uint32, C.POINTER(Points), _P, C.c_uint64, _P, _P]),
+    "brt_closest_surface": (C.c_int, [_P, C.POINTER(Points), C.c_uint64, _P, _P, _P, _P]),
     "brt_scene_read_device_array": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64,
                                               C.POINTER(C.c_uint64)]),
     "brt_scene_build_stats": (C.c_int, [_P, C.c_uint32, C.POINTER(BuildStats)]),

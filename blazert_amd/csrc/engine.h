@@ -89,6 +89,21 @@ hipError_t launch_mirror_adjoint(const RaysView &rays, const uint32_t *prims, ui
 // Writes the default RayHit (t = DBL_MAX, ids invalid, u = v = 0, normal 0) for n rays.
 hipError_t launch_init_hits(const HitsView &hits, uint64_t n, hipStream_t stream);
 
+// Distance queries (distance.hip). inv maps primitive ids to leaf positions (spheres, planes, cylinders; built by
+// launch_inverse_order from the view's prim_ids); ids >= n_prims give NaN.
+hipError_t launch_inverse_order(const uint32_t *order, uint32_t n, uint32_t *inv, hipStream_t stream);
+hipError_t launch_distance_pairs(GeomType type, const BvhView &bvh, const uint32_t *inv, const double *px,
+                                 const double *py, const double *pz, const uint32_t *ids, uint64_t n, double *out,
+                                 hipStream_t stream);
+// All geometry types of a scene for the closest-surface query; extent = largest |coordinate| of a type's boxes.
+struct ClosestViews {
+  BvhView view[NUM_TYPES];
+  double extent[NUM_TYPES];
+  uint32_t present[NUM_TYPES];
+};
+hipError_t launch_closest_surface(const ClosestViews &views, const double *px, const double *py, const double *pz,
+                                  uint64_t n, double *dist, uint32_t *prim, uint32_t *geom, hipStream_t stream);
+
 // Device build of one geometry type's BVH (build.hip): the host builder's tree, primitive order and flattened
 // records, bit for bit, provided no primitive box coordinate is NaN or -0.0 (the caller checks) and
 // device_build_supported() holds for the options.

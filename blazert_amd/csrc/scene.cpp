@@ -639,6 +639,8 @@ struct DeviceScene {
   BvhView views[NUM_TYPES]{};
   unsigned long long *counters = nullptr;  // ray counters of the persistent schedule, one per launch in flight
   std::atomic<uint32_t> next_counter{0};
+  std::mutex inv_mu;
+  uint32_t *inv_order[NUM_TYPES]{};  // primitive id -> leaf position, made on the first distance query
   std::mutex sort_mu;
   std::vector<std::pair<hipStream_t, std::unique_ptr<SortWorkspace>>> sorts;  // one ray-sort workspace per stream
   ~DeviceScene() {
@@ -1056,6 +1058,63 @@ int brt_scene_read_device_array(const brt_scene *cs, uint32_t type, uint32_t whi
   if (hipSetDevice(s->dev->device) != hipSuccess) return fail(BRT_ERR_DEVICE, "hipSetDevice failed");
   hipError_t e = hipMemcpy(dst, src, *bytes, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return device_error(e, "read failed");
+  return BRT_OK;
+}
+
+int brt_distance_to_surface(const brt_scene *cs, uint32_t type, const brt_points *points, const uint32_t *prim_ids,
+                            uint64_t n, double *out, void *stream) {
+  brt_scene *s = const_cast<brt_scene *>(cs);
+  if (!s || !points || type >= NUM_TYPES) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  if (!s->geo[type].present) return fail(BRT_ERR_INVALID_ARGUMENT, "the scene holds no geometry of this type");
+  if (n == 0) return BRT_OK;
+  if (!points->x || !points->y || !points->z || !prim_ids || !out)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "point, id and output arrays are required");
+  if (int rc = upload(s)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const Geometry &g = s->geo[type];
+  BvhView v = s->dev->views[type];
+  v.n_prims = g.boxes.n;  // an empty geometry has no device arrays: every id is out of range
+  const uint32_t *inv = nullptr;
+  if (type != TRIANGLES && g.boxes.n) {
+    std::lock_guard<std::mutex> lk(s->dev->inv_mu);
+    uint32_t *&slot = s->dev->inv_order[type];
+    if (!slot) {
+      void *p = nullptr;
+      hipError_t e = hipMalloc(&p, (size_t)g.boxes.n * 4);
+      if (e != hipSuccess) return device_error(e, "inverse order allocation");
+      s->dev->allocs.push_back(p);
+      e = launch_inverse_order(v.prim_ids, g.boxes.n, static_cast<uint32_t *>(p), st);
+      if (e != hipSuccess) return device_error(e, "launch failed");
+      slot = static_cast<uint32_t *>(p);
+    }
+    inv = slot;
+  }
+  hipError_t e = launch_distance_pairs((GeomType)type, v, inv, points->x, points->y, points->z, prim_ids, n, out, st);
+  if (e != hipSuccess) return device_error(e, "launch failed");
+  return BRT_OK;
+}
+
+int brt_closest_surface(const brt_scene *cs, const brt_points *points, uint64_t n, double *distance,
+                        uint32_t *prim_id, uint32_t *geom_id, void *stream) {
+  brt_scene *s = const_cast<brt_scene *>(cs);
+  if (!s || !points) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  if (n == 0) return BRT_OK;
+  if (!points->x || !points->y || !points->z || !distance || !prim_id || !geom_id)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "point and output arrays are required");
+  if (int rc = upload(s)) return rc;
+  ClosestViews cv{};
+  for (uint32_t t = 0; t < NUM_TYPES; t++) {
+    const Geometry &g = s->geo[t];
+    cv.present[t] = (g.present && g.boxes.n) ? 1u : 0u;
+    if (!cv.present[t]) continue;
+    cv.view[t] = s->dev->views[t];
+    cv.extent[t] = g.extent_ok ? g.extent : INFINITY;
+  }
+  hipError_t e = launch_closest_surface(cv, points->x, points->y, points->z, n, distance, prim_id, geom_id,
+                                        static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return device_error(e, "launch failed");
   return BRT_OK;
 }
 

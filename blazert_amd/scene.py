@@ -181,6 +181,39 @@ class Scene:
         check(lib().brt_intersect(self._h, C.byref(rays), C.byref(H), n, C.byref(L), C.c_void_p(st.cuda_stream)))
         return hits
 
+    # -------------------------------------------------------------------------------------------- distances
+    def distance_to_surface(self, geom_type: int, points, prim_ids, stream=None):
+        """distance_to_surface(primitive prim_ids[i], point i) for device points ((n,3) or three columns) and int32
+        ids of one geometry type; FP64 tensor, NaN for ids out of range (brt_distance_to_surface)."""
+        import torch
+        cols = _columns(points)
+        n = cols[0].numel()
+        ids = torch.as_tensor(prim_ids).to(device=cols[0].device, dtype=torch.int32).contiguous()
+        if ids.numel() != n:
+            raise ValueError("one primitive id per point")
+        out = torch.empty(n, dtype=torch.float64, device=cols[0].device)
+        st = stream if stream is not None else torch.cuda.current_stream(cols[0].device)
+        pts = capi.Points(*[c.data_ptr() for c in cols])
+        check(lib().brt_distance_to_surface(self._h, geom_type, C.byref(pts), C.c_void_p(ids.data_ptr()), n,
+                                            C.c_void_p(out.data_ptr()), C.c_void_p(st.cuda_stream)))
+        return out
+
+    def closest_surface(self, points, stream=None) -> dict:
+        """Closest surface of the scene for each device point: distance, prim_id, geom_id (brt_closest_surface)."""
+        import torch
+        cols = _columns(points)
+        n = cols[0].numel()
+        dev = cols[0].device
+        out = {"distance": torch.empty(n, dtype=torch.float64, device=dev),
+               "prim_id": torch.empty(n, dtype=torch.int32, device=dev),
+               "geom_id": torch.empty(n, dtype=torch.int32, device=dev)}
+        st = stream if stream is not None else torch.cuda.current_stream(dev)
+        pts = capi.Points(*[c.data_ptr() for c in cols])
+        check(lib().brt_closest_surface(self._h, C.byref(pts), n, *[C.c_void_p(out[k].data_ptr())
+                                                                   for k in ("distance", "prim_id", "geom_id")],
+                                        C.c_void_p(st.cuda_stream)))
+        return out
+
     def traverse(self, geom_type: int, org, dirs, hits: dict, tmin=None, tmax=None, **kw) -> dict:
         """Low-level traverse of one geometry type (blazert/bvh/accel.h:60-120) over device rays: rays that hit get
         their record in `hits` (same layout as intersect's) overwritten, rays that miss keep it."""

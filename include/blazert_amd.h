@@ -186,6 +186,26 @@ int brt_scene_build_stats(const brt_scene *scene, uint32_t geom_type, brt_build_
 int brt_scene_export_tree(const brt_scene *scene, uint32_t geom_type, brt_tree_node *nodes, uint32_t capacity,
                           uint32_t *prim_order, uint32_t *num_nodes);
 
+/* Device points, structure of arrays (device pointers). */
+typedef struct brt_points {
+  const double *x, *y, *z;
+} brt_points;
+
+/* distance_to_surface(primitive_from_collection(collection, prim_ids[i]), point i) of the reference for n pairs on
+ * the device (sphere.h:179-182, plane.h:272-284, cylinder.h:677-759): out[i] in FP64, bit-identical to the reference
+ * arithmetic; NaN for an id >= the type's primitive count. Triangles: the distance to the closest point of
+ * trimesh_distance.h:176-221 on the face (the reference's triangle overload, trimesh.h:183-191, does not compile
+ * when instantiated). Asynchronous on `stream`. */
+int brt_distance_to_surface(const brt_scene *scene, uint32_t geom_type, const brt_points *points,
+                            const uint32_t *prim_ids, uint64_t n, double *out, void *stream);
+
+/* Closest surface of the whole scene for n device points: the smallest distance_to_surface over every primitive,
+ * ties to the earlier geometry type (triangles, spheres, planes, cylinders) and then the lower primitive id; +inf
+ * and BRT_INVALID_ID when no primitive gives a distance. Extension: the reference has no scene-level distance query
+ * (docs/source/roadmap.rst). Asynchronous on `stream`. */
+int brt_closest_surface(const brt_scene *scene, const brt_points *points, uint64_t n, double *distance,
+                        uint32_t *prim_id, uint32_t *geom_id, void *stream);
+
 /* Batched intersect1 (blazert/scene.h:161-211) for n rays resident on the device, enqueued on `stream`
  * (a hipStream_t, NULL = default stream). launch may be NULL (FP64, no flags, persistent). Asynchronous.
  * The persistent schedule uses one of 256 per-scene device counters per launch, taken round robin, so more than

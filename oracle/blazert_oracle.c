@@ -315,6 +315,40 @@ int orc_prim_center(const orc_scene *s, int type, uint32_t prim, double ctr[3]) 
   return 0;
 }
 
+/* parameters of one primitive (user order): triangle v0 v1 v2 (9); sphere c r (4); plane c dx dy R (14);
+ * cylinder c a b h R (15) */
+int orc_prim_params(const orc_scene *s, int type, uint32_t prim, double *out) {
+  const orc_coll *c = &s->c[type];
+  if (!c->present || prim >= c->n) return -1;
+  const size_t i = prim;
+  switch (type) {
+  case ORC_TRIANGLES:
+    for (int j = 0; j < 3; j++) memcpy(out + 3 * j, &c->vertices[3 * (size_t)c->faces[3 * i + j]], 24);
+    break;
+  case ORC_SPHERES:
+    memcpy(out, &c->centers[3 * i], 24);
+    out[3] = c->radii[i];
+    break;
+  case ORC_PLANES:
+    memcpy(out, &c->centers[3 * i], 24);
+    out[3] = c->dxs[i];
+    out[4] = c->dys[i];
+    memcpy(out + 5, &c->rot[9 * i], 72);
+    break;
+  default:
+    memcpy(out, &c->centers[3 * i], 24);
+    out[3] = c->sa[i];
+    out[4] = c->sb[i];
+    out[5] = c->heights[i];
+    memcpy(out + 6, &c->rot[9 * i], 72);
+    break;
+  }
+  return 0;
+}
+
+uint32_t orc_prim_count(const orc_scene *s, int type) { return s->c[type].present ? s->c[type].n : 0u; }
+uint32_t orc_geom_id(const orc_scene *s, int type) { return s->geom_id[type]; }
+
 /* ------------------------------------------------------------------------------------------------ */
 /* binned SAH builder                                                                                */
 /* ------------------------------------------------------------------------------------------------ */

@@ -16,6 +16,7 @@
  *   planes                  blazert/primitives/plane.h:71-270
  *   cylinders               blazert/primitives/cylinder.h:74-495
  *   scene merge             blazert/scene.h:161-211
+ *   distance_to_surface     distance_oracle.c (sphere.h, plane.h, cylinder.h, trimesh_distance.h)
  *
  * Arithmetic conventions (blaze 3.7 is not available, so these are restated from its published
  * semantics and documented as unpinned at the bit level):
@@ -92,6 +93,11 @@ int orc_commit(orc_scene *s);
 int orc_prim_box(const orc_scene *s, int type, uint32_t prim, double bmin[3], double bmax[3]);
 int orc_prim_center(const orc_scene *s, int type, uint32_t prim, double c[3]);
 
+/* primitive parameters in user order (triangle v0 v1 v2; sphere c r; plane c dx dy R; cylinder c a b h R) */
+int orc_prim_params(const orc_scene *s, int type, uint32_t prim, double *out);
+uint32_t orc_prim_count(const orc_scene *s, int type);
+uint32_t orc_geom_id(const orc_scene *s, int type);
+
 /* built tree export (reference pre-order layout) */
 uint32_t orc_tree_size(const orc_scene *s, int type);
 int orc_tree_export(const orc_scene *s, int type, orc_node *nodes, uint32_t *prim_order, orc_stats *stats);
@@ -136,6 +142,23 @@ void orc_mirror_grad(const double *V, const uint32_t *F, uint64_t n, const doubl
                      const double *oz, const double *dx, const double *dy, const double *dz, uint32_t bounces,
                      const uint32_t *prims, const double *gt, const double *gpx, const double *gpy,
                      const double *gpz, double *gv, double *go, double *gd, int nthreads);
+
+/* ---- distance_to_surface (distance_oracle.c) ----
+ * sphere.h:179-182, plane.h:272-284, cylinder.h:507-759, trimesh_distance.h:36-221 (the triangle distance is
+ * norm(p - closest_point_on_triangle); the reference's own triangle overload cannot be instantiated). */
+double orc_sphere_distance(const double c[3], double r, const double p[3]);
+double orc_plane_distance(const double c[3], double dx, double dy, const double R[9], const double p[3]);
+double orc_cylinder_distance(const double c[3], double a, double b, double h, const double R[9], const double p[3]);
+void orc_closest_point_on_triangle(const double v0[3], const double v1[3], const double v2[3], const double p[3],
+                                   double q[3]);
+double orc_triangle_distance(const double v0[3], const double v1[3], const double v2[3], const double p[3]);
+/* distance from point i to primitive prims[i] (user id) of one geometry type; NaN for an id out of range */
+void orc_distance_batch(const orc_scene *s, int type, uint64_t n, const double *px, const double *py,
+                        const double *pz, const uint32_t *prims, double *out, int nthreads);
+/* brute-force closest surface over every primitive of the scene: smallest distance, ties to the earlier geometry
+ * type then the lower primitive id; +inf and invalid ids when every distance is NaN */
+void orc_closest_batch(const orc_scene *s, uint64_t n, const double *px, const double *py, const double *pz,
+                       double *dist, uint32_t *prim, uint32_t *geom, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -2,7 +2,8 @@
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg import this module, and only
 as the checker or the timed CPU baseline; the product package ``blazert_amd`` never imports it.
-The oracle restates blazeRT's ``intersect1`` path (blazert/scene.h:161-211 and the functions it calls).
+The oracle restates blazeRT's ``intersect1`` path (blazert/scene.h:161-211 and the functions it calls) and its
+``distance_to_surface`` functions (distance_oracle.c).
 """
 from __future__ import annotations
 
@@ -112,6 +113,21 @@ def lib():
             [dp] * 6 + [C.c_int]
         L.orc_mirror_loss.restype = None
         L.orc_mirror_loss.argtypes = [dp, up, C.c_uint64] + [dp] * 6 + [C.c_uint32, up] + [dp] * 5 + [C.c_int]
+        L.orc_prim_params.argtypes = [P, C.c_int, C.c_uint32, dp]
+        L.orc_prim_count.restype = C.c_uint32
+        L.orc_prim_count.argtypes = [P, C.c_int]
+        for name, nargs in (("orc_sphere_distance", [dp, C.c_double, dp]),
+                            ("orc_plane_distance", [dp, C.c_double, C.c_double, dp, dp]),
+                            ("orc_cylinder_distance", [dp, C.c_double, C.c_double, C.c_double, dp, dp]),
+                            ("orc_triangle_distance", [dp, dp, dp, dp])):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = nargs
+        L.orc_closest_point_on_triangle.restype = None
+        L.orc_closest_point_on_triangle.argtypes = [dp, dp, dp, dp, dp]
+        L.orc_distance_batch.restype = None
+        L.orc_distance_batch.argtypes = [P, C.c_int, C.c_uint64, dp, dp, dp, up, dp, C.c_int]
+        L.orc_closest_batch.restype = None
+        L.orc_closest_batch.argtypes = [P, C.c_uint64, dp, dp, dp, dp, up, up, C.c_int]
         L.orc_mirror_grad.restype = None
         L.orc_mirror_grad.argtypes = [dp, up, C.c_uint64] + [dp] * 6 + [C.c_uint32, up] + [dp] * 7 + [C.c_int]
         _lib = L
@@ -265,6 +281,53 @@ class Scene:
         if trace:
             out["trace"] = tr
         return out
+
+
+    # ------------------------------------------------------------------------------------ distance_to_surface
+    def distance_to_surface(self, type_, points, prims, nthreads=0):
+        """Distance from point i to primitive prims[i] (user id) of one geometry type (distance_oracle.c)."""
+        px, py, pz = _cols(points, len(prims))
+        ids = np.ascontiguousarray(prims, dtype=np.uint32)
+        out = np.zeros(len(ids))
+        lib().orc_distance_batch(self.h, type_, len(ids), _dp(px), _dp(py), _dp(pz), _up(ids), _dp(out),
+                                 int(nthreads))
+        return out
+
+    def closest_surface(self, points, nthreads=0):
+        """Brute-force closest surface over the whole scene: distance, primitive id, geometry id."""
+        points = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+        n = len(points)
+        px, py, pz = _cols(points, n)
+        out = {"distance": np.zeros(n), "prim_id": np.zeros(n, np.uint32), "geom_id": np.zeros(n, np.uint32)}
+        lib().orc_closest_batch(self.h, n, _dp(px), _dp(py), _dp(pz), _dp(out["distance"]), _up(out["prim_id"]),
+                                _up(out["geom_id"]), int(nthreads))
+        return out
+
+
+def _v3(v):
+    return np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+
+
+def sphere_distance(c, r, p) -> float:
+    return lib().orc_sphere_distance(_dp(_v3(c)), float(r), _dp(_v3(p)))
+
+
+def plane_distance(c, dx, dy, R, p) -> float:
+    return lib().orc_plane_distance(_dp(_v3(c)), float(dx), float(dy), _dp(_v3(R)), _dp(_v3(p)))
+
+
+def cylinder_distance(c, a, b, h, R, p) -> float:
+    return lib().orc_cylinder_distance(_dp(_v3(c)), float(a), float(b), float(h), _dp(_v3(R)), _dp(_v3(p)))
+
+
+def triangle_distance(v0, v1, v2, p) -> float:
+    return lib().orc_triangle_distance(_dp(_v3(v0)), _dp(_v3(v1)), _dp(_v3(v2)), _dp(_v3(p)))
+
+
+def closest_point_on_triangle(v0, v1, v2, p):
+    q = np.zeros(3)
+    lib().orc_closest_point_on_triangle(_dp(_v3(v0)), _dp(_v3(v1)), _dp(_v3(v2)), _dp(_v3(p)), _dp(q))
+    return q
 
 
 # ---------------------------------------------------------------------------------------------- mirror tracing
