@@ -379,6 +379,34 @@ def device_commit_leg(v, f, host_scene, host_s) -> dict:
             "host_threads": len(os.sched_getaffinity(0))}
 
 
+def distance_leg(args, dev, v, f, scene, osc) -> dict:
+    """Distance queries on the 1,310,720-triangle icosphere: the closest surface of 1,048,576 points uniform in
+    [-2,2]^3 (brt_closest_surface), and distance_to_surface for 4,194,304 random (point, triangle) pairs. The first
+    256 closest-surface results are compared bit for bit with the oracle's brute-force minimum."""
+    import torch
+
+    from blazert_amd import workloads as W
+    n = 1 << 20
+    pts = W.incoherent_rays(n, 61, 2.0)[0]
+    P = torch.from_numpy(pts).to(dev)
+    ms_c, _ = time_launch(lambda: scene.closest_surface(P), max(3, args.steps // 4), 1)
+    out = scene.closest_surface(P)
+    torch.cuda.synchronize()
+    ns = 256
+    ref = osc.closest_surface(pts[:ns], nthreads=len(os.sched_getaffinity(0)))
+    exact = bool(np.array_equal(out["distance"][:ns].cpu().numpy().view(np.uint64), ref["distance"].view(np.uint64))
+                 and np.array_equal(out["prim_id"][:ns].cpu().numpy().view(np.uint32), ref["prim_id"]))
+    npair = 1 << 22
+    qp = torch.from_numpy(W.incoherent_rays(npair, 62, 2.0)[0]).to(dev)
+    ids = torch.from_numpy((W.Rng(63).uniform(npair) * len(f)).astype(np.int32)).to(dev)
+    ms_p, _ = time_launch(lambda: scene.distance_to_surface(0, qp, ids), args.steps, 2)
+    log(f"closest surface {n / ms_c / 1e3:.2f} Mpoints/s (exact on {ns}: {exact}), "
+        f"pairs {npair / ms_p / 1e3:.1f} Mpairs/s")
+    return {"closest_surface_mpoints_s": round(n / ms_c / 1e3, 3), "closest_ms": round(ms_c, 3), "points": n,
+            "closest_bit_exact_vs_bruteforce_sample": exact, "sample": ns,
+            "pairs_mpairs_s": round(npair / ms_p / 1e3, 2), "pairs": npair}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,6 +417,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the mirror-tracing adjoint leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-primitive split-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven multi-scene batch leg")
+    ap.add_argument("--no-distance", action="store_true", help="skip the distance-query leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -508,6 +537,8 @@ def main():
     out["split_precision"]["disagreements"] = classify_disagreements(osc, org, dirs, dis_idx, dis_other)
     del h64, h32, O, D, CO, CD
     out["device_commit"] = device_commit_leg(v, f, scene, build_s)
+    if not args.no_distance:
+        out["distance_queries"] = distance_leg(args, dev, v, f, scene, osc)
     if not args.no_mixed:
         out["mixed_split_precision"] = mixed_leg(args, dev)
     if not args.no_batch:
