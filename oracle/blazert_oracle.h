@@ -36,22 +36,30 @@ extern "C" {
 
 enum { ORC_TRIANGLES = 0, ORC_SPHERES = 1, ORC_PLANES = 2, ORC_CYLINDERS = 3, ORC_NUM_TYPES = 4 };
 
+/* The scene type T of blazert::Scene<T>: double, or float in the liboracle_f32 build (-DORC_FLOAT), which holds the
+ * intersect path only (no mirror tracing or distance queries). */
+#ifdef ORC_FLOAT
+typedef float orc_real;
+#else
+typedef double orc_real;
+#endif
+
 /* reference Stack<unsigned, BLAZERT_MAX_TREE_DEPTH> capacity (blazert/defines.h:5-7, blazert/bvh/accel.h:83) */
 #define ORC_REF_STACK_CAPACITY 28u
 
 typedef struct orc_ray {
-  double org[3];
-  double dir[3]; /* normalize(raw direction) */
-  double inv[3]; /* 1 / raw direction */
+  orc_real org[3];
+  orc_real dir[3]; /* normalize(raw direction) */
+  orc_real inv[3]; /* 1 / raw direction */
   uint32_t sign[3]; /* raw direction < 0 */
-  double tmin, tmax;
+  orc_real tmin, tmax;
   int any_hit;
 } orc_ray;
 
 typedef struct orc_hit {
-  double normal[3];
-  double u, v;
-  double t;
+  orc_real normal[3];
+  orc_real u, v;
+  orc_real t;
   uint32_t prim_id, geom_id;
 } orc_hit;
 
@@ -66,7 +74,7 @@ typedef struct orc_trace {
 } orc_trace;
 
 typedef struct orc_node {
-  double min[3], max[3];
+  orc_real min[3], max[3];
   uint32_t leaf, axis;
   uint32_t children[2];
   uint32_t prim_begin, prim_count; /* range into orc_bvh.prim_order */
@@ -81,20 +89,20 @@ typedef struct orc_scene orc_scene;
 /* scene construction mirrors blazert/scene.h:227-346; returns geom id or 0xFFFFFFFF */
 orc_scene *orc_scene_new(uint32_t min_leaf_primitives, uint32_t max_tree_depth, uint32_t bin_size);
 void orc_scene_free(orc_scene *s);
-uint32_t orc_add_mesh(orc_scene *s, const double *vertices, uint32_t nv, const uint32_t *faces, uint32_t nf);
-uint32_t orc_add_spheres(orc_scene *s, const double *centers, const double *radii, uint32_t n);
-uint32_t orc_add_planes(orc_scene *s, const double *centers, const double *dxs, const double *dys,
-                        const double *rot9, uint32_t n);
-uint32_t orc_add_cylinders(orc_scene *s, const double *centers, const double *a, const double *b,
-                           const double *heights, const double *rot9, uint32_t n);
+uint32_t orc_add_mesh(orc_scene *s, const orc_real *vertices, uint32_t nv, const uint32_t *faces, uint32_t nf);
+uint32_t orc_add_spheres(orc_scene *s, const orc_real *centers, const orc_real *radii, uint32_t n);
+uint32_t orc_add_planes(orc_scene *s, const orc_real *centers, const orc_real *dxs, const orc_real *dys,
+                        const orc_real *rot9, uint32_t n);
+uint32_t orc_add_cylinders(orc_scene *s, const orc_real *centers, const orc_real *a, const orc_real *b,
+                           const orc_real *heights, const orc_real *rot9, uint32_t n);
 int orc_commit(orc_scene *s);
 
 /* collection precompute accessors (boxes / centers) for known-answer tests */
-int orc_prim_box(const orc_scene *s, int type, uint32_t prim, double bmin[3], double bmax[3]);
-int orc_prim_center(const orc_scene *s, int type, uint32_t prim, double c[3]);
+int orc_prim_box(const orc_scene *s, int type, uint32_t prim, orc_real bmin[3], orc_real bmax[3]);
+int orc_prim_center(const orc_scene *s, int type, uint32_t prim, orc_real c[3]);
 
 /* primitive parameters in user order (triangle v0 v1 v2; sphere c r; plane c dx dy R; cylinder c a b h R) */
-int orc_prim_params(const orc_scene *s, int type, uint32_t prim, double *out);
+int orc_prim_params(const orc_scene *s, int type, uint32_t prim, orc_real *out);
 uint32_t orc_prim_count(const orc_scene *s, int type);
 uint32_t orc_geom_id(const orc_scene *s, int type);
 
@@ -102,8 +110,8 @@ uint32_t orc_geom_id(const orc_scene *s, int type);
 uint32_t orc_tree_size(const orc_scene *s, int type);
 int orc_tree_export(const orc_scene *s, int type, orc_node *nodes, uint32_t *prim_order, orc_stats *stats);
 
-void orc_ray_init(orc_ray *r, const double o[3], const double d[3], double tmin, double tmax, int any_hit);
-int orc_intersect_node(double *min_distance, double *max_distance, const double bmin[3], const double bmax[3],
+void orc_ray_init(orc_ray *r, const orc_real o[3], const orc_real d[3], orc_real tmin, orc_real tmax, int any_hit);
+int orc_intersect_node(orc_real *min_distance, orc_real *max_distance, const orc_real bmin[3], const orc_real bmax[3],
                        const orc_ray *ray);
 
 /* single primitive call sequence prepare -> intersect(prim) -> post (the reference tests' helper) */
@@ -115,10 +123,10 @@ int orc_intersect1(const orc_scene *s, const orc_ray *ray, orc_hit *hit, orc_tra
 
 /* batched intersect1 over SoA rays, OpenMP over rays. hits start as a default RayHit.
  * trace arrays may be NULL. Returns number of hits. */
-uint64_t orc_intersect_batch(const orc_scene *s, uint64_t n, const double *ox, const double *oy, const double *oz,
-                             const double *dx, const double *dy, const double *dz, const double *tmin,
-                             const double *tmax, double *t, double *u, double *v, uint32_t *prim_id,
-                             uint32_t *geom_id, double *nx, double *ny, double *nz, orc_trace *trace, int nthreads,
+uint64_t orc_intersect_batch(const orc_scene *s, uint64_t n, const orc_real *ox, const orc_real *oy, const orc_real *oz,
+                             const orc_real *dx, const orc_real *dy, const orc_real *dz, const orc_real *tmin,
+                             const orc_real *tmax, orc_real *t, orc_real *u, orc_real *v, uint32_t *prim_id,
+                             uint32_t *geom_id, orc_real *nx, orc_real *ny, orc_real *nz, orc_trace *trace, int nthreads,
                              int any_hit);
 
 /* brute-force closest candidate per type (no BVH): every primitive tested from a fresh intersector */

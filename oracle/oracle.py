@@ -392,3 +392,167 @@ def mirror_grad(vertices, faces, org, dirs, prim_id, gt=None, gp=None, nthreads=
     lib().orc_mirror_grad(_dp(V), _up(F), n, *[_dp(c) for c in cols], B, _up(prim), _dp(g_t),
                           *[_dp(c) for c in gps], _dp(gv), _dp(go), _dp(gd), int(nthreads))
     return gv.reshape(-1, 3), go.reshape(3, n).T.copy(), gd.reshape(3, n).T.copy()
+
+
+# ---------------------------------------------------------------------------------------------- Scene<float>
+# The same restatement compiled with -DORC_FLOAT (liboracle_f32.so): blazert::Scene<float>, every operation in FP32.
+class RayF(C.Structure):
+    _fields_ = [("org", C.c_float * 3), ("dir", C.c_float * 3), ("inv", C.c_float * 3),
+                ("sign", C.c_uint32 * 3), ("tmin", C.c_float), ("tmax", C.c_float), ("any_hit", C.c_int)]
+
+
+class HitF(C.Structure):
+    _fields_ = [("normal", C.c_float * 3), ("u", C.c_float), ("v", C.c_float), ("t", C.c_float),
+                ("prim_id", C.c_uint32), ("geom_id", C.c_uint32)]
+
+    @staticmethod
+    def default() -> "HitF":
+        h = HitF()
+        h.t = np.finfo(np.float32).max
+        h.prim_id = INVALID
+        h.geom_id = INVALID
+        return h
+
+
+NODE_DTYPE_F32 = np.dtype([("min", np.float32, 3), ("max", np.float32, 3), ("leaf", np.uint32), ("axis", np.uint32),
+                           ("children", np.uint32, 2), ("prim_begin", np.uint32), ("prim_count", np.uint32)])
+LIB_F32_PATH = os.path.join(HERE, "liboracle_f32.so")
+_lib_f32 = None
+
+
+def lib_f32():
+    global _lib_f32
+    if _lib_f32 is None:
+        if not os.path.exists(LIB_F32_PATH):
+            build(force=True)
+        L = C.CDLL(LIB_F32_PATH)
+        P = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        up = C.POINTER(C.c_uint32)
+        L.orc_scene_new.restype = P
+        L.orc_scene_new.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_scene_free.argtypes = [P]
+        L.orc_add_mesh.restype = C.c_uint32
+        L.orc_add_mesh.argtypes = [P, fp, C.c_uint32, up, C.c_uint32]
+        L.orc_add_spheres.restype = C.c_uint32
+        L.orc_add_spheres.argtypes = [P, fp, fp, C.c_uint32]
+        L.orc_add_planes.restype = C.c_uint32
+        L.orc_add_planes.argtypes = [P, fp, fp, fp, fp, C.c_uint32]
+        L.orc_add_cylinders.restype = C.c_uint32
+        L.orc_add_cylinders.argtypes = [P, fp, fp, fp, fp, fp, C.c_uint32]
+        L.orc_commit.argtypes = [P]
+        L.orc_tree_size.restype = C.c_uint32
+        L.orc_tree_size.argtypes = [P, C.c_int]
+        L.orc_tree_export.argtypes = [P, C.c_int, C.c_void_p, up, C.POINTER(Stats)]
+        L.orc_ray_init.argtypes = [C.POINTER(RayF), fp, fp, C.c_float, C.c_float, C.c_int]
+        L.orc_intersect_single.argtypes = [P, C.c_int, C.c_uint32, C.POINTER(RayF), C.POINTER(HitF)]
+        L.orc_intersect1.argtypes = [P, C.POINTER(RayF), C.POINTER(HitF), C.POINTER(Trace)]
+        L.orc_intersect_batch.restype = C.c_uint64
+        L.orc_intersect_batch.argtypes = [P, C.c_uint64] + [fp] * 8 + [fp, fp, fp, up, up, fp, fp, fp, C.c_void_p,
+                                                                        C.c_int, C.c_int]
+        _lib_f32 = L
+    return _lib_f32
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def make_ray_f32(org, direction, tmin=0.0, tmax=None, any_hit=False) -> RayF:
+    r = RayF()
+    if tmax is None:
+        tmax = np.finfo(np.float32).max
+    o = (C.c_float * 3)(*[float(np.float32(x)) for x in org])
+    d = (C.c_float * 3)(*[float(np.float32(x)) for x in direction])
+    lib_f32().orc_ray_init(C.byref(r), o, d, float(np.float32(tmin)), float(np.float32(tmax)), int(any_hit))
+    return r
+
+
+class SceneF32:
+    """Oracle of blazert::Scene<float>: the add/commit/intersect1 path with T = float."""
+
+    def __init__(self, min_leaf_primitives=4, max_tree_depth=28, bin_size=64):
+        self._keep = []
+        self.h = lib_f32().orc_scene_new(min_leaf_primitives, max_tree_depth, bin_size)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        self.h = None
+        if h:
+            try:
+                lib_f32().orc_scene_free(h)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
+
+    @staticmethod
+    def _f(a, cols=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        return a.reshape(-1, cols) if cols else a.reshape(-1)
+
+    def add_mesh(self, vertices, faces) -> int:
+        v = self._f(vertices, 3)
+        f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+        self._keep.extend([v, f])
+        return lib_f32().orc_add_mesh(self.h, _fp(v), len(v), _up(f), len(f))
+
+    def add_spheres(self, centers, radii) -> int:
+        c, r = self._f(centers, 3), self._f(radii)
+        return lib_f32().orc_add_spheres(self.h, _fp(c), _fp(r), len(r))
+
+    def add_planes(self, centers, dxs, dys, rotations) -> int:
+        c, dx, dy, R = self._f(centers, 3), self._f(dxs), self._f(dys), self._f(rotations, 9)
+        return lib_f32().orc_add_planes(self.h, _fp(c), _fp(dx), _fp(dy), _fp(R), len(dx))
+
+    def add_cylinders(self, centers, semi_axes_a, semi_axes_b, heights, rotations) -> int:
+        c, a, b = self._f(centers, 3), self._f(semi_axes_a), self._f(semi_axes_b)
+        h, R = self._f(heights), self._f(rotations, 9)
+        return lib_f32().orc_add_cylinders(self.h, _fp(c), _fp(a), _fp(b), _fp(h), _fp(R), len(a))
+
+    def commit(self) -> bool:
+        return bool(lib_f32().orc_commit(self.h))
+
+    def tree(self, type_):
+        n = lib_f32().orc_tree_size(self.h, type_)
+        nodes = np.zeros(n, dtype=NODE_DTYPE_F32)
+        st = Stats()
+        lib_f32().orc_tree_export(self.h, type_, nodes.ctypes.data_as(C.c_void_p), None, C.byref(st))
+        nprim = int(nodes["prim_count"][nodes["leaf"] == 1].sum())
+        order = np.zeros(max(nprim, 1), dtype=np.uint32)
+        lib_f32().orc_tree_export(self.h, type_, nodes.ctypes.data_as(C.c_void_p), _up(order), C.byref(st))
+        return nodes, order[:nprim], {name: getattr(st, name) for name, _ in Stats._fields_}
+
+    def intersect_single(self, type_, prim, ray: RayF):
+        h = HitF.default()
+        hit = lib_f32().orc_intersect_single(self.h, type_, prim, C.byref(ray), C.byref(h))
+        return bool(hit), h
+
+    def intersect1(self, ray: RayF, hit: HitF | None = None):
+        h = hit if hit is not None else HitF.default()
+        tr = Trace()
+        res = lib_f32().orc_intersect1(self.h, C.byref(ray), C.byref(h), C.byref(tr))
+        return bool(res), h, tr
+
+    def intersect_batch(self, org, dirs, tmin=None, tmax=None, normals=True, trace=False, nthreads=0, any_hit=False):
+        """Batched intersect1 with T = float: float32 in, float32 out."""
+        org = self._f(org, 3)
+        dirs = self._f(dirs, 3)
+        n = len(org)
+        cols = [np.ascontiguousarray(org[:, k]) for k in range(3)] + [np.ascontiguousarray(dirs[:, k])
+                                                                       for k in range(3)]
+        tmn = None if tmin is None else np.ascontiguousarray(np.broadcast_to(tmin, (n,)), dtype=np.float32)
+        tmx = None if tmax is None else np.ascontiguousarray(np.broadcast_to(tmax, (n,)), dtype=np.float32)
+        out = {k: np.zeros(n, np.float32) for k in ("t", "u", "v")}
+        out["prim_id"] = np.zeros(n, dtype=np.uint32)
+        out["geom_id"] = np.zeros(n, dtype=np.uint32)
+        nrm = [np.zeros(n, np.float32) for _ in range(3)] if normals else [None, None, None]
+        tr = np.zeros(n, dtype=TRACE_DTYPE) if trace else None
+        lib_f32().orc_intersect_batch(self.h, n, *[_fp(c) for c in cols], _fp(tmn), _fp(tmx), _fp(out["t"]),
+                                      _fp(out["u"]), _fp(out["v"]), _up(out["prim_id"]), _up(out["geom_id"]),
+                                      *[_fp(c) for c in nrm],
+                                      tr.ctypes.data_as(C.c_void_p) if tr is not None else None, int(nthreads),
+                                      int(any_hit))
+        if normals:
+            out["normal"] = np.stack(nrm, axis=1)
+        if trace:
+            out["trace"] = tr
+        return out
