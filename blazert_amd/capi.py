@@ -81,6 +81,11 @@ class Hit1(C.Structure):
                 ("prim_id", C.c_uint32), ("geom_id", C.c_uint32)]
 
 
+class Hit1F(C.Structure):  # brt_hit1_f32: RayHit<float>
+    _fields_ = [("normal", C.c_float * 3), ("u", C.c_float), ("v", C.c_float), ("t", C.c_float),
+                ("prim_id", C.c_uint32), ("geom_id", C.c_uint32)]
+
+
 class MirrorOut(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("t", "u", "v", "prim_id", "px", "py", "pz", "rx", "ry", "rz")]
 
@@ -99,6 +104,7 @@ class TuneResult(C.Structure):
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _U = C.POINTER(C.c_uint32)
+_F = C.POINTER(C.c_float)
 SIGNATURES = {
     "brt_last_error": (C.c_char_p, []),
     "brt_version": (C.c_char_p, []),
@@ -106,6 +112,15 @@ SIGNATURES = {
     "brt_scene_create": (C.c_int, [C.POINTER(BuildOptions), C.POINTER(C.c_void_p)]),
     "brt_scene_destroy": (C.c_int, [_P]),
     "brt_scene_add_mesh": (C.c_int, [_P, _D, C.c_uint32, _U, C.c_uint32, _U]),
+    "brt_scene_create_f32": (C.c_int, [C.POINTER(BuildOptions), C.POINTER(C.c_void_p)]),
+    "brt_scene_add_mesh_f32": (C.c_int, [_P, _F, C.c_uint32, _U, C.c_uint32, _U]),
+    "brt_scene_add_spheres_f32": (C.c_int, [_P, _F, _F, C.c_uint32, _U]),
+    "brt_scene_add_planes_f32": (C.c_int, [_P, _F, _F, _F, _F, C.c_uint32, _U]),
+    "brt_scene_add_cylinders_f32": (C.c_int, [_P, _F, _F, _F, _F, _F, C.c_uint32, _U]),
+    "brt_intersect_f32": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_uint64, C.POINTER(Launch),
+                                    C.c_void_p]),
+    "brt_intersect1_f32": (C.c_int, [_P, _F, _F, C.c_float, C.c_float, C.c_uint32, C.POINTER(Hit1F),
+                                     C.POINTER(C.c_int)]),
     "brt_scene_add_spheres": (C.c_int, [_P, _D, _D, C.c_uint32, _U]),
     "brt_scene_add_planes": (C.c_int, [_P, _D, _D, _D, _D, C.c_uint32, _U]),
     "brt_scene_add_cylinders": (C.c_int, [_P, _D, _D, _D, _D, _D, C.c_uint32, _U]),

@@ -89,6 +89,11 @@ hipError_t launch_mirror_adjoint(const RaysView &rays, const uint32_t *prims, ui
 // Writes the default RayHit (t = DBL_MAX, ids invalid, u = v = 0, normal 0) for n rays.
 hipError_t launch_init_hits(const HitsView &hits, uint64_t n, hipStream_t stream);
 
+// Scene<float>: traversal of one geometry type in FP32 (bvh.nodes32 and FP32 records), and the FP32 default RayHit.
+hipError_t launch_trace_f32(GeomType type, const BvhView &bvh, const RaysViewF &rays, const HitsViewF &hits,
+                            uint64_t n, const TraceLaunch &cfg, hipStream_t stream);
+hipError_t launch_init_hits_f32(const HitsViewF &hits, uint64_t n, hipStream_t stream);
+
 // Distance queries (distance.hip). inv maps primitive ids to leaf positions (spheres, planes, cylinders; built by
 // launch_inverse_order from the view's prim_ids); ids >= n_prims give NaN.
 hipError_t launch_inverse_order(const uint32_t *order, uint32_t n, uint32_t *inv, hipStream_t stream);

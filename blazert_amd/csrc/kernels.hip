@@ -36,8 +36,24 @@ constexpr uint32_t FLAG_ANY_HIT = 2u;
 template <bool SPLIT>
 constexpr int stack_cap() { return SPLIT ? 128 : 64; }
 
-__device__ __forceinline__ double std_min(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double std_max(double a, double b) { return (a < b) ? b : a; }
+// Constants of the scene type T (blazert::Scene<T>): double for the drop-in path, float for Scene<float>.
+template <class R>
+struct Real;
+template <>
+struct Real<double> {
+  static constexpr double eps = DBL_EPSILON;
+  static constexpr double max = DBL_MAX;
+};
+template <>
+struct Real<float> {
+  static constexpr float eps = FLT_EPSILON;
+  static constexpr float max = FLT_MAX;
+};
+
+template <class R>
+__device__ __forceinline__ R std_min(R a, R b) { return (b < a) ? b : a; }
+template <class R>
+__device__ __forceinline__ R std_max(R a, R b) { return (a < b) ? b : a; }
 
 // blazert/datatypes.h:83-85
 __device__ __forceinline__ double product_sign(double x, double y) {
@@ -45,76 +61,88 @@ __device__ __forceinline__ double product_sign(double x, double y) {
   const unsigned long long uy = (unsigned long long)__double_as_longlong(y);
   return __longlong_as_double((long long)(ux ^ (uy & 0x8000000000000000ull)));
 }
+__device__ __forceinline__ float product_sign(float x, float y) {
+  return __uint_as_float(__float_as_uint(x) ^ (__float_as_uint(y) & 0x80000000u));
+}
 
+// correctly rounded in both precisions
 __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
 
-struct Ray {
-  double o[3];
-  double d[3];    // normalize(raw direction)
-  double inv[3];  // 1 / raw direction
+template <class R>
+struct RayT {
+  R o[3];
+  R d[3];    // normalize(raw direction)
+  R inv[3];  // 1 / raw direction
   uint32_t sign;  // bit i: raw direction[i] < 0; bit 3: no slab term can be NaN (see make_ray)
-  double tmin, tmax;
+  R tmin, tmax;
 };
+using Ray = RayT<double>;
 constexpr uint32_t RAY_NAN_FREE = 8u;
 
 // blazert/ray.h:56-65 with normalize(v) = v * (len != 0 ? 1/len : 0)
-__device__ __forceinline__ void make_ray(const RaysView &R, uint64_t i, Ray &r) {
-  const double raw[3] = {R.dx[i], R.dy[i], R.dz[i]};
-  r.o[0] = R.ox[i];
-  r.o[1] = R.oy[i];
-  r.o[2] = R.oz[i];
-  const double len = dsqrt(raw[0] * raw[0] + raw[1] * raw[1] + raw[2] * raw[2]);
-  const double il = (len != 0.0) ? 1.0 / len : 0.0;
+template <class R>
+__device__ __forceinline__ void make_ray(const RaysViewT<R> &V, uint64_t i, RayT<R> &r) {
+  const R raw[3] = {V.dx[i], V.dy[i], V.dz[i]};
+  r.o[0] = V.ox[i];
+  r.o[1] = V.oy[i];
+  r.o[2] = V.oz[i];
+  const R len = dsqrt(raw[0] * raw[0] + raw[1] * raw[1] + raw[2] * raw[2]);
+  const R il = (len != R(0)) ? R(1) / len : R(0);
   r.sign = 0;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     r.d[k] = raw[k] * il;
-    r.inv[k] = 1.0 / raw[k];
-    r.sign |= (raw[k] < 0.0 ? 1u : 0u) << k;
+    r.inv[k] = R(1) / raw[k];
+    r.sign |= (raw[k] < R(0) ? 1u : 0u) << k;
   }
-  r.tmin = R.tmin ? R.tmin[i] : R.tmin0;
-  r.tmax = R.tmax ? R.tmax[i] : R.tmax0;
+  r.tmin = V.tmin ? V.tmin[i] : V.tmin0;
+  r.tmax = V.tmax ? V.tmax[i] : V.tmax0;
   // With a finite origin, finite non-zero direction components (so 1/d is finite and non-zero), a non-NaN tmin and
   // finite node boxes (checked on the host), (b - o) * inv is never NaN: the slab chains reduce to plain max/min.
   bool ok = r.tmin == r.tmin;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    ok = ok && fabs(r.o[k]) <= DBL_MAX && raw[k] != 0.0 && fabs(raw[k]) <= DBL_MAX && fabs(r.inv[k]) <= DBL_MAX;
+    ok = ok && fabs(r.o[k]) <= Real<R>::max && raw[k] != R(0) && fabs(raw[k]) <= Real<R>::max &&
+         fabs(r.inv[k]) <= Real<R>::max;
   }
   if (ok) r.sign |= RAY_NAN_FREE;
 }
 
 // ------------------------------------------------------------------------------------------------ slab tests
-// Exact FP64 slab test of one child box, split into its hit-distance independent part. Returns false when the
-// box fails for every hit distance; otherwise key is the value the current hit distance h must not be below
-// (pass iff !(h < key)); key = -inf when a NaN far-plane term removes h from the reference's min chain.
-__device__ __forceinline__ bool slab64(const double *b, const Ray &r, double &key) {
+// Exact slab test of one child box (blazert/bvh/intersect.h:32-51 in the scene type), split into its hit-distance
+// independent part. Returns false when the box fails for every hit distance; otherwise key is the value the current
+// hit distance h must not be below (pass iff !(h < key)); key = -inf when a NaN far-plane term removes h from the
+// reference's min chain.
+template <class R>
+__device__ __forceinline__ bool slab_exact(const R *b, const RayT<R> &r, R &key) {
+  const R L1 = R(1) + R(4) * Real<R>::eps;  // far-plane widening of blazert/bvh/intersect.h:40
   if (r.sign & RAY_NAN_FREE) {
-    double A = r.tmin, M = __builtin_inf();
+    R A = r.tmin, M = std::numeric_limits<R>::infinity();
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       const bool s = (r.sign >> i) & 1u;
-      const double nearp = s ? b[3 + i] : b[i];
-      const double farp = s ? b[i] : b[3 + i];
+      const R nearp = s ? b[3 + i] : b[i];
+      const R farp = s ? b[i] : b[3 + i];
       A = fmax(A, (nearp - r.o[i]) * r.inv[i]);
       M = fmin(M, ((farp - r.o[i]) * r.inv[i]) * L1);
     }
     key = A;
     return A <= M;
   }
-  double acc = r.tmin;
-  double m[3];
+  R acc = r.tmin;
+  R m[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const bool s = (r.sign >> i) & 1u;
-    const double nearp = s ? b[3 + i] : b[i];
-    const double farp = s ? b[i] : b[3 + i];
-    const double mn = (nearp - r.o[i]) * r.inv[i];
+    const R nearp = s ? b[3 + i] : b[i];
+    const R farp = s ? b[i] : b[3 + i];
+    const R mn = (nearp - r.o[i]) * r.inv[i];
     m[i] = ((farp - r.o[i]) * r.inv[i]) * L1;
     acc = std_max(mn, acc);
   }
   const bool n0 = m[0] != m[0], n1 = m[1] != m[1], n2 = m[2] != m[2];
-  double M;
+  R M;
   bool hdep = false;
   if (n2) {
     return false;
@@ -127,7 +155,7 @@ __device__ __forceinline__ bool slab64(const double *b, const Ray &r, double &ke
     hdep = true;
   }
   if (!(acc <= M)) return false;
-  key = hdep ? acc : -__builtin_inf();
+  key = hdep ? acc : -std::numeric_limits<R>::infinity();
   return true;
 }
 
@@ -180,32 +208,40 @@ __device__ __forceinline__ bool slab32(const float *b, const Ray32 &q, float &ke
 
 // ------------------------------------------------------------------------------------------------ intersectors
 // Closest-hit state of one traversal (prepare_traversal: hit = tmax, uv = 0, prim invalid).
-struct Hit {
-  double h;
-  double u, v;
+template <class R>
+struct HitT {
+  R h;
+  R u, v;
   uint32_t leaf_idx;  // leaf-order index of the current hit
 };
+using Hit = HitT<double>;
+
+// Every intersector runs in the scene type of its record (Rec::real): the same operation sequence for double and
+// float, with the reference's constants of that type.
 
 // blazert/primitives/trimesh.h:153-181
-__device__ __forceinline__ bool isect(const TriRec &T, const Ray &r, Hit &s, uint32_t idx) {
-  const double tol = 4.0 * DBL_EPSILON;
-  const double c0 = T.v0[0] - r.o[0], c1 = T.v0[1] - r.o[1], c2 = T.v0[2] - r.o[2];
-  const double rx = r.d[1] * c2 - r.d[2] * c1;
-  const double ry = r.d[2] * c0 - r.d[0] * c2;
-  const double rz = r.d[0] * c1 - r.d[1] * c0;
-  const double *e1 = T.e1, *e2 = T.e2;
-  const double nx = e1[1] * e2[2] - e1[2] * e2[1];
-  const double ny = e1[2] * e2[0] - e1[0] * e2[2];
-  const double nz = e1[0] * e2[1] - e1[1] * e2[0];
-  const double det = nx * r.d[0] + ny * r.d[1] + nz * r.d[2];
-  const double abs_det = fabs(det);
-  const double u = product_sign(rx * e2[0] + ry * e2[1] + rz * e2[2], det);
-  const double v = product_sign(rx * e1[0] + ry * e1[1] + rz * e1[2], det);
-  const double w = abs_det - u - v;
+template <class Rec>
+__device__ __forceinline__ bool isect_tri(const Rec &T, const RayT<typename Rec::real> &r,
+                                          HitT<typename Rec::real> &s, uint32_t idx) {
+  using R = typename Rec::real;
+  const R tol = R(4) * Real<R>::eps;
+  const R c0 = T.v0[0] - r.o[0], c1 = T.v0[1] - r.o[1], c2 = T.v0[2] - r.o[2];
+  const R rx = r.d[1] * c2 - r.d[2] * c1;
+  const R ry = r.d[2] * c0 - r.d[0] * c2;
+  const R rz = r.d[0] * c1 - r.d[1] * c0;
+  const R *e1 = T.e1, *e2 = T.e2;
+  const R nx = e1[1] * e2[2] - e1[2] * e2[1];
+  const R ny = e1[2] * e2[0] - e1[0] * e2[2];
+  const R nz = e1[0] * e2[1] - e1[1] * e2[0];
+  const R det = nx * r.d[0] + ny * r.d[1] + nz * r.d[2];
+  const R abs_det = fabs(det);
+  const R u = product_sign(rx * e2[0] + ry * e2[1] + rz * e2[2], det);
+  const R v = product_sign(rx * e1[0] + ry * e1[1] + rz * e1[2], det);
+  const R w = abs_det - u - v;
   if (u >= -tol && v >= -tol && w >= -tol) {
-    const double t = product_sign(nx * c0 + ny * c1 + nz * c2, det);
+    const R t = product_sign(nx * c0 + ny * c1 + nz * c2, det);
     if (t >= abs_det * r.tmin && abs_det * s.h > t) {
-      const double inv_det = 1.0 / abs_det;
+      const R inv_det = R(1) / abs_det;
       s.h = t * inv_det;
       s.u = u * inv_det;
       s.v = v * inv_det;
@@ -216,23 +252,27 @@ __device__ __forceinline__ bool isect(const TriRec &T, const Ray &r, Hit &s, uin
   return false;
 }
 
-__device__ __forceinline__ double norm_scaled(double t, const double *d) {
-  const double x = t * d[0], y = t * d[1], z = t * d[2];
+template <class R>
+__device__ __forceinline__ R norm_scaled(R t, const R *d) {
+  const R x = t * d[0], y = t * d[1], z = t * d[2];
   return dsqrt(x * x + y * y + z * z);
 }
 
 // blazert/primitives/sphere.h:364-406
-__device__ __forceinline__ bool isect(const SphRec &S, const Ray &r, Hit &s, uint32_t idx) {
-  const double v0 = r.o[0] - S.c[0], v1 = r.o[1] - S.c[1], v2 = r.o[2] - S.c[2];
-  const double A = r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2];
-  const double B = 2.0 * (v0 * r.d[0] + v1 * r.d[1] + v2 * r.d[2]);
-  const double C = (v0 * v0 + v1 * v1 + v2 * v2) - S.r * S.r;
-  const double D = B * B - 4.0 * A * C;
-  if (D < 0.0) return false;
-  const double Q = dsqrt(D);
-  const double rcpA = 1.0 / A;
-  const double t0 = 0.5 * rcpA * (-B - Q);
-  const double t1 = 0.5 * rcpA * (-B + Q);
+template <class Rec>
+__device__ __forceinline__ bool isect_sph(const Rec &S, const RayT<typename Rec::real> &r,
+                                          HitT<typename Rec::real> &s, uint32_t idx) {
+  using R = typename Rec::real;
+  const R v0 = r.o[0] - S.c[0], v1 = r.o[1] - S.c[1], v2 = r.o[2] - S.c[2];
+  const R A = r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2];
+  const R B = R(2) * (v0 * r.d[0] + v1 * r.d[1] + v2 * r.d[2]);
+  const R C = (v0 * v0 + v1 * v1 + v2 * v2) - S.r * S.r;
+  const R D = B * B - R(4) * A * C;
+  if (D < R(0)) return false;
+  const R Q = dsqrt(D);
+  const R rcpA = R(1) / A;
+  const R t0 = R(0.5) * rcpA * (-B - Q);
+  const R t1 = R(0.5) * rcpA * (-B + Q);
   if ((t0 > r.tmin) && (t0 < s.h)) {
     s.h = norm_scaled(t0, r.d);
     s.leaf_idx = idx;
@@ -247,28 +287,34 @@ __device__ __forceinline__ bool isect(const SphRec &S, const Ray &r, Hit &s, uin
 }
 
 // trans(R) * v and R * v for row-major R
-__device__ __forceinline__ void matTvec(const double *R, const double *v, double *o) {
-  o[0] = R[0] * v[0] + R[3] * v[1] + R[6] * v[2];
-  o[1] = R[1] * v[0] + R[4] * v[1] + R[7] * v[2];
-  o[2] = R[2] * v[0] + R[5] * v[1] + R[8] * v[2];
+template <class R>
+__device__ __forceinline__ void matTvec(const R *M, const R *v, R *o) {
+  o[0] = M[0] * v[0] + M[3] * v[1] + M[6] * v[2];
+  o[1] = M[1] * v[0] + M[4] * v[1] + M[7] * v[2];
+  o[2] = M[2] * v[0] + M[5] * v[1] + M[8] * v[2];
 }
-__device__ __forceinline__ void matvec(const double *R, const double *v, double *o) {
-  o[0] = R[0] * v[0] + R[1] * v[1] + R[2] * v[2];
-  o[1] = R[3] * v[0] + R[4] * v[1] + R[5] * v[2];
-  o[2] = R[6] * v[0] + R[7] * v[1] + R[8] * v[2];
+template <class R>
+__device__ __forceinline__ void matvec(const R *M, const R *v, R *o) {
+  o[0] = M[0] * v[0] + M[1] * v[1] + M[2] * v[2];
+  o[1] = M[3] * v[0] + M[4] * v[1] + M[5] * v[2];
+  o[2] = M[6] * v[0] + M[7] * v[1] + M[8] * v[2];
 }
 
 // Plane: local frame, t = -o_z / d_z, interior / edge / corner branches (blazert/primitives/plane.h:180-270).
 // Returns the branch (0 = miss, 1 interior, 2..5 edges, 6..9 corners) for the given [tnear, tfar].
-__device__ __forceinline__ int plane_branch(const PlaRec &P, const Ray &r, double tnear, double tfar, double *org,
-                                            double *dir, double &t1, bool check_t) {
-  const double oc[3] = {r.o[0] - P.c[0], r.o[1] - P.c[1], r.o[2] - P.c[2]};
+template <class Rec>
+__device__ __forceinline__ int plane_branch(const Rec &P, const RayT<typename Rec::real> &r,
+                                            typename Rec::real tnear, typename Rec::real tfar,
+                                            typename Rec::real *org, typename Rec::real *dir,
+                                            typename Rec::real &t1, bool check_t) {
+  using R = typename Rec::real;
+  const R oc[3] = {r.o[0] - P.c[0], r.o[1] - P.c[1], r.o[2] - P.c[2]};
   matTvec(P.R, oc, org);
   matTvec(P.R, r.d, dir);
   t1 = -org[2] / dir[2];
-  const double X = org[0] + t1 * dir[0];
-  const double Y = org[1] + t1 * dir[1];
-  const double x_min = -P.dx / 2, x_max = P.dx / 2, y_min = -P.dy / 2, y_max = P.dy / 2;
+  const R X = org[0] + t1 * dir[0];
+  const R Y = org[1] + t1 * dir[1];
+  const R x_min = -P.dx / 2, x_max = P.dx / 2, y_min = -P.dy / 2, y_max = P.dy / 2;
   const bool tok = !check_t || ((t1 > tnear) && (t1 < tfar));
   if (!tok) return 0;
   if ((X > x_min) && (X < x_max) && (Y > y_min) && (Y < y_max)) return 1;
@@ -283,8 +329,11 @@ __device__ __forceinline__ int plane_branch(const PlaRec &P, const Ray &r, doubl
   return 0;
 }
 
-__device__ __forceinline__ bool isect(const PlaRec &P, const Ray &r, Hit &s, uint32_t idx) {
-  double org[3], dir[3], t1;
+template <class Rec>
+__device__ __forceinline__ bool isect_pla(const Rec &P, const RayT<typename Rec::real> &r,
+                                          HitT<typename Rec::real> &s, uint32_t idx) {
+  using R = typename Rec::real;
+  R org[3], dir[3], t1;
   const int br = plane_branch(P, r, r.tmin, s.h, org, dir, t1, true);
   if (!br) return false;
   s.h = norm_scaled(t1, dir);
@@ -293,67 +342,74 @@ __device__ __forceinline__ bool isect(const PlaRec &P, const Ray &r, Hit &s, uin
 }
 
 // Plane normal of the accepted branch (before the final normalize of post_traversal).
-__device__ void plane_normal(const PlaRec &P, const Ray &r, double *nrm) {
-  double org[3], dir[3], t1;
-  const int br = plane_branch(P, r, 0.0, 0.0, org, dir, t1, false);
-  const double sgn = org[2] / fabs(org[2]);
+template <class Rec>
+__device__ void plane_normal(const Rec &P, const RayT<typename Rec::real> &r, typename Rec::real *nrm) {
+  using R = typename Rec::real;
+  R org[3], dir[3], t1;
+  const int br = plane_branch(P, r, R(0), R(0), org, dir, t1, false);
+  const R sgn = org[2] / fabs(org[2]);
   if (br == 1) {
-    const double ez[3] = {0.0, 0.0, 1.0};
-    double rz[3];
+    const R ez[3] = {R(0), R(0), R(1)};
+    R rz[3];
     matvec(P.R, ez, rz);
     nrm[0] = sgn * rz[0];
     nrm[1] = sgn * rz[1];
     nrm[2] = sgn * rz[2];
     return;
   }
-  const double lx[10] = {0, 0, -1.0, 1.0, 0.0, 0.0, -1.0, -1.0, 1.0, 1.0};
-  const double ly[10] = {0, 0, 0.0, 0.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0};
-  const double l[3] = {lx[br], ly[br], sgn};
+  const R lx[10] = {0, 0, -1, 1, 0, 0, -1, -1, 1, 1};
+  const R ly[10] = {0, 0, 0, 0, -1, 1, -1, 1, -1, 1};
+  const R l[3] = {lx[br], ly[br], sgn};
   matvec(P.R, l, nrm);
 }
 
 // Cylinder (blazert/primitives/cylinder.h:181-495). No hit-distance checks: the last accepted cylinder wins.
 // Returns true on a hit with the parametric t and the (pre-normalisation) normal.
+template <class R>
 struct CylOut {
-  double t;
-  double n[3];
+  R t;
+  R n[3];
 };
 
-__device__ __forceinline__ void cyl_ng(const double *R, const double *nn, double *out) {
-  double rn[3];
-  matvec(R, nn, rn);
-  const double len = dsqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
-  const double is = 1.0 / len;
+template <class R>
+__device__ __forceinline__ void cyl_ng(const R *M, const R *nn, R *out) {
+  R rn[3];
+  matvec(M, nn, rn);
+  const R len = dsqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+  const R is = R(1) / len;
   out[0] = rn[0] * is;
   out[1] = rn[1] * is;
   out[2] = rn[2] * is;
 }
 
-__device__ bool cyl_eval(const CylRec &Cy, const Ray &r, double *dir, CylOut &out, bool want_normal) {
-  const double oc[3] = {r.o[0] - Cy.c[0], r.o[1] - Cy.c[1], r.o[2] - Cy.c[2]};
-  double org[3];
+template <class Rec>
+__device__ bool cyl_eval(const Rec &Cy, const RayT<typename Rec::real> &r, typename Rec::real *dir,
+                         CylOut<typename Rec::real> &out, bool want_normal) {
+  using R = typename Rec::real;
+  const R oc[3] = {r.o[0] - Cy.c[0], r.o[1] - Cy.c[1], r.o[2] - Cy.c[2]};
+  R org[3];
   matTvec(Cy.R, oc, org);
   matTvec(Cy.R, r.d, dir);
-  const double x0 = org[0], y0 = org[1], z0 = org[2];
-  const double j = dir[0], k = dir[1], l = dir[2];
-  const double a = Cy.a, b = Cy.b, h = Cy.h;
-  const double zt = h / 2.0, zb = -h / 2.0;
+  const R x0 = org[0], y0 = org[1], z0 = org[2];
+  const R j = dir[0], k = dir[1], l = dir[2];
+  const R a = Cy.a, b = Cy.b, h = Cy.h;
+  const R zt = h / R(2), zb = -h / R(2);
 
   // shell quadratic shared by every region
   auto shell = [&](bool ordered, bool &hit) {
     hit = false;
-    const double A = a * a * k * k + b * b * j * j;
-    const double B = a * a * k * y0 + b * b * j * x0;
-    const double C = a * a * k * k + b * b * j * j - j * j * y0 * y0 + 2 * j * k * x0 * y0 - k * k * x0 * x0;
+    const R A = a * a * k * k + b * b * j * j;
+    const R B = a * a * k * y0 + b * b * j * x0;
+    const R C = a * a * k * k + b * b * j * j - j * j * y0 * y0 + 2 * j * k * x0 * y0 - k * k * x0 * x0;
     if (C < 0) return true;  // reference returns false immediately
-    const double s0 = 1.0 / A * (-B + a * b * dsqrt(C));
-    const double s1 = -1.0 / A * (B + a * b * dsqrt(C));
-    const double i0z = org[2] + dir[2] * s0;
-    const double i1z = org[2] + dir[2] * s1;
+    const R s0 = R(1) / A * (-B + a * b * dsqrt(C));
+    const R s1 = R(-1) / A * (B + a * b * dsqrt(C));
+    const R i0z = org[2] + dir[2] * s0;
+    const R i1z = org[2] + dir[2] * s1;
     if ((s1 > 0) && (!ordered || s1 < s0) && (i1z <= zt) && (i1z >= zb)) {
       out.t = s1;
       if (want_normal) {
-        const double nn[3] = {2 * (org[0] + dir[0] * s1) / (a * a), 2 * (org[1] + dir[1] * s1) / (b * b), 0};
+        const R nn[3] = {2 * (org[0] + dir[0] * s1) / (a * a), 2 * (org[1] + dir[1] * s1) / (b * b), 0};
         cyl_ng(Cy.R, nn, out.n);
       }
       hit = true;
@@ -362,7 +418,7 @@ __device__ bool cyl_eval(const CylRec &Cy, const Ray &r, double *dir, CylOut &ou
     if ((s0 > 0) && (!ordered || s0 < s1) && (i0z <= zt) && (i0z >= zb)) {
       out.t = s0;
       if (want_normal) {
-        const double nn[3] = {2 * (org[0] + dir[0] * s0) / (a * a), 2 * (org[1] + dir[1] * s0) / (b * b), 0};
+        const R nn[3] = {2 * (org[0] + dir[0] * s0) / (a * a), 2 * (org[1] + dir[1] * s0) / (b * b), 0};
         cyl_ng(Cy.R, nn, out.n);
       }
       hit = true;
@@ -370,10 +426,10 @@ __device__ bool cyl_eval(const CylRec &Cy, const Ray &r, double *dir, CylOut &ou
     }
     return false;  // fall through
   };
-  auto cap = [&](double zc, double sgn) {
+  auto cap = [&](R zc, R sgn) {
     out.t = (zc - z0) / l;
     if (want_normal) {
-      const double nz[3] = {0.0, 0.0, sgn};
+      const R nz[3] = {R(0), R(0), sgn};
       cyl_ng(Cy.R, nz, out.n);
     }
   };
@@ -381,11 +437,11 @@ __device__ bool cyl_eval(const CylRec &Cy, const Ray &r, double *dir, CylOut &ou
   if (z0 > zt || z0 < zb) {  // areas 1 and 2
     const bool top = z0 > zt;
     if (l == 0) return false;
-    const double t0 = top ? (zt - z0) / l : (zb - z0) / l;
+    const R t0 = top ? (zt - z0) / l : (zb - z0) / l;
     if (t0 < 0) return false;
-    const double ix = org[0] + t0 * dir[0], iy = org[1] + t0 * dir[1];
+    const R ix = org[0] + t0 * dir[0], iy = org[1] + t0 * dir[1];
     if (b * b * ix * ix + a * a * iy * iy <= a * a * b * b) {
-      cap(top ? zt : zb, top ? 1.0 : -1.0);
+      cap(top ? zt : zb, top ? R(1) : R(-1));
       return true;
     }
     bool hit;
@@ -399,29 +455,29 @@ __device__ bool cyl_eval(const CylRec &Cy, const Ray &r, double *dir, CylOut &ou
       if ((z0 == zb) || (z0 == zt)) return false;
       bool hit;
       if (shell(false, hit)) return hit;
-    } else if ((j == 0.0) && (k == 0.0)) {
+    } else if ((j == R(0)) && (k == R(0))) {
       if (b * b * x0 * x0 + a * a * y0 * y0 == a * a * b * b) return false;
       if (l < 0) {
-        cap(zb, -1.0);
+        cap(zb, R(-1));
         return true;
       }
       if (l > 0) {
-        cap(zt, 1.0);
+        cap(zt, R(1));
         return true;
       }
     } else {
-      const double A = a * a * k * k + b * b * j * j;
-      const double B = a * a * k * y0 + b * b * j * x0;
-      const double C = a * a * k * k + b * b * j * j - j * j * y0 * y0 + 2 * j * k * x0 * y0 - k * k * x0 * x0;
+      const R A = a * a * k * k + b * b * j * j;
+      const R B = a * a * k * y0 + b * b * j * x0;
+      const R C = a * a * k * k + b * b * j * j - j * j * y0 * y0 + 2 * j * k * x0 * y0 - k * k * x0 * x0;
       if (C < 0) return false;
-      const double s0 = 1.0 / A * (-B + a * b * dsqrt(C));
-      const double s1 = -1.0 / A * (B + a * b * dsqrt(C));
-      const double i0z = org[2] + dir[2] * s0;
-      const double i1z = org[2] + dir[2] * s1;
+      const R s0 = R(1) / A * (-B + a * b * dsqrt(C));
+      const R s1 = R(-1) / A * (B + a * b * dsqrt(C));
+      const R i0z = org[2] + dir[2] * s0;
+      const R i1z = org[2] + dir[2] * s1;
       if ((s1 > 0) && (i1z <= zt) && (i1z >= zb)) {
         out.t = s1;
         if (want_normal) {
-          const double nn[3] = {2 * (org[0] + dir[0] * s1) / (a * a), 2 * (org[1] + dir[1] * s1) / (b * b), 0};
+          const R nn[3] = {2 * (org[0] + dir[0] * s1) / (a * a), 2 * (org[1] + dir[1] * s1) / (b * b), 0};
           cyl_ng(Cy.R, nn, out.n);
         }
         return true;
@@ -436,11 +492,11 @@ __device__ bool cyl_eval(const CylRec &Cy, const Ray &r, double *dir, CylOut &ou
         return true;
       }
       if (l < 0) {
-        cap(zb, -1.0);
+        cap(zb, R(-1));
         return true;
       }
       if (l > 0) {
-        cap(zt, 1.0);
+        cap(zt, R(1));
         return true;
       }
     }
@@ -448,51 +504,67 @@ __device__ bool cyl_eval(const CylRec &Cy, const Ray &r, double *dir, CylOut &ou
   return false;
 }
 
-__device__ __forceinline__ bool isect(const CylRec &Cy, const Ray &r, Hit &s, uint32_t idx) {
-  double dir[3];
-  CylOut o;
+template <class Rec>
+__device__ __forceinline__ bool isect_cyl(const Rec &Cy, const RayT<typename Rec::real> &r,
+                                          HitT<typename Rec::real> &s, uint32_t idx) {
+  using R = typename Rec::real;
+  R dir[3];
+  CylOut<R> o;
   if (!cyl_eval(Cy, r, dir, o, false)) return false;
   s.h = norm_scaled(o.t, dir);
   s.leaf_idx = idx;
   return true;
 }
 
-__device__ __forceinline__ void normalize3(double *v) {
-  const double len = dsqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-  const double il = (len != 0.0) ? 1.0 / len : 0.0;
+template <class Rec>
+__device__ __forceinline__ bool isect(const Rec &rec, const RayT<typename Rec::real> &r, HitT<typename Rec::real> &s,
+                                      uint32_t idx) {
+  if constexpr (Rec::kind == TRIANGLES) return isect_tri(rec, r, s, idx);
+  else if constexpr (Rec::kind == SPHERES) return isect_sph(rec, r, s, idx);
+  else if constexpr (Rec::kind == PLANES) return isect_pla(rec, r, s, idx);
+  else return isect_cyl(rec, r, s, idx);
+}
+
+template <class R>
+__device__ __forceinline__ void normalize3(R *v) {
+  const R len = dsqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const R il = (len != R(0)) ? R(1) / len : R(0);
   v[0] *= il;
   v[1] *= il;
   v[2] *= il;
 }
 
 // post_traversal normals (trimesh.h:133-143, sphere.h:336-343, plane.h:153-159, cylinder.h:154-160)
-__device__ void post_normal(const TriRec *, const BvhView &bvh, const Ray &, const Hit &s, uint32_t pid, double *n) {
-  const uint32_t *f = bvh.faces + 3ull * pid;
-  const double *n0 = bvh.vnormals + 3ull * f[0];
-  const double *n1 = bvh.vnormals + 3ull * f[1];
-  const double *n2 = bvh.vnormals + 3ull * f[2];
-  const double w = (1.0 - s.u) - s.v;
+template <class Rec>
+__device__ void post_normal(const BvhView &bvh, const RayT<typename Rec::real> &r, const HitT<typename Rec::real> &s,
+                            uint32_t pid, typename Rec::real *n) {
+  using R = typename Rec::real;
+  const Rec *recs = reinterpret_cast<const Rec *>(bvh.prims);
+  if constexpr (Rec::kind == TRIANGLES) {
+    const uint32_t *f = bvh.faces + 3ull * pid;
+    const R *vn;
+    if constexpr (sizeof(R) == 8) vn = bvh.vnormals;
+    else vn = bvh.vnormals32;
+    const R *n0 = vn + 3ull * f[0];
+    const R *n1 = vn + 3ull * f[1];
+    const R *n2 = vn + 3ull * f[2];
+    const R w = (R(1) - s.u) - s.v;
 #pragma unroll
-  for (int k = 0; k < 3; k++) n[k] = (w * n0[k] + s.u * n1[k]) + s.v * n2[k];
-  normalize3(n);
-}
-__device__ void post_normal(const SphRec *recs, const BvhView &, const Ray &r, const Hit &s, uint32_t, double *n) {
-  const SphRec S = recs[s.leaf_idx];
+    for (int k = 0; k < 3; k++) n[k] = (w * n0[k] + s.u * n1[k]) + s.v * n2[k];
+  } else if constexpr (Rec::kind == SPHERES) {
+    const Rec S = recs[s.leaf_idx];
 #pragma unroll
-  for (int k = 0; k < 3; k++) n[k] = (r.o[k] + s.h * r.d[k]) - S.c[k];
-  normalize3(n);
-}
-__device__ void post_normal(const PlaRec *recs, const BvhView &, const Ray &r, const Hit &s, uint32_t, double *n) {
-  plane_normal(recs[s.leaf_idx], r, n);
-  normalize3(n);
-}
-__device__ void post_normal(const CylRec *recs, const BvhView &, const Ray &r, const Hit &s, uint32_t, double *n) {
-  double dir[3];
-  CylOut o;
-  cyl_eval(recs[s.leaf_idx], r, dir, o, true);
-  n[0] = o.n[0];
-  n[1] = o.n[1];
-  n[2] = o.n[2];
+    for (int k = 0; k < 3; k++) n[k] = (r.o[k] + s.h * r.d[k]) - S.c[k];
+  } else if constexpr (Rec::kind == PLANES) {
+    plane_normal(recs[s.leaf_idx], r, n);
+  } else {
+    R dir[3];
+    CylOut<R> o;
+    cyl_eval(recs[s.leaf_idx], r, dir, o, true);
+    n[0] = o.n[0];
+    n[1] = o.n[1];
+    n[2] = o.n[2];
+  }
   normalize3(n);
 }
 
@@ -515,20 +587,24 @@ struct Trav {
 template <bool SPLIT>
 constexpr uint32_t ring_default() { return SPLIT ? 16u : 8u; }
 
-template <bool SPLIT>
+// Stack keys: FP32 entry distances on the split path, otherwise the scene type's.
+template <bool SPLIT, class R>
+using StackKey = typename std::conditional<SPLIT, float, R>::type;
+
+template <bool SPLIT, class R = double>
 struct SpillArea {  // private (scratch) memory for entries pushed deeper than the ring
-  using Key = typename std::conditional<SPLIT, float, double>::type;
+  using Key = StackKey<SPLIT, R>;
   Key key[stack_cap<SPLIT>()];
   uint32_t ref[stack_cap<SPLIT>()], cnt[stack_cap<SPLIT>()];
 };
 
 // The counters and pointers are kept apart from the spill arrays so that they stay in registers.
-template <bool SPLIT>
+template <bool SPLIT, class R = double>
 struct RingStack {
-  using Key = typename std::conditional<SPLIT, float, double>::type;
+  using Key = StackKey<SPLIT, R>;
   Key *lkey;  // this lane's slot 0 in LDS; slot j is at [64 * j]
   uint32_t *lref, *lcnt;
-  SpillArea<SPLIT> *spill;
+  SpillArea<SPLIT, R> *spill;
   int sp, lo;
   int ring;  // LDS slots per lane (power of two)
 
@@ -568,15 +644,15 @@ struct RingStack {
 };
 
 // LDS bytes of the ring stacks of one block
-template <bool SPLIT>
+template <bool SPLIT, class R = double>
 constexpr size_t ring_lds_bytes(uint32_t block, uint32_t ring) {
-  return (size_t)(block / 64) * ring * 64 * (sizeof(typename RingStack<SPLIT>::Key) + 8);
+  return (size_t)(block / 64) * ring * 64 * (sizeof(StackKey<SPLIT, R>) + 8);
 }
 
-template <bool SPLIT>
-__device__ __forceinline__ void ring_bind(RingStack<SPLIT> &st, unsigned char *smem, SpillArea<SPLIT> *spill,
+template <bool SPLIT, class R>
+__device__ __forceinline__ void ring_bind(RingStack<SPLIT, R> &st, unsigned char *smem, SpillArea<SPLIT, R> *spill,
                                           uint32_t ring) {
-  using Key = typename RingStack<SPLIT>::Key;
+  using Key = StackKey<SPLIT, R>;
   const uint32_t waves = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   Key *keys = reinterpret_cast<Key *>(smem);
   uint32_t *refs = reinterpret_cast<uint32_t *>(keys + (size_t)waves * ring * 64);
@@ -612,35 +688,39 @@ __device__ __forceinline__ void sort4(float (&k)[4], uint32_t (&r)[4], uint32_t 
 }
 
 // Root test (the reference pushes node 0 and slab-tests it on the first pop with the ray's tmax).
-template <bool SPLIT>
-__device__ __forceinline__ void trav_begin(const BvhView &bvh, const Ray &r, const Ray32 &q, const Hit &s, Trav &t) {
+template <bool SPLIT, class R>
+__device__ __forceinline__ void trav_begin(const BvhView &bvh, const RayT<R> &r, const Ray32 &q, const HitT<R> &s,
+                                           Trav &t) {
   t.cur = SPLIT ? bvh.root_ref4 : bvh.root_ref;
   t.cnt = bvh.root_cnt;
   t.pos = 0;
-  if (SPLIT) {
+  if constexpr (SPLIT) {
     float k32;
     const float rb[6] = {bvh.root_min32[0], bvh.root_min32[1], bvh.root_min32[2],
                          bvh.root_max32[0], bvh.root_max32[1], bvh.root_max32[2]};
     t.h32 = f32_up(s.h);
     t.active = slab32(rb, q, k32) && !(t.h32 < k32);
   } else {
-    double key;
-    const double rb[6] = {bvh.root_min[0], bvh.root_min[1], bvh.root_min[2],
-                          bvh.root_max[0], bvh.root_max[1], bvh.root_max[2]};
+    // the descriptor holds the root box in FP64; a Scene<float> box converts back exactly
+    R key;
+    const R rb[6] = {R(bvh.root_min[0]), R(bvh.root_min[1]), R(bvh.root_min[2]),
+                     R(bvh.root_max[0]), R(bvh.root_max[1]), R(bvh.root_max[2])};
     t.h32 = 0.0f;
-    t.active = slab64(rb, r, key) && !(s.h < key);
+    t.active = slab_exact(rb, r, key) && !(s.h < key);
   }
 }
 
 // One unit of the DFS: an internal node (test both children, push far, descend near) or a leaf (test its
 // primitives in order, blazert/bvh/intersect.h:66-78), followed by pops until a node passes or the stack is empty.
 template <class Rec, bool SPLIT>
-__device__ __forceinline__ void trav_step(const BvhView &bvh, const Ray &r, const Ray32 &q, bool any_hit, Hit &s,
-                                          Trav &t, RingStack<SPLIT> &st) {
+__device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typename Rec::real> &r, const Ray32 &q,
+                                          bool any_hit, HitT<typename Rec::real> &s, Trav &t,
+                                          RingStack<SPLIT, typename Rec::real> &st) {
+  using R = typename Rec::real;
   if (!(t.cur & LEAF_BIT)) {
     bool near_ok, far_ok;
     uint32_t nref, ncnt, fref, fcnt;
-    typename std::conditional<SPLIT, float, double>::type fkey;
+    StackKey<SPLIT, R> fkey;
     if constexpr (SPLIT) {
       // up to four children per record, visited nearest entry first (the FP32 path needs no particular order:
       // it only has to test every primitive the FP64 path tests)
@@ -676,12 +756,16 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const Ray &r, cons
       nref = ncnt = fref = fcnt = 0;
       fkey = 0.0f;
     } else {
-      const Node64 nd = bvh.nodes64[t.cur];
+      using Node = typename std::conditional<sizeof(R) == 8, Node64, Node32>::type;
+      const Node *nodes;
+      if constexpr (sizeof(R) == 8) nodes = bvh.nodes64;
+      else nodes = bvh.nodes32;
+      const Node nd = nodes[t.cur];
       const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
       const uint32_t nc = (r.sign >> axis) & 1u;
-      double k0 = 0.0, k1 = 0.0;
-      const bool p0 = slab64(nd.b, r, k0);
-      const bool p1 = slab64(nd.b + 6, r, k1);
+      R k0 = R(0), k1 = R(0);
+      const bool p0 = slab_exact(nd.b, r, k0);
+      const bool p1 = slab_exact(nd.b + 6, r, k1);
       near_ok = (nc ? p1 : p0) && !(s.h < (nc ? k1 : k0));
       far_ok = nc ? p0 : p1;
       fkey = nc ? k0 : k1;
@@ -719,9 +803,11 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const Ray &r, cons
   // pop the next far child whose entry distance does not exceed the current hit distance
   while (st.sp > 0) {
     uint32_t ref, cnt;
-    typename RingStack<SPLIT>::Key key;
+    StackKey<SPLIT, R> key;
     st.pop(ref, cnt, key);
-    const bool ok = SPLIT ? !(t.h32 < (float)key) : !(s.h < (double)key);
+    bool ok;
+    if constexpr (SPLIT) ok = !(t.h32 < key);
+    else ok = !(s.h < key);
     if (ok) {
       t.cur = ref;
       t.cnt = cnt;
@@ -735,13 +821,15 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const Ray &r, cons
 // Merge into the hit record (blazert/scene.h:169-209): triangles replace unconditionally, later types only if
 // strictly closer; the first type of the scene also writes the default RayHit for misses.
 template <class Rec>
-__device__ __forceinline__ void finish_ray(const BvhView &bvh, const HitsView &hits, uint64_t i, const Ray &r,
-                                           const Hit &s, uint32_t flags, uint32_t first, uint32_t is_tri) {
+__device__ __forceinline__ void finish_ray(const BvhView &bvh, const HitsViewT<typename Rec::real> &hits, uint64_t i,
+                                           const RayT<typename Rec::real> &r, const HitT<typename Rec::real> &s,
+                                           uint32_t flags, uint32_t first, uint32_t is_tri) {
+  using R = typename Rec::real;
   const bool hit = s.h < r.tmax;
   bool write = false;
   if (hit) {
     if (is_tri) write = true;
-    else write = s.h < ((first && !(flags & FLAG_INOUT)) ? DBL_MAX : hits.t[i]);
+    else write = s.h < ((first && !(flags & FLAG_INOUT)) ? Real<R>::max : hits.t[i]);
   }
   if (write) {
     const uint32_t pid = bvh.prim_ids[s.leaf_idx];
@@ -752,54 +840,56 @@ __device__ __forceinline__ void finish_ray(const BvhView &bvh, const HitsView &h
     hits.prim_id[i] = pid;
     if (hits.geom_id) hits.geom_id[i] = bvh.geom_id;
     if (hits.nx) {
-      double nrm[3];
-      post_normal(reinterpret_cast<const Rec *>(bvh.prims), bvh, r, s, pid, nrm);
+      R nrm[3];
+      post_normal<Rec>(bvh, r, s, pid, nrm);
       hits.nx[i] = nrm[0];
       hits.ny[i] = nrm[1];
       hits.nz[i] = nrm[2];
     }
   } else if (first && !(flags & FLAG_INOUT)) {
-    hits.t[i] = DBL_MAX;
-    if (hits.u) hits.u[i] = 0.0;
-    if (hits.v) hits.v[i] = 0.0;
+    hits.t[i] = Real<R>::max;
+    if (hits.u) hits.u[i] = R(0);
+    if (hits.v) hits.v[i] = R(0);
     hits.prim_id[i] = INVALID_ID;
     if (hits.geom_id) hits.geom_id[i] = INVALID_ID;
     if (hits.nx) {
-      hits.nx[i] = 0.0;
-      hits.ny[i] = 0.0;
-      hits.nz[i] = 0.0;
+      hits.nx[i] = R(0);
+      hits.ny[i] = R(0);
+      hits.nz[i] = R(0);
     }
   }
 }
 
-template <bool SPLIT>
-__device__ __forceinline__ void start_ray(const BvhView &bvh, const RaysView &rays, uint64_t i, double S, Ray &r,
-                                          Ray32 &q, Hit &s) {
+template <bool SPLIT, class R>
+__device__ __forceinline__ void start_ray(const BvhView &bvh, const RaysViewT<R> &rays, uint64_t i, double S,
+                                          RayT<R> &r, Ray32 &q, HitT<R> &s) {
   make_ray(rays, i, r);
   if (bvh.nan_boxes) r.sign &= ~RAY_NAN_FREE;
-  if (SPLIT) make_ray32(r, S, q);
+  if constexpr (SPLIT) make_ray32(r, S, q);
   s.h = r.tmax;  // prepare_traversal
-  s.u = 0.0;
-  s.v = 0.0;
+  s.u = R(0);
+  s.v = R(0);
   s.leaf_idx = INVALID_ID;
 }
 
 // One ray per thread (grid-stride).
 template <class Rec, bool SPLIT>
-__global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysView rays, HitsView hits, uint64_t n, uint32_t flags,
+__global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysViewT<typename Rec::real> rays,
+                                               HitsViewT<typename Rec::real> hits, uint64_t n, uint32_t flags,
                                                uint32_t first, uint32_t is_tri, double S, const uint32_t *order,
                                                uint32_t ring) {
+  using R = typename Rec::real;
   extern __shared__ unsigned char smem[];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
-  SpillArea<SPLIT> spill;
-  RingStack<SPLIT> st;
-  ring_bind<SPLIT>(st, smem, &spill, ring);
+  SpillArea<SPLIT, R> spill;
+  RingStack<SPLIT, R> st;
+  ring_bind<SPLIT, R>(st, smem, &spill, ring);
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     const uint64_t i = order ? (uint64_t)order[k] : k;
-    Ray r;
+    RayT<R> r;
     Ray32 q;
-    Hit s;
+    HitT<R> s;
     start_ray<SPLIT>(bvh, rays, i, S, r, q, s);
     Trav t;
     trav_begin<SPLIT>(bvh, r, q, s, t);
@@ -859,19 +949,19 @@ __device__ __forceinline__ void pool_refill(WavePool &P, unsigned long long *cou
 
 template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
-k_trace_persistent(BvhView bvh, RaysView rays, HitsView hits, uint64_t n,
-                                                          uint32_t flags, uint32_t first, uint32_t is_tri, double S,
-                                                          unsigned long long *counter, uint32_t refill,
-                                                          const uint32_t *order, uint32_t ring) {
+k_trace_persistent(BvhView bvh, RaysViewT<typename Rec::real> rays, HitsViewT<typename Rec::real> hits, uint64_t n,
+                   uint32_t flags, uint32_t first, uint32_t is_tri, double S, unsigned long long *counter,
+                   uint32_t refill, const uint32_t *order, uint32_t ring) {
+  using R = typename Rec::real;
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
   const uint32_t lane = __lane_id();
-  SpillArea<SPLIT> spill;
-  RingStack<SPLIT> st;
-  ring_bind<SPLIT>(st, smem, &spill, ring);
-  Ray r;
+  SpillArea<SPLIT, R> spill;
+  RingStack<SPLIT, R> st;
+  ring_bind<SPLIT, R>(st, smem, &spill, ring);
+  RayT<R> r;
   Ray32 q;
-  Hit s;
+  HitT<R> s;
   Trav t;
   t.active = false;
   bool has_ray = false;
@@ -1003,29 +1093,31 @@ __global__ void k_sort_keys(RaysView rays, uint64_t n, double lx, double ly, dou
   }
 }
 
-__global__ void k_init_hits(HitsView hits, uint64_t n) {
+template <class R>
+__global__ void k_init_hits(HitsViewT<R> hits, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    hits.t[i] = DBL_MAX;
-    if (hits.u) hits.u[i] = 0.0;
-    if (hits.v) hits.v[i] = 0.0;
+    hits.t[i] = Real<R>::max;
+    if (hits.u) hits.u[i] = R(0);
+    if (hits.v) hits.v[i] = R(0);
     hits.prim_id[i] = INVALID_ID;
     if (hits.geom_id) hits.geom_id[i] = INVALID_ID;
     if (hits.nx) {
-      hits.nx[i] = 0.0;
-      hits.ny[i] = 0.0;
-      hits.nz[i] = 0.0;
+      hits.nx[i] = R(0);
+      hits.ny[i] = R(0);
+      hits.nz[i] = R(0);
     }
   }
 }
 
 template <class Rec, bool SPLIT>
-hipError_t launch_kernel(const BvhView &bvh, const RaysView &rays, const HitsView &hits, uint64_t n,
-                         const TraceLaunch &cfg, const SplitParams &sp, uint32_t is_tri, hipStream_t stream) {
+hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real> &rays,
+                         const HitsViewT<typename Rec::real> &hits, uint64_t n, const TraceLaunch &cfg,
+                         const SplitParams &sp, uint32_t is_tri, hipStream_t stream) {
   const uint32_t block = cfg.block ? cfg.block : 256;
   const uint32_t first = cfg.first ? 1u : 0u;
   const uint32_t ring = cfg.ring ? cfg.ring : ring_default<SPLIT>();
-  const size_t lds = ring_lds_bytes<SPLIT>(block, ring);
+  const size_t lds = ring_lds_bytes<SPLIT, typename Rec::real>(block, ring);
   if (cfg.schedule == SCHEDULE_PERSISTENT) {
     if (!cfg.counter) return hipErrorInvalidValue;
     int dev = 0, cus = 0, per_cu = 0;
@@ -1169,12 +1261,30 @@ hipError_t launch_trace_batch(GeomType type, const BatchJob *jobs, uint32_t njob
   }
 }
 
-hipError_t launch_init_hits(const HitsView &hits, uint64_t n, hipStream_t stream) {
+template <class R>
+hipError_t init_hits(const HitsViewT<R> &hits, uint64_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   uint64_t blocks = (n + 255) / 256;
   if (blocks > (1ull << 30)) blocks = 1ull << 30;
-  hipLaunchKernelGGL(k_init_hits, dim3((uint32_t)blocks), dim3(256), 0, stream, hits, n);
+  hipLaunchKernelGGL(k_init_hits<R>, dim3((uint32_t)blocks), dim3(256), 0, stream, hits, n);
   return hipGetLastError();
+}
+hipError_t launch_init_hits(const HitsView &hits, uint64_t n, hipStream_t stream) { return init_hits(hits, n, stream); }
+hipError_t launch_init_hits_f32(const HitsViewF &hits, uint64_t n, hipStream_t stream) {
+  return init_hits(hits, n, stream);
+}
+
+// Scene<float>: the exact path in FP32 (no split precision: the scene type is already FP32).
+hipError_t launch_trace_f32(GeomType type, const BvhView &bvh, const RaysViewF &rays, const HitsViewF &hits,
+                            uint64_t n, const TraceLaunch &cfg, hipStream_t stream) {
+  const SplitParams sp{0.0};
+  const uint32_t ow = cfg.overwrite ? 1u : 0u;
+  switch (type) {
+  case TRIANGLES: return launch_kernel<TriRecF, false>(bvh, rays, hits, n, cfg, sp, 1u, stream);
+  case SPHERES: return launch_kernel<SphRecF, false>(bvh, rays, hits, n, cfg, sp, ow, stream);
+  case PLANES: return launch_kernel<PlaRecF, false>(bvh, rays, hits, n, cfg, sp, ow, stream);
+  default: return launch_kernel<CylRecF, false>(bvh, rays, hits, n, cfg, sp, ow, stream);
+  }
 }
 
 }  // namespace brt

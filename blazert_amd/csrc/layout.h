@@ -8,6 +8,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <limits>
+
 namespace brt {
 
 constexpr uint32_t LEAF_BIT = 0x80000000u;
@@ -41,6 +43,8 @@ static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 
 // Triangle record in leaf order: v0, e1 = v0 - v1, e2 = v2 - v0 (blazert/primitives/trimesh.h:26-27).
 struct alignas(16) TriRec {
+  using real = double;
+  static constexpr uint32_t kind = TRIANGLES;
   double v0[3];
   double e1[3];
   double e2[3];
@@ -49,6 +53,8 @@ struct alignas(16) TriRec {
 static_assert(sizeof(TriRec) == 80, "TriRec size");
 
 struct alignas(32) SphRec {
+  using real = double;
+  static constexpr uint32_t kind = SPHERES;
   double c[3];
   double r;
 };
@@ -56,6 +62,8 @@ static_assert(sizeof(SphRec) == 32, "SphRec size");
 
 // Plane: center, extents and the row-major rotation (blazert/primitives/plane.h:19-39).
 struct alignas(128) PlaRec {
+  using real = double;
+  static constexpr uint32_t kind = PLANES;
   double c[3];
   double dx, dy;
   double R[9];
@@ -65,12 +73,61 @@ static_assert(sizeof(PlaRec) == 128, "PlaRec size");
 
 // Cylinder: center, semi axes a/b, height and the row-major rotation (blazert/primitives/cylinder.h:19-40).
 struct alignas(128) CylRec {
+  using real = double;
+  static constexpr uint32_t kind = CYLINDERS;
   double c[3];
   double a, b, h;
   double R[9];
   double pad;
 };
 static_assert(sizeof(CylRec) == 128, "CylRec size");
+
+// Scene<float> (the reference instantiated with T = float): the same children-in-parent node with FP32 boxes
+// (64 B) and FP32 primitive records, traversed and tested in FP32 throughout.
+struct alignas(64) Node32 {
+  float b[12];
+  uint32_t ref[2];
+  uint32_t cnt[2];
+};
+static_assert(sizeof(Node32) == 64, "Node32 size");
+
+struct alignas(16) TriRecF {
+  using real = float;
+  static constexpr uint32_t kind = TRIANGLES;
+  float v0[3];
+  float e1[3];
+  float e2[3];
+  float pad[3];
+};
+static_assert(sizeof(TriRecF) == 48, "TriRecF size");
+
+struct alignas(16) SphRecF {
+  using real = float;
+  static constexpr uint32_t kind = SPHERES;
+  float c[3];
+  float r;
+};
+static_assert(sizeof(SphRecF) == 16, "SphRecF size");
+
+struct alignas(64) PlaRecF {
+  using real = float;
+  static constexpr uint32_t kind = PLANES;
+  float c[3];
+  float dx, dy;
+  float R[9];
+  float pad[2];
+};
+static_assert(sizeof(PlaRecF) == 64, "PlaRecF size");
+
+struct alignas(64) CylRecF {
+  using real = float;
+  static constexpr uint32_t kind = CYLINDERS;
+  float c[3];
+  float a, b, h;
+  float R[9];
+  float pad;
+};
+static_assert(sizeof(CylRecF) == 64, "CylRecF size");
 
 // Kernel-side view of one uploaded geometry BVH (passed by value).
 struct BvhView {
@@ -81,6 +138,8 @@ struct BvhView {
   const uint32_t *faces;      // triangles only: user faces (3 per primitive id)
   const double *vnormals;     // triangles only: vertex normals (3 per vertex)
   const double *vertices;     // triangles only: user vertices (3 per vertex), for mirror tracing and its adjoint
+  const Node32 *nodes32;      // Scene<float>: FP32 nodes (nodes64 / nodes4 unused)
+  const float *vnormals32;    // Scene<float> triangles: vertex normals
   double root_min[3], root_max[3];
   float root_min32[3], root_max32[3];
   uint32_t root_ref, root_cnt;
@@ -92,18 +151,24 @@ struct BvhView {
   uint32_t nan_boxes;         // some node box coordinate is NaN (disables the NaN-free slab path)
 };
 
-// Structure-of-arrays rays / hits in device memory (mirrors brt_rays / brt_hits of the C ABI). tmin / tmax may be
-// NULL: every ray then uses tmin0 / tmax0.
-struct RaysView {
-  const double *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
-  double tmin0 = 0.0;
-  double tmax0 = 1.7976931348623157e308;
+// Structure-of-arrays rays / hits in device memory (mirrors brt_rays / brt_hits of the C ABI, and their FP32
+// forms for Scene<float>). tmin / tmax may be NULL: every ray then uses tmin0 / tmax0 (0 and the type's max).
+template <class R>
+struct RaysViewT {
+  const R *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+  R tmin0 = R(0);
+  R tmax0 = std::numeric_limits<R>::max();
 };
-struct HitsView {
-  double *t, *u, *v;
+template <class R>
+struct HitsViewT {
+  R *t, *u, *v;
   uint32_t *prim_id, *geom_id;
-  double *nx, *ny, *nz;
+  R *nx, *ny, *nz;
   uint32_t *merged;  // optional: set to 1 where a geometry type's hit was merged (intersect1's return value)
 };
+using RaysView = RaysViewT<double>;
+using HitsView = HitsViewT<double>;
+using RaysViewF = RaysViewT<float>;
+using HitsViewF = HitsViewT<float>;
 
 }  // namespace brt

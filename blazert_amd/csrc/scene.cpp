@@ -20,6 +20,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/blazert_amd.h"
@@ -41,69 +42,94 @@ namespace {
 // ------------------------------------------------------------------------------------------------ vec helpers
 // blaze conventions: dot = (a0 b0 + a1 b1) + a2 b2, normalize = v * (len != 0 ? 1/len : 0),
 // vec / s = vec * (1/s), matrix-vector rows summed left to right.
-struct V3 {
-  double x[3];
-  double &operator[](int i) { return x[i]; }
-  double operator[](int i) const { return x[i]; }
+// Every helper is generic over the scene type R (double, or float for Scene<float>), with the reference's
+// operation order in either type.
+template <class R>
+struct V3T {
+  R x[3];
+  R &operator[](int i) { return x[i]; }
+  R operator[](int i) const { return x[i]; }
 };
-inline V3 sub(const V3 &a, const V3 &b) { return {{a[0] - b[0], a[1] - b[1], a[2] - b[2]}}; }
-inline double dot(const V3 &a, const V3 &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-inline V3 cross(const V3 &a, const V3 &b) {
+using V3 = V3T<double>;
+template <class T>
+struct Id {
+  using type = T;
+};
+template <class R>
+inline V3T<R> sub(const V3T<R> &a, const V3T<R> &b) {
+  return {{a[0] - b[0], a[1] - b[1], a[2] - b[2]}};
+}
+template <class R>
+inline R dot(const V3T<R> &a, const V3T<R> &b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+template <class R>
+inline V3T<R> cross(const V3T<R> &a, const V3T<R> &b) {
   return {{a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}};
 }
-inline V3 scale_by_inverse(const V3 &a, double s) {
-  const double is = 1.0 / s;
+template <class R>
+inline V3T<R> scale_by_inverse(const V3T<R> &a, typename Id<R>::type s) {
+  const R is = R(1) / s;
   return {{a[0] * is, a[1] * is, a[2] * is}};
 }
-inline V3 normalized(const V3 &a) {
-  const double len = std::sqrt(dot(a, a));
-  const double il = (len != 0.0) ? 1.0 / len : 0.0;
+template <class R>
+inline V3T<R> normalized(const V3T<R> &a) {
+  const R len = std::sqrt(dot(a, a));
+  const R il = (len != R(0)) ? R(1) / len : R(0);
   return {{a[0] * il, a[1] * il, a[2] * il}};
 }
-inline V3 rot_apply(const double *R, const V3 &v) {
-  return {{R[0] * v[0] + R[1] * v[1] + R[2] * v[2], R[3] * v[0] + R[4] * v[1] + R[5] * v[2],
-           R[6] * v[0] + R[7] * v[1] + R[8] * v[2]}};
+template <class R>
+inline V3T<R> rot_apply(const R *M, const V3T<R> &v) {
+  return {{M[0] * v[0] + M[1] * v[1] + M[2] * v[2], M[3] * v[0] + M[4] * v[1] + M[5] * v[2],
+           M[6] * v[0] + M[7] * v[1] + M[8] * v[2]}};
 }
-inline double ref_min(double a, double b) { return (b < a) ? b : a; }  // std::min(a, b)
-inline double ref_max(double a, double b) { return (a < b) ? b : a; }  // std::max(a, b)
+template <class R>
+inline R ref_min(R a, R b) { return (b < a) ? b : a; }  // std::min(a, b)
+template <class R>
+inline R ref_max(R a, R b) { return (a < b) ? b : a; }  // std::max(a, b)
 
 // ------------------------------------------------------------------------------------------------ collections
 // Per-primitive bounding boxes and centers in structure-of-arrays form (the collection precompute of
 // trimesh.h:78-130, sphere.h:301-329, plane.h:89-150, cylinder.h:91-147).
-struct Boxes {
-  std::vector<double> lo, hi, ctr;  // 3 per primitive
+template <class R>
+struct BoxesT {
+  std::vector<R> lo, hi, ctr;  // 3 per primitive
   uint32_t n = 0;
   // every box coordinate is finite and none is -0.0: then every min / max the builder takes has one result
   // whatever the evaluation order, which the device builder relies on
   bool device_buildable = false;
   void resize(uint32_t m) {
     n = m;
-    lo.assign(3ull * m, 0.0);
-    hi.assign(3ull * m, 0.0);
-    ctr.assign(3ull * m, 0.0);
+    lo.assign(3ull * m, R(0));
+    hi.assign(3ull * m, R(0));
+    ctr.assign(3ull * m, R(0));
   }
   void classify() {
-    auto plain = [](double x) { return std::isfinite(x) && !(x == 0.0 && std::signbit(x)); };
+    auto plain = [](R x) { return std::isfinite(x) && !(x == R(0) && std::signbit(x)); };
     device_buildable = std::all_of(lo.begin(), lo.end(), plain) && std::all_of(hi.begin(), hi.end(), plain);
   }
 };
 
-struct Geometry {
+using Boxes = BoxesT<double>;
+
+template <class R>
+struct GeometryT {
   bool present = false;
   uint32_t geom_id = INVALID_ID;
-  Boxes boxes;
+  BoxesT<R> boxes;
   // inputs (copied)
-  std::vector<double> vertices;  // triangles
+  std::vector<R> vertices;  // triangles
   std::vector<uint32_t> faces;
-  std::vector<double> vnormals;
-  std::vector<double> centers, radii, dxs, dys, rot, sa, sb, heights;
+  std::vector<R> vnormals;
+  std::vector<R> centers, radii, dxs, dys, rot, sa, sb, heights;
   // build output
   std::vector<brt_tree_node> tree;  // reference form, pre-order
   std::vector<uint32_t> order;      // primitive ids in leaf order
   brt_build_stats stats{};
   // flattened
-  std::vector<Node64> n64;
+  std::vector<Node64> n64;  // double scenes
   std::vector<Node4> n4;
+  std::vector<Node32> n32;  // Scene<float>
   std::vector<unsigned char> recs;
   size_t rec_size = 0;
   uint32_t num_n4 = 0;
@@ -115,19 +141,24 @@ struct Geometry {
   const brt_tree_node *dtree = nullptr;
   uint32_t dtree_nodes = 0;
 };
+using Geometry = GeometryT<double>;
+using GeometryF = GeometryT<float>;
 
-void mesh_precompute(Geometry &g, uint32_t nv) {
+template <class R>
+void mesh_precompute(GeometryT<R> &g, uint32_t nv) {
   const uint32_t nf = (uint32_t)(g.faces.size() / 3);
   g.boxes.resize(nf);
-  g.vnormals.assign(3ull * nv, 0.0);
-  auto vert = [&](uint32_t i) { return V3{{g.vertices[3ull * i], g.vertices[3ull * i + 1], g.vertices[3ull * i + 2]}}; };
+  g.vnormals.assign(3ull * nv, R(0));
+  auto vert = [&](uint32_t i) {
+    return V3T<R>{{g.vertices[3ull * i], g.vertices[3ull * i + 1], g.vertices[3ull * i + 2]}};
+  };
   for (uint32_t f = 0; f < nf; f++) {
     const uint32_t i0 = g.faces[3ull * f], i1 = g.faces[3ull * f + 1], i2 = g.faces[3ull * f + 2];
-    const V3 v0 = vert(i0), v1 = vert(i1), v2 = vert(i2);
-    const V3 sum{{(v0[0] + v1[0]) + v2[0], (v0[1] + v1[1]) + v2[1], (v0[2] + v1[2]) + v2[2]}};
-    const V3 c = scale_by_inverse(sum, 3.0);
-    V3 lo = v0, hi = v0;
-    for (const V3 *p : {&v1, &v2})
+    const V3T<R> v0 = vert(i0), v1 = vert(i1), v2 = vert(i2);
+    const V3T<R> sum{{(v0[0] + v1[0]) + v2[0], (v0[1] + v1[1]) + v2[1], (v0[2] + v1[2]) + v2[2]}};
+    const V3T<R> c = scale_by_inverse(sum, R(3));
+    V3T<R> lo = v0, hi = v0;
+    for (const V3T<R> *p : {&v1, &v2})
       for (int k = 0; k < 3; k++) {
         lo[k] = ref_min((*p)[k], lo[k]);
         hi[k] = ref_max((*p)[k], hi[k]);
@@ -138,18 +169,19 @@ void mesh_precompute(Geometry &g, uint32_t nv) {
       g.boxes.ctr[3ull * f + k] = c[k];
     }
     // face normal normalize(cross(v1 - v0, v2 - v0)); each vertex accumulates fn / 3 in face order
-    const V3 fn3 = scale_by_inverse(normalized(cross(sub(v1, v0), sub(v2, v0))), 3.0);
+    const V3T<R> fn3 = scale_by_inverse(normalized(cross(sub(v1, v0), sub(v2, v0))), R(3));
     for (uint32_t vi : {i0, i1, i2})
       for (int k = 0; k < 3; k++) g.vnormals[3ull * vi + k] = g.vnormals[3ull * vi + k] + fn3[k];
   }
 }
 
-void sphere_precompute(Geometry &g) {
+template <class R>
+void sphere_precompute(GeometryT<R> &g) {
   const uint32_t n = (uint32_t)g.radii.size();
   g.boxes.resize(n);
   for (uint32_t i = 0; i < n; i++)
     for (int k = 0; k < 3; k++) {
-      const double c = g.centers[3ull * i + k];
+      const R c = g.centers[3ull * i + k];
       g.boxes.lo[3ull * i + k] = c - g.radii[i];
       g.boxes.hi[3ull * i + k] = c + g.radii[i];
       g.boxes.ctr[3ull * i + k] = c;
@@ -157,9 +189,10 @@ void sphere_precompute(Geometry &g) {
 }
 
 // Box of six points: first element wins ties as in std::min / std::max over an initializer list.
-void six_point_box(const V3 *pts, double *lo, double *hi) {
+template <class R>
+void six_point_box(const V3T<R> *pts, R *lo, R *hi) {
   for (int k = 0; k < 3; k++) {
-    double mn = pts[0][k], mx = pts[0][k];
+    R mn = pts[0][k], mx = pts[0][k];
     for (int p = 1; p < 6; p++) {
       if (pts[p][k] < mn) mn = pts[p][k];
       if (mx < pts[p][k]) mx = pts[p][k];
@@ -169,19 +202,20 @@ void six_point_box(const V3 *pts, double *lo, double *hi) {
   }
 }
 
-void plane_precompute(Geometry &g) {
+template <class R>
+void plane_precompute(GeometryT<R> &g) {
   const uint32_t n = (uint32_t)g.dxs.size();
   g.boxes.resize(n);
-  const double th = DBL_MIN;  // std::numeric_limits<double>::min(), plane.h:29
+  const R th = std::numeric_limits<R>::min();  // plane.h:29
   for (uint32_t i = 0; i < n; i++) {
-    const double *R = &g.rot[9ull * i];
-    const V3 c{{g.centers[3ull * i], g.centers[3ull * i + 1], g.centers[3ull * i + 2]}};
-    const double dx = g.dxs[i], dy = g.dys[i];
-    const V3 loc[6] = {{{-dx / 2.0, -dy / 2.0, 0.0}}, {{dx / 2.0, -dy / 2.0, 0.0}}, {{-dx / 2.0, dy / 2.0, 0.0}},
-                       {{dx / 2.0, dy / 2.0, 0.0}},   {{0.0, 0.0, -th}},           {{0.0, 0.0, th}}};
-    V3 pts[6];
+    const R *M = &g.rot[9ull * i];
+    const V3T<R> c{{g.centers[3ull * i], g.centers[3ull * i + 1], g.centers[3ull * i + 2]}};
+    const R dx = g.dxs[i], dy = g.dys[i];
+    const V3T<R> loc[6] = {{{-dx / 2, -dy / 2, R(0)}}, {{dx / 2, -dy / 2, R(0)}}, {{-dx / 2, dy / 2, R(0)}},
+                           {{dx / 2, dy / 2, R(0)}},   {{R(0), R(0), -th}},      {{R(0), R(0), th}}};
+    V3T<R> pts[6];
     for (int p = 0; p < 6; p++) {
-      const V3 r = rot_apply(R, loc[p]);
+      const V3T<R> r = rot_apply(M, loc[p]);
       pts[p] = {{c[0] + r[0], c[1] + r[1], c[2] + r[2]}};
     }
     six_point_box(pts, &g.boxes.lo[3ull * i], &g.boxes.hi[3ull * i]);
@@ -189,16 +223,17 @@ void plane_precompute(Geometry &g) {
   }
 }
 
-void cylinder_precompute(Geometry &g) {
+template <class R>
+void cylinder_precompute(GeometryT<R> &g) {
   const uint32_t n = (uint32_t)g.sa.size();
   g.boxes.resize(n);
   for (uint32_t i = 0; i < n; i++) {
-    const double *R = &g.rot[9ull * i];
-    const V3 c{{g.centers[3ull * i], g.centers[3ull * i + 1], g.centers[3ull * i + 2]}};
-    const V3 ra = rot_apply(R, V3{{g.sa[i], 0.0, 0.0}});
-    const V3 rb = rot_apply(R, V3{{0.0, g.sb[i], 0.0}});
-    const V3 rh = rot_apply(R, V3{{0.0, 0.0, g.heights[i] / 2.0}});
-    V3 pts[6];
+    const R *M = &g.rot[9ull * i];
+    const V3T<R> c{{g.centers[3ull * i], g.centers[3ull * i + 1], g.centers[3ull * i + 2]}};
+    const V3T<R> ra = rot_apply(M, V3T<R>{{g.sa[i], R(0), R(0)}});
+    const V3T<R> rb = rot_apply(M, V3T<R>{{R(0), g.sb[i], R(0)}});
+    const V3T<R> rh = rot_apply(M, V3T<R>{{R(0), R(0), g.heights[i] / 2}});
+    V3T<R> pts[6];
     for (int k = 0; k < 3; k++) {
       pts[0][k] = c[k] + ra[k];  // a1
       pts[1][k] = c[k] + rb[k];  // b1
@@ -228,22 +263,23 @@ struct BuildCounters {
   }
 };
 
-struct Builder {
-  const Boxes &B;
+template <class R>
+struct BuilderT {
+  const BoxesT<R> &B;
   uint32_t *idx;
   uint32_t min_leaf, max_depth, nbins;
   BuildCounters cnt;
 
   struct BinAcc {
-    double lo[3], hi[3];
+    R lo[3], hi[3];
     uint32_t count;
-    double cost;
+    R cost;
   };
 
-  void bounds(uint32_t begin, uint32_t end, double *lo, double *hi) const {
+  void bounds(uint32_t begin, uint32_t end, R *lo, R *hi) const {
     for (int k = 0; k < 3; k++) {
-      lo[k] = DBL_MAX;
-      hi[k] = -DBL_MAX;
+      lo[k] = std::numeric_limits<R>::max();
+      hi[k] = -std::numeric_limits<R>::max();
     }
     for (uint32_t i = begin; i < end; i++) {
       const uint32_t p = idx[i];
@@ -254,29 +290,30 @@ struct Builder {
     }
   }
 
-  static double surface(const double *lo, const double *hi) {
-    const double b0 = std::fabs(hi[0] - lo[0]), b1 = std::fabs(hi[1] - lo[1]), b2 = std::fabs(hi[2] - lo[2]);
-    return 2.0 * (b0 * b1 + b1 * b2 + b2 * b0);
+  static R surface(const R *lo, const R *hi) {
+    const R b0 = std::fabs(hi[0] - lo[0]), b1 = std::fabs(hi[1] - lo[1]), b2 = std::fabs(hi[2] - lo[2]);
+    return R(2) * (b0 * b1 + b1 * b2 + b2 * b0);
   }
 
   // unsigned(std::round(x)) as the reference's x86-64 build evaluates it (64-bit truncation, low 32 bits)
-  static uint32_t bin_of(double x, uint32_t last) {
-    const double r = std::round(x);
+  static uint32_t bin_of(R x, uint32_t last) {
+    const R r = std::round(x);
     uint32_t u = 0;
-    if (r == r && r < 9.2e18 && r > -9.2e18) u = (uint32_t)(uint64_t)(int64_t)r;
+    if (r == r && (double)r < 9.2e18 && (double)r > -9.2e18) u = (uint32_t)(uint64_t)(int64_t)r;
     return u < last ? u : last;
   }
 
   // find_best_split_binned: returns the axis and writes the three cut positions
-  uint32_t best_split(uint32_t begin, uint32_t end, const double *lo, const double *hi, double *cut,
+  uint32_t best_split(uint32_t begin, uint32_t end, const R *lo, const R *hi, R *cut,
                       std::vector<BinAcc> &bins) const {
-    bins.assign(3ull * nbins, BinAcc{{DBL_MAX, DBL_MAX, DBL_MAX}, {-DBL_MAX, -DBL_MAX, -DBL_MAX}, 0u, 0.0});
-    double inv[3];
+    const R MX = std::numeric_limits<R>::max();
+    bins.assign(3ull * nbins, BinAcc{{MX, MX, MX}, {-MX, -MX, -MX}, 0u, R(0)});
+    R inv[3];
     for (int k = 0; k < 3; k++) {
-      const double size = hi[k] - lo[k];
-      inv[k] = (size > 0.0) ? 1.0 / size : 0.0;
+      const R size = hi[k] - lo[k];
+      inv[k] = (size > R(0)) ? R(1) / size : R(0);
     }
-    const double scale = (double)(nbins - 1);
+    const R scale = (R)(nbins - 1);
     for (uint32_t i = begin; i < end; i++) {
       const uint32_t p = idx[i];
       for (uint32_t j = 0; j < 3; j++) {
@@ -289,10 +326,10 @@ struct Builder {
         }
       }
     }
-    double min_cost[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    R min_cost[3] = {MX, MX, MX};
     for (uint32_t j = 0; j < 3; j++) {
       BinAcc *ax = &bins[j * nbins];
-      double alo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, ahi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+      R alo[3] = {MX, MX, MX}, ahi[3] = {-MX, -MX, -MX};
       uint64_t count = 0;
       for (uint32_t i = nbins - 1; i > 0; i--) {  // right-to-left sweep: cost of the right side
         for (int k = 0; k < 3; k++) {
@@ -300,11 +337,11 @@ struct Builder {
           ahi[k] = ref_max(ax[i].hi[k], ahi[k]);
         }
         count += ax[i].count;
-        ax[i].cost = (double)count * surface(alo, ahi);
+        ax[i].cost = (R)count * surface(alo, ahi);
       }
       for (int k = 0; k < 3; k++) {
-        alo[k] = DBL_MAX;
-        ahi[k] = -DBL_MAX;
+        alo[k] = MX;
+        ahi[k] = -MX;
       }
       count = 0;
       uint32_t min_bin = 1;
@@ -314,13 +351,13 @@ struct Builder {
           ahi[k] = ref_max(ax[i].hi[k], ahi[k]);
         }
         count += ax[i].count;
-        const double cost = (double)count * surface(alo, ahi) + ax[i + 1].cost;
+        const R cost = (R)count * surface(alo, ahi) + ax[i + 1].cost;
         if (cost < min_cost[j]) {
           min_cost[j] = cost;
           min_bin = i + 1;
         }
       }
-      cut[j] = (double)min_bin * ((hi[j] - lo[j]) / (double)nbins) + lo[j];
+      cut[j] = (R)min_bin * ((hi[j] - lo[j]) / (R)nbins) + lo[j];
     }
     uint32_t axis = 0;
     if (min_cost[0] > min_cost[1]) axis = 1;
@@ -340,11 +377,13 @@ struct Builder {
     out.emplace_back();
     cnt.atomic_max(cnt.max_depth, depth);
     const uint32_t n = end - begin;
-    double lo[3], hi[3];
+    R lo[3], hi[3];
     bounds(begin, end, lo, hi);
     brt_tree_node node{};
-    std::memcpy(node.min, lo, sizeof lo);
-    std::memcpy(node.max, hi, sizeof hi);
+    for (int k = 0; k < 3; k++) {  // a float box widens exactly
+      node.min[k] = lo[k];
+      node.max[k] = hi[k];
+    }
     node.children[0] = node.children[1] = INVALID_ID;
     if (n <= min_leaf || depth >= max_depth) {
       node.leaf = 1;
@@ -357,12 +396,12 @@ struct Builder {
       cnt.atomic_min(cnt.min_prims, n);
       return self;
     }
-    double cut[3];
+    R cut[3];
     uint32_t axis = best_split(begin, end, lo, hi, cut, bins);
     uint32_t mid = begin;
     for (int attempt = 0; attempt < 3; attempt++) {
-      const double c = cut[axis];
-      const double *ctr = B.ctr.data();
+      const R c = cut[axis];
+      const R *ctr = B.ctr.data();
       const uint32_t ax = axis;
       uint32_t *m = std::partition(idx + begin, idx + end,
                                    [ctr, ax, c](uint32_t p) { return ctr[3ull * p + ax] < c; });
@@ -599,7 +638,106 @@ unsigned hw_threads() {
   return h ? std::min(h, 64u) : 4u;
 }
 
-void build_geometry(Geometry &g, GeomType type, const brt_build_options &o) {
+// Scene<float>: the same children-in-parent layout with FP32 boxes, and FP32 records in leaf order.
+void flatten_f32(GeometryF &g, GeomType type) {
+  const auto &T = g.tree;
+  std::vector<uint32_t> internal(T.size(), INVALID_ID);
+  uint32_t ni = 0;
+  for (size_t i = 0; i < T.size(); i++)
+    if (!T[i].leaf) internal[i] = ni++;
+  g.n32.assign(ni, Node32{});
+  auto child_ref = [&](uint32_t c, uint32_t &ref, uint32_t &cnt) {
+    if (T[c].leaf) {
+      ref = LEAF_BIT | T[c].prim_begin;
+      cnt = T[c].prim_count;
+    } else {
+      ref = internal[c];
+      cnt = 0;
+    }
+  };
+  for (size_t i = 0; i < T.size(); i++) {
+    if (T[i].leaf) continue;
+    Node32 &a = g.n32[internal[i]];
+    for (int c = 0; c < 2; c++) {
+      const brt_tree_node &ch = T[T[i].children[c]];
+      for (int k = 0; k < 3; k++) {  // the tree holds FP32 values widened to FP64: narrowing is exact
+        a.b[6 * c + k] = (float)ch.min[k];
+        a.b[6 * c + 3 + k] = (float)ch.max[k];
+      }
+      child_ref(T[i].children[c], a.ref[c], a.cnt[c]);
+    }
+    a.cnt[1] |= T[i].axis << AXIS_SHIFT;
+  }
+  BvhView &v = g.view;
+  v = BvhView{};
+  bool nan_boxes = false;
+  for (const auto &x : T)
+    for (int k = 0; k < 3; k++) nan_boxes = nan_boxes || x.min[k] != x.min[k] || x.max[k] != x.max[k];
+  for (int k = 0; k < 3; k++) {
+    v.root_min[k] = T[0].min[k];
+    v.root_max[k] = T[0].max[k];
+  }
+  child_ref(0, v.root_ref, v.root_cnt);
+  v.root_ref4 = v.root_ref;
+  v.geom_id = g.geom_id;
+  v.n_prims = g.boxes.n;
+  v.n_nodes = ni;
+  v.depth = g.stats.max_tree_depth;
+  v.nan_boxes = nan_boxes ? 1u : 0u;
+  g.extent_ok = false;  // no split precision for FP32 scenes
+  const uint32_t n = g.boxes.n;
+  auto fill = [&](auto *r, auto &&one) {
+    for (uint32_t i = 0; i < n; i++) one(r[i], g.order[i]);
+  };
+  switch (type) {
+  case TRIANGLES:
+    g.rec_size = sizeof(TriRecF);
+    g.recs.assign((size_t)n * sizeof(TriRecF), 0);
+    fill(reinterpret_cast<TriRecF *>(g.recs.data()), [&](TriRecF &r, uint32_t p) {
+      const float *v0 = &g.vertices[3ull * g.faces[3ull * p]];
+      const float *v1 = &g.vertices[3ull * g.faces[3ull * p + 1]];
+      const float *v2 = &g.vertices[3ull * g.faces[3ull * p + 2]];
+      for (int k = 0; k < 3; k++) {
+        r.v0[k] = v0[k];
+        r.e1[k] = v0[k] - v1[k];  // Triangle::c = a - b
+        r.e2[k] = v2[k] - v0[k];  // Triangle::b = c - a
+      }
+    });
+    break;
+  case SPHERES:
+    g.rec_size = sizeof(SphRecF);
+    g.recs.assign((size_t)n * sizeof(SphRecF), 0);
+    fill(reinterpret_cast<SphRecF *>(g.recs.data()), [&](SphRecF &r, uint32_t p) {
+      for (int k = 0; k < 3; k++) r.c[k] = g.centers[3ull * p + k];
+      r.r = g.radii[p];
+    });
+    break;
+  case PLANES:
+    g.rec_size = sizeof(PlaRecF);
+    g.recs.assign((size_t)n * sizeof(PlaRecF), 0);
+    fill(reinterpret_cast<PlaRecF *>(g.recs.data()), [&](PlaRecF &r, uint32_t p) {
+      for (int k = 0; k < 3; k++) r.c[k] = g.centers[3ull * p + k];
+      r.dx = g.dxs[p];
+      r.dy = g.dys[p];
+      for (int k = 0; k < 9; k++) r.R[k] = g.rot[9ull * p + k];
+    });
+    break;
+  default:
+    g.rec_size = sizeof(CylRecF);
+    g.recs.assign((size_t)n * sizeof(CylRecF), 0);
+    fill(reinterpret_cast<CylRecF *>(g.recs.data()), [&](CylRecF &r, uint32_t p) {
+      for (int k = 0; k < 3; k++) r.c[k] = g.centers[3ull * p + k];
+      r.a = g.sa[p];
+      r.b = g.sb[p];
+      r.h = g.heights[p];
+      for (int k = 0; k < 9; k++) r.R[k] = g.rot[9ull * p + k];
+    });
+    break;
+  }
+}
+
+template <class R>
+void build_geometry(GeometryT<R> &g, GeomType type, const brt_build_options &o) {
   const auto t0 = std::chrono::steady_clock::now();
   g.built_on_device = false;
   g.dtree = nullptr;
@@ -607,7 +745,7 @@ void build_geometry(Geometry &g, GeomType type, const brt_build_options &o) {
   const uint32_t n = g.boxes.n;
   g.order.resize(n);
   std::iota(g.order.begin(), g.order.end(), 0u);
-  Builder b{g.boxes, g.order.data(), o.min_leaf_primitives, o.max_tree_depth, o.bin_size};
+  BuilderT<R> b{g.boxes, g.order.data(), o.min_leaf_primitives, o.max_tree_depth, o.bin_size};
   int par = 0;
   for (unsigned t = hw_threads(); t > 1; t >>= 1) par++;
   g.tree.clear();
@@ -624,10 +762,11 @@ void build_geometry(Geometry &g, GeomType type, const brt_build_options &o) {
   s.num_primitives = n;
   s.geom_id = g.geom_id;
   s.built_on_device = 0;
-  flatten(g, type);
+  if constexpr (std::is_same<R, double>::value) flatten(g, type);
+  else flatten_f32(g, type);
   s.build_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  s.device_bytes = g.n64.size() * sizeof(Node64) + g.n4.size() * sizeof(Node4) + g.recs.size() +
-                   g.order.size() * 4 + g.faces.size() * 4 + g.vnormals.size() * 8 + g.vertices.size() * 8;
+  s.device_bytes = g.n64.size() * sizeof(Node64) + g.n4.size() * sizeof(Node4) + g.n32.size() * sizeof(Node32) +
+                   g.recs.size() + g.order.size() * 4 + g.faces.size() * 4 + (g.vnormals.size() + g.vertices.size()) * sizeof(R);
 }
 
 // ------------------------------------------------------------------------------------------------ device
@@ -694,27 +833,47 @@ struct brt_scene {
   brt_build_options opts{4, 28, 64};
   bool committed = false;
   bool device_commit = false;  // committed with brt_scene_commit_device (rebuilds build on the device too)
+  bool f32 = false;            // blazert::Scene<float>: geometry in geoF, FP32 throughout
   uint32_t geometries = 0;
   Geometry geo[NUM_TYPES];
+  GeometryF geoF[NUM_TYPES];
   std::unique_ptr<DeviceScene> dev;
   std::mutex upload_mu;
 };
 
 namespace {
 
+template <class R>
+GeometryT<R> *geo_of(brt_scene *s) {
+  if constexpr (std::is_same<R, double>::value) return s->geo;
+  else return s->geoF;
+}
+
+template <class R>
 int add_common(brt_scene *s, uint32_t type, uint32_t *geom_id) {
   if (geom_id) *geom_id = BRT_INVALID_ID;
   if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (s->f32 != std::is_same<R, float>::value)
+    return fail(BRT_ERR_INVALID_ARGUMENT, s->f32 ? "Scene<float>: use the _f32 add functions"
+                                                 : "Scene<double>: the _f32 add functions need brt_scene_create_f32");
   if (s->committed) return fail(BRT_ERR_STATE, "geometry added after commit");
-  if (s->geo[type].present) return fail(BRT_ERR_STATE, "geometry type already added (one add per type)");
+  if (geo_of<R>(s)[type].present) return fail(BRT_ERR_STATE, "geometry type already added (one add per type)");
   return BRT_OK;
 }
 
+template <class R>
 void finish_add(brt_scene *s, uint32_t type, uint32_t *geom_id) {
-  Geometry &g = s->geo[type];
+  GeometryT<R> &g = geo_of<R>(s)[type];
   g.present = true;
   g.geom_id = s->geometries++;
   if (geom_id) *geom_id = g.geom_id;
+}
+
+int double_only(const brt_scene *s, const char *what) {
+  if (s && s->f32)
+    return fail(BRT_ERR_UNSUPPORTED, std::string(what) + " is not available for Scene<float> scenes "
+                                     "(add, commit, intersect and intersect1 are)");
+  return BRT_OK;
 }
 
 hipError_t new_device_scene(int dev, std::unique_ptr<DeviceScene> &out) {
@@ -754,6 +913,24 @@ int put_geometry(DeviceScene &d, Geometry &g, uint32_t t) {
   return BRT_OK;
 }
 
+// Copies one host-built Scene<float> geometry to the device.
+int put_geometry_f32(DeviceScene &d, GeometryF &g, uint32_t t) {
+  if (g.stats.max_tree_depth >= 64) return fail(BRT_ERR_UNSUPPORTED, "tree deeper than the 64-entry device stack");
+  BvhView v = g.view;
+  hipError_t e = d.put(g.n32, &v.nodes32);
+  const unsigned char *recs = nullptr;
+  if (e == hipSuccess) e = d.put(g.recs, &recs);
+  v.prims = recs;
+  if (e == hipSuccess) e = d.put(g.order, &v.prim_ids);
+  if (t == TRIANGLES) {
+    if (e == hipSuccess) e = d.put(g.faces, &v.faces);
+    if (e == hipSuccess) e = d.put(g.vnormals, &v.vnormals32);
+  }
+  if (e != hipSuccess) return device_error(e, "upload failed");
+  d.views[t] = v;
+  return BRT_OK;
+}
+
 int upload(brt_scene *s) {
   std::lock_guard<std::mutex> lk(s->upload_mu);
   int dev = 0;
@@ -765,6 +942,12 @@ int upload(brt_scene *s) {
   std::unique_ptr<DeviceScene> d;
   if (hipError_t e = new_device_scene(dev, d)) return device_error(e, "counter allocation");
   for (uint32_t t = 0; t < NUM_TYPES; t++) {
+    if (s->f32) {
+      GeometryF &g = s->geoF[t];
+      if (!g.present || g.boxes.n == 0) continue;
+      if (int rc = put_geometry_f32(*d, g, t)) return rc;
+      continue;
+    }
     Geometry &g = s->geo[t];
     if (!g.present || g.boxes.n == 0) continue;
     if (int rc = put_geometry(*d, g, t)) return rc;
@@ -904,6 +1087,7 @@ int download_tree(brt_scene *s, Geometry &g, uint32_t t) {
 
 int mirror_check(brt_scene *s, uint64_t n, uint32_t bounces) {
   if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (s->f32) return fail(BRT_ERR_UNSUPPORTED, "mirror tracing is not available for Scene<float> scenes");
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
   if (bounces < 1 || bounces > 8) return fail(BRT_ERR_INVALID_ARGUMENT, "bounces must be 1..8");
   const Geometry &tri = s->geo[TRIANGLES];
@@ -914,6 +1098,132 @@ int mirror_check(brt_scene *s, uint64_t n, uint32_t bounces) {
   return upload(s);
 }
 
+
+
+template <class R>
+int add_mesh_t(brt_scene *s, const R *vertices, uint32_t nv, const uint32_t *faces, uint32_t nf, uint32_t *geom_id) {
+  int rc = add_common<R>(s, TRIANGLES, geom_id);
+  if (rc) return rc;
+  if ((nv && !vertices) || (nf && !faces)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL vertex or face array");
+  for (uint64_t i = 0; i < 3ull * nf; i++)
+    if (faces[i] >= nv) return fail(BRT_ERR_INVALID_ARGUMENT, "face index out of range");
+  if (nf >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many triangles");
+  GeometryT<R> &g = geo_of<R>(s)[TRIANGLES];
+  g.vertices.assign(vertices, vertices + 3ull * nv);
+  g.faces.assign(faces, faces + 3ull * nf);
+  mesh_precompute(g, nv);
+  g.boxes.classify();
+  finish_add<R>(s, TRIANGLES, geom_id);
+  return BRT_OK;
+}
+
+template <class R>
+int add_spheres_t(brt_scene *s, const R *centers, const R *radii, uint32_t n, uint32_t *geom_id) {
+  int rc = add_common<R>(s, SPHERES, geom_id);
+  if (rc) return rc;
+  if (n && (!centers || !radii)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL sphere array");
+  if (n >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many spheres");
+  GeometryT<R> &g = geo_of<R>(s)[SPHERES];
+  g.centers.assign(centers, centers + 3ull * n);
+  g.radii.assign(radii, radii + n);
+  sphere_precompute(g);
+  g.boxes.classify();
+  finish_add<R>(s, SPHERES, geom_id);
+  return BRT_OK;
+}
+
+template <class R>
+int add_planes_t(brt_scene *s, const R *centers, const R *dxs, const R *dys, const R *rotations, uint32_t n,
+                 uint32_t *geom_id) {
+  int rc = add_common<R>(s, PLANES, geom_id);
+  if (rc) return rc;
+  if (n && (!centers || !dxs || !dys || !rotations)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL plane array");
+  if (n >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many planes");
+  GeometryT<R> &g = geo_of<R>(s)[PLANES];
+  g.centers.assign(centers, centers + 3ull * n);
+  g.dxs.assign(dxs, dxs + n);
+  g.dys.assign(dys, dys + n);
+  g.rot.assign(rotations, rotations + 9ull * n);
+  plane_precompute(g);
+  g.boxes.classify();
+  finish_add<R>(s, PLANES, geom_id);
+  return BRT_OK;
+}
+
+template <class R>
+int add_cylinders_t(brt_scene *s, const R *centers, const R *a, const R *b, const R *heights, const R *rotations,
+                    uint32_t n, uint32_t *geom_id) {
+  int rc = add_common<R>(s, CYLINDERS, geom_id);
+  if (rc) return rc;
+  if (n && (!centers || !a || !b || !heights || !rotations))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "NULL cylinder array");
+  if (n >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many cylinders");
+  GeometryT<R> &g = geo_of<R>(s)[CYLINDERS];
+  g.centers.assign(centers, centers + 3ull * n);
+  g.sa.assign(a, a + n);
+  g.sb.assign(b, b + n);
+  g.heights.assign(heights, heights + n);
+  g.rot.assign(rotations, rotations + 9ull * n);
+  cylinder_precompute(g);
+  g.boxes.classify();
+  finish_add<R>(s, CYLINDERS, geom_id);
+  return BRT_OK;
+}
+
+
+template <class R, class H>
+int intersect1_t(const brt_scene *s, const R org[3], const R dir[3], R tmin, R tmax, uint32_t flags, H *hit,
+                 int *hit_out) {
+  if (!s || !org || !dir || !hit) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  // one ray through the batched kernels, keeping RayHit's in/out semantics: 8 ray values, 6 hit values,
+  // then prim id, geom id and the merged flag
+  R host[14] = {org[0], org[1], org[2], dir[0], dir[1], dir[2], tmin, tmax,
+                     hit->t, hit->u, hit->v, hit->normal[0], hit->normal[1], hit->normal[2]};
+  uint32_t ids[3] = {hit->prim_id, hit->geom_id, 0u};
+  char *d = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&d), sizeof host + sizeof ids);
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+  R *dd = reinterpret_cast<R *>(d);
+  uint32_t *di = reinterpret_cast<uint32_t *>(d + sizeof host);
+  e = hipMemcpy(d, host, sizeof host, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(di, ids, sizeof ids, hipMemcpyHostToDevice);
+  int rc = BRT_OK;
+  brt_scene *ms = const_cast<brt_scene *>(s);
+  if (e == hipSuccess && !ms->committed) rc = fail(BRT_ERR_STATE, "scene not committed");
+  if (e == hipSuccess && rc == BRT_OK) rc = upload(ms);
+  if (e == hipSuccess && rc == BRT_OK) {
+    RaysViewT<R> rv{dd, dd + 1, dd + 2, dd + 3, dd + 4, dd + 5, dd + 6, dd + 7};
+    HitsViewT<R> hv{dd + 8, dd + 9, dd + 10, di, di + 1, dd + 11, dd + 12, dd + 13, di + 2};
+    for (uint32_t t = 0; t < NUM_TYPES && e == hipSuccess; t++) {
+      const GeometryT<R> &g = geo_of<R>(ms)[t];
+      if (!g.present || g.boxes.n == 0) continue;
+      TraceLaunch cfg{BRT_PRECISION_FP64, flags | BRT_FLAG_HITS_INOUT, 64, false, SCHEDULE_PER_THREAD, 0, nullptr,
+                      nullptr};
+      if constexpr (std::is_same<R, float>::value)
+        e = launch_trace_f32((GeomType)t, ms->dev->views[t], rv, hv, 1, cfg, nullptr);
+      else
+        e = launch_trace((GeomType)t, ms->dev->views[t], rv, hv, 1, cfg, SplitParams{g.extent}, nullptr);
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(host, d, sizeof host, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ids, di, sizeof ids, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+      hit->t = host[8];
+      hit->u = host[9];
+      hit->v = host[10];
+      hit->normal[0] = host[11];
+      hit->normal[1] = host[12];
+      hit->normal[2] = host[13];
+      hit->prim_id = ids[0];
+      hit->geom_id = ids[1];
+      if (hit_out) *hit_out = ids[2] ? 1 : 0;
+    }
+  }
+  (void)hipFree(d);
+  if (rc != BRT_OK) return rc;
+  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("intersect1: ") + hipGetErrorString(e));
+  return BRT_OK;
+}
 
 }  // namespace
 
@@ -951,82 +1261,55 @@ int brt_scene_destroy(brt_scene *scene) {
 
 int brt_scene_add_mesh(brt_scene *s, const double *vertices, uint32_t nv, const uint32_t *faces, uint32_t nf,
                        uint32_t *geom_id) {
-  int rc = add_common(s, TRIANGLES, geom_id);
-  if (rc) return rc;
-  if ((nv && !vertices) || (nf && !faces)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL vertex or face array");
-  for (uint64_t i = 0; i < 3ull * nf; i++)
-    if (faces[i] >= nv) return fail(BRT_ERR_INVALID_ARGUMENT, "face index out of range");
-  if (nf >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many triangles");
-  Geometry &g = s->geo[TRIANGLES];
-  g.vertices.assign(vertices, vertices + 3ull * nv);
-  g.faces.assign(faces, faces + 3ull * nf);
-  mesh_precompute(g, nv);
-  g.boxes.classify();
-  finish_add(s, TRIANGLES, geom_id);
-  return BRT_OK;
+  return add_mesh_t(s, vertices, nv, faces, nf, geom_id);
 }
-
 int brt_scene_add_spheres(brt_scene *s, const double *centers, const double *radii, uint32_t n, uint32_t *geom_id) {
-  int rc = add_common(s, SPHERES, geom_id);
-  if (rc) return rc;
-  if (n && (!centers || !radii)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL sphere array");
-  if (n >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many spheres");
-  Geometry &g = s->geo[SPHERES];
-  g.centers.assign(centers, centers + 3ull * n);
-  g.radii.assign(radii, radii + n);
-  sphere_precompute(g);
-  g.boxes.classify();
-  finish_add(s, SPHERES, geom_id);
-  return BRT_OK;
+  return add_spheres_t(s, centers, radii, n, geom_id);
 }
-
 int brt_scene_add_planes(brt_scene *s, const double *centers, const double *dxs, const double *dys,
                          const double *rotations, uint32_t n, uint32_t *geom_id) {
-  int rc = add_common(s, PLANES, geom_id);
-  if (rc) return rc;
-  if (n && (!centers || !dxs || !dys || !rotations)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL plane array");
-  if (n >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many planes");
-  Geometry &g = s->geo[PLANES];
-  g.centers.assign(centers, centers + 3ull * n);
-  g.dxs.assign(dxs, dxs + n);
-  g.dys.assign(dys, dys + n);
-  g.rot.assign(rotations, rotations + 9ull * n);
-  plane_precompute(g);
-  g.boxes.classify();
-  finish_add(s, PLANES, geom_id);
-  return BRT_OK;
+  return add_planes_t(s, centers, dxs, dys, rotations, n, geom_id);
 }
-
 int brt_scene_add_cylinders(brt_scene *s, const double *centers, const double *a, const double *b,
                             const double *heights, const double *rotations, uint32_t n, uint32_t *geom_id) {
-  int rc = add_common(s, CYLINDERS, geom_id);
-  if (rc) return rc;
-  if (n && (!centers || !a || !b || !heights || !rotations))
-    return fail(BRT_ERR_INVALID_ARGUMENT, "NULL cylinder array");
-  if (n >= (1u << 30)) return fail(BRT_ERR_UNSUPPORTED, "too many cylinders");
-  Geometry &g = s->geo[CYLINDERS];
-  g.centers.assign(centers, centers + 3ull * n);
-  g.sa.assign(a, a + n);
-  g.sb.assign(b, b + n);
-  g.heights.assign(heights, heights + n);
-  g.rot.assign(rotations, rotations + 9ull * n);
-  cylinder_precompute(g);
-  g.boxes.classify();
-  finish_add(s, CYLINDERS, geom_id);
-  return BRT_OK;
+  return add_cylinders_t(s, centers, a, b, heights, rotations, n, geom_id);
+}
+
+int brt_scene_create_f32(const brt_build_options *opts, brt_scene **out) {
+  const int rc = brt_scene_create(opts, out);
+  if (rc == BRT_OK) (*out)->f32 = true;
+  return rc;
+}
+int brt_scene_add_mesh_f32(brt_scene *s, const float *vertices, uint32_t nv, const uint32_t *faces, uint32_t nf,
+                           uint32_t *geom_id) {
+  return add_mesh_t(s, vertices, nv, faces, nf, geom_id);
+}
+int brt_scene_add_spheres_f32(brt_scene *s, const float *centers, const float *radii, uint32_t n, uint32_t *geom_id) {
+  return add_spheres_t(s, centers, radii, n, geom_id);
+}
+int brt_scene_add_planes_f32(brt_scene *s, const float *centers, const float *dxs, const float *dys,
+                             const float *rotations, uint32_t n, uint32_t *geom_id) {
+  return add_planes_t(s, centers, dxs, dys, rotations, n, geom_id);
+}
+int brt_scene_add_cylinders_f32(brt_scene *s, const float *centers, const float *a, const float *b,
+                                const float *heights, const float *rotations, uint32_t n, uint32_t *geom_id) {
+  return add_cylinders_t(s, centers, a, b, heights, rotations, n, geom_id);
 }
 
 int brt_scene_commit(brt_scene *s) {
   if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
   if (s->committed) return BRT_OK;
-  for (uint32_t t = 0; t < NUM_TYPES; t++)
-    if (s->geo[t].present) build_geometry(s->geo[t], (GeomType)t, s->opts);
+  for (uint32_t t = 0; t < NUM_TYPES; t++) {
+    if (s->f32 && s->geoF[t].present) build_geometry(s->geoF[t], (GeomType)t, s->opts);
+    if (!s->f32 && s->geo[t].present) build_geometry(s->geo[t], (GeomType)t, s->opts);
+  }
   s->committed = true;
   return BRT_OK;
 }
 
 int brt_scene_commit_device(brt_scene *s, void *stream) {
   if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (int rc = double_only(s, "brt_scene_commit_device")) return rc;
   if (s->committed) return BRT_OK;
   if (int rc = commit_on_device(s, static_cast<hipStream_t>(stream))) return rc;
   s->committed = true;
@@ -1037,6 +1320,7 @@ int brt_scene_commit_device(brt_scene *s, void *stream) {
 int brt_scene_read_device_array(const brt_scene *cs, uint32_t type, uint32_t which, void *dst, uint64_t capacity,
                                 uint64_t *bytes) {
   brt_scene *s = const_cast<brt_scene *>(cs);
+  if (int rc = double_only(s, "brt_scene_read_device_array")) return rc;
   if (!s || type >= NUM_TYPES || !bytes || which > BRT_ARRAY_PRIM_IDS)
     return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
@@ -1064,6 +1348,7 @@ int brt_scene_read_device_array(const brt_scene *cs, uint32_t type, uint32_t whi
 int brt_distance_to_surface(const brt_scene *cs, uint32_t type, const brt_points *points, const uint32_t *prim_ids,
                             uint64_t n, double *out, void *stream) {
   brt_scene *s = const_cast<brt_scene *>(cs);
+  if (int rc = double_only(s, "brt_distance_to_surface")) return rc;
   if (!s || !points || type >= NUM_TYPES) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
   if (!s->geo[type].present) return fail(BRT_ERR_INVALID_ARGUMENT, "the scene holds no geometry of this type");
@@ -1098,6 +1383,7 @@ int brt_distance_to_surface(const brt_scene *cs, uint32_t type, const brt_points
 int brt_closest_surface(const brt_scene *cs, const brt_points *points, uint64_t n, double *distance,
                         uint32_t *prim_id, uint32_t *geom_id, void *stream) {
   brt_scene *s = const_cast<brt_scene *>(cs);
+  if (int rc = double_only(s, "brt_closest_surface")) return rc;
   if (!s || !points) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
   if (n == 0) return BRT_OK;
@@ -1127,6 +1413,11 @@ int brt_scene_upload(brt_scene *s) {
 int brt_scene_build_stats(const brt_scene *s, uint32_t type, brt_build_stats *stats) {
   if (!s || !stats || type >= NUM_TYPES) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  if (s->f32) {
+    if (!s->geoF[type].present) return fail(BRT_ERR_INVALID_ARGUMENT, "geometry type not in scene");
+    *stats = s->geoF[type].stats;
+    return BRT_OK;
+  }
   if (!s->geo[type].present) return fail(BRT_ERR_INVALID_ARGUMENT, "geometry type not in scene");
   *stats = s->geo[type].stats;
   return BRT_OK;
@@ -1136,6 +1427,17 @@ int brt_scene_export_tree(const brt_scene *s, uint32_t type, brt_tree_node *node
                           uint32_t *prim_order, uint32_t *num_nodes) {
   if (!s || type >= NUM_TYPES || !num_nodes) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  if (s->f32) {
+    const GeometryF &g = s->geoF[type];
+    if (!g.present) return fail(BRT_ERR_INVALID_ARGUMENT, "geometry type not in scene");
+    *num_nodes = (uint32_t)g.tree.size();
+    if (nodes) {
+      if (capacity < g.tree.size()) return fail(BRT_ERR_INVALID_ARGUMENT, "capacity too small");
+      std::memcpy(nodes, g.tree.data(), g.tree.size() * sizeof(brt_tree_node));
+    }
+    if (prim_order && !g.order.empty()) std::memcpy(prim_order, g.order.data(), g.order.size() * 4);
+    return BRT_OK;
+  }
   Geometry &g = const_cast<brt_scene *>(s)->geo[type];
   if (!g.present) return fail(BRT_ERR_INVALID_ARGUMENT, "geometry type not in scene");
   if (int rc = download_tree(const_cast<brt_scene *>(s), g, type)) return rc;
@@ -1152,6 +1454,7 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
                   const brt_launch *launch, void *stream) {
   brt_scene *s = const_cast<brt_scene *>(cs);
   if (!s || !rays || !hits) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (int rc = double_only(s, "brt_intersect (use brt_intersect_f32)")) return rc;
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
   if (n == 0) return BRT_OK;
   if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
@@ -1214,6 +1517,7 @@ int brt_traverse(const brt_scene *cs, uint32_t geom_type, const brt_rays *rays, 
                  const brt_launch *launch, void *stream) {
   brt_scene *s = const_cast<brt_scene *>(cs);
   if (!s || !rays || !hits) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (int rc = double_only(s, "brt_traverse")) return rc;
   if (geom_type >= NUM_TYPES) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown geometry type");
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
   if (!s->geo[geom_type].present) return fail(BRT_ERR_INVALID_ARGUMENT, "the scene holds no geometry of this type");
@@ -1251,51 +1555,56 @@ int brt_traverse(const brt_scene *cs, uint32_t geom_type, const brt_rays *rays, 
 
 int brt_intersect1(const brt_scene *s, const double org[3], const double dir[3], double tmin, double tmax,
                    uint32_t flags, brt_hit1 *hit, int *hit_out) {
-  if (!s || !org || !dir || !hit) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
-  // one ray through the batched kernels, keeping RayHit's in/out semantics: 8 ray doubles, 6 hit doubles,
-  // then prim id, geom id and the merged flag
-  double host[14] = {org[0], org[1], org[2], dir[0], dir[1], dir[2], tmin, tmax,
-                     hit->t, hit->u, hit->v, hit->normal[0], hit->normal[1], hit->normal[2]};
-  uint32_t ids[3] = {hit->prim_id, hit->geom_id, 0u};
-  char *d = nullptr;
-  hipError_t e = hipMalloc(reinterpret_cast<void **>(&d), sizeof host + sizeof ids);
-  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
-  double *dd = reinterpret_cast<double *>(d);
-  uint32_t *di = reinterpret_cast<uint32_t *>(d + sizeof host);
-  e = hipMemcpy(d, host, sizeof host, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(di, ids, sizeof ids, hipMemcpyHostToDevice);
-  int rc = BRT_OK;
-  brt_scene *ms = const_cast<brt_scene *>(s);
-  if (e == hipSuccess && !ms->committed) rc = fail(BRT_ERR_STATE, "scene not committed");
-  if (e == hipSuccess && rc == BRT_OK) rc = upload(ms);
-  if (e == hipSuccess && rc == BRT_OK) {
-    RaysView rv{dd, dd + 1, dd + 2, dd + 3, dd + 4, dd + 5, dd + 6, dd + 7};
-    HitsView hv{dd + 8, dd + 9, dd + 10, di, di + 1, dd + 11, dd + 12, dd + 13, di + 2};
-    for (uint32_t t = 0; t < NUM_TYPES && e == hipSuccess; t++) {
-      const Geometry &g = ms->geo[t];
-      if (!g.present || g.boxes.n == 0) continue;
-      TraceLaunch cfg{BRT_PRECISION_FP64, flags | BRT_FLAG_HITS_INOUT, 64, false, SCHEDULE_PER_THREAD, 0, nullptr,
-                      nullptr};
-      e = launch_trace((GeomType)t, ms->dev->views[t], rv, hv, 1, cfg, SplitParams{g.extent}, nullptr);
-    }
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(host, d, sizeof host, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(ids, di, sizeof ids, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) {
-      hit->t = host[8];
-      hit->u = host[9];
-      hit->v = host[10];
-      hit->normal[0] = host[11];
-      hit->normal[1] = host[12];
-      hit->normal[2] = host[13];
-      hit->prim_id = ids[0];
-      hit->geom_id = ids[1];
-      if (hit_out) *hit_out = ids[2] ? 1 : 0;
-    }
+  if (int rc = double_only(s, "brt_intersect1 (use brt_intersect1_f32)")) return rc;
+  return intersect1_t(s, org, dir, tmin, tmax, flags, hit, hit_out);
+}
+
+int brt_intersect1_f32(const brt_scene *s, const float org[3], const float dir[3], float tmin, float tmax,
+                       uint32_t flags, brt_hit1_f32 *hit, int *hit_out) {
+  if (s && !s->f32) return fail(BRT_ERR_INVALID_ARGUMENT, "brt_intersect1_f32 needs a Scene<float>");
+  return intersect1_t(s, org, dir, tmin, tmax, flags, hit, hit_out);
+}
+
+int brt_intersect_f32(const brt_scene *cs, const brt_rays_f32 *rays, const brt_hits_f32 *hits, uint64_t n,
+                      const brt_launch *launch, void *stream) {
+  brt_scene *s = const_cast<brt_scene *>(cs);
+  if (!s || !rays || !hits) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!s->f32) return fail(BRT_ERR_INVALID_ARGUMENT, "brt_intersect_f32 needs a Scene<float>");
+  if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
+  if (n == 0) return BRT_OK;
+  if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "ray origin / direction arrays are required");
+  if (!hits->t || !hits->prim_id || !hits->geom_id)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "hit t / prim_id / geom_id arrays are required");
+  if ((hits->nx != nullptr) != (hits->ny != nullptr) || (hits->nx != nullptr) != (hits->nz != nullptr))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "normal arrays must be all set or all NULL");
+  brt_launch L{};
+  L.schedule = BRT_SCHEDULE_PERSISTENT;
+  if (launch) L = *launch;
+  if (int rc = check_launch(L)) return rc;
+  if (L.precision != BRT_PRECISION_FP64)
+    return fail(BRT_ERR_UNSUPPORTED, "a Scene<float> is traversed in FP32: precision must be BRT_PRECISION_FP64 (0)");
+  if (L.sort != BRT_SORT_NONE) return fail(BRT_ERR_UNSUPPORTED, "Scene<float> launches keep the input ray order");
+  if (int rc = upload(s)) return rc;
+  RaysViewF rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+  HitsViewF hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz, nullptr};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  bool first = true;
+  for (uint32_t t = 0; t < NUM_TYPES; t++) {
+    const GeometryF &g = s->geoF[t];
+    if (!g.present || g.boxes.n == 0) continue;
+    TraceLaunch cfg{L.precision, L.flags, L.block_size, first, L.schedule, L.refill, nullptr, nullptr};
+    cfg.ring = L.stack_ring;
+    if (L.schedule == BRT_SCHEDULE_PERSISTENT)
+      cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
+    hipError_t e = launch_trace_f32((GeomType)t, s->dev->views[t], rv, hv, n, cfg, st);
+    if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+    first = false;
   }
-  (void)hipFree(d);
-  if (rc != BRT_OK) return rc;
-  if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("intersect1: ") + hipGetErrorString(e));
+  if (first && !(L.flags & BRT_FLAG_HITS_INOUT)) {
+    hipError_t e = launch_init_hits_f32(hv, n, st);
+    if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
+  }
   return BRT_OK;
 }
 
@@ -1363,6 +1672,7 @@ int brt_batch_create(const brt_batch_item *items, uint32_t count, const brt_laun
     const brt_batch_item &it = items[i];
     brt_scene *s = const_cast<brt_scene *>(it.scene);
     if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "item without a scene");
+    if (int rc = double_only(s, "brt_batch_create")) return rc;
     if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
     if (it.n && (!it.rays.ox || !it.rays.oy || !it.rays.oz || !it.rays.dx || !it.rays.dy || !it.rays.dz ||
                  !it.hits.t || !it.hits.prim_id || !it.hits.geom_id))
@@ -1531,6 +1841,7 @@ namespace brt {
 
 int scene_rebuild(brt_scene *s, const brt_build_options &opts) {
   if (!s) return fail(BRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+  if (int rc = double_only(s, "leaf-size tuning")) return rc;
   if (opts.min_leaf_primitives == 0 || opts.bin_size == 0)
     return fail(BRT_ERR_INVALID_ARGUMENT, "leaf size and bin count must be positive");
   {

@@ -27,18 +27,19 @@ def _up(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_uint32))
 
 
-def _f64(a, cols=None):
-    a = np.ascontiguousarray(a, dtype=np.float64)
-    return a.reshape(-1, cols) if cols else a.reshape(-1)
-
-
 class Scene:
-    """blazert::Scene<double> (blazert/blazert.h:17-20) on the MI355X."""
+    """blazert::Scene<T> (blazert/blazert.h:17-20) on the MI355X: T = double by default, dtype=np.float32 for
+    Scene<float> (FP32 geometry, build, traversal and hits; add / commit / intersect / intersect1 only)."""
 
-    def __init__(self, min_leaf_primitives: int = 4, max_tree_depth: int = 28, bin_size: int = 64):
+    def __init__(self, min_leaf_primitives: int = 4, max_tree_depth: int = 28, bin_size: int = 64,
+                 dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise ValueError("Scene dtype must be float64 or float32")
+        self.f32 = self.dtype == np.float32
         opts = capi.BuildOptions(min_leaf_primitives, max_tree_depth, bin_size)
         h = C.c_void_p()
-        check(lib().brt_scene_create(C.byref(opts), C.byref(h)))
+        check((lib().brt_scene_create_f32 if self.f32 else lib().brt_scene_create)(C.byref(opts), C.byref(h)))
         self._h = h
         self.has_been_committed = False
         self.num_vertices = 0
@@ -67,36 +68,49 @@ class Scene:
         check(rc)
         return gid.value
 
+    def _a(self, a, cols=None):
+        """Contiguous array of the scene type and its ctypes pointer."""
+        x = np.ascontiguousarray(a, dtype=self.dtype)
+        x = x.reshape(-1, cols) if cols else x.reshape(-1)
+        return x
+
+    def _p(self, x):
+        return x.ctypes.data_as(C.POINTER(C.c_float if self.f32 else C.c_double))
+
+    def _fn(self, name):
+        return getattr(lib(), name + ("_f32" if self.f32 else ""))
+
     def add_mesh(self, vertices, triangles) -> int:
-        v = _f64(vertices, 3)
+        v = self._a(vertices, 3)
         f = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1, 3)
-        gid = self._add(lib().brt_scene_add_mesh, _dp(v), len(v), _up(f), len(f))
+        gid = self._add(self._fn("brt_scene_add_mesh"), self._p(v), len(v), _up(f), len(f))
         if gid != INVALID_ID:
             self.num_vertices = len(v)
         return gid
 
     def add_spheres(self, centers, radii) -> int:
-        c = _f64(centers, 3)
-        r = _f64(radii)
+        c = self._a(centers, 3)
+        r = self._a(radii)
         if len(c) != len(r):
             raise ValueError("centers and radii must have the same length")
-        return self._add(lib().brt_scene_add_spheres, _dp(c), _dp(r), len(r))
+        return self._add(self._fn("brt_scene_add_spheres"), self._p(c), self._p(r), len(r))
 
     def add_planes(self, centers, dxs, dys, rotations) -> int:
-        c = _f64(centers, 3)
-        dx, dy = _f64(dxs), _f64(dys)
-        R = _f64(rotations, 9)
+        c = self._a(centers, 3)
+        dx, dy = self._a(dxs), self._a(dys)
+        R = self._a(rotations, 9)
         if not (len(c) == len(dx) == len(dy) == len(R)):
             raise ValueError("plane arrays must have the same length")
-        return self._add(lib().brt_scene_add_planes, _dp(c), _dp(dx), _dp(dy), _dp(R), len(dx))
+        return self._add(self._fn("brt_scene_add_planes"), self._p(c), self._p(dx), self._p(dy), self._p(R), len(dx))
 
     def add_cylinders(self, centers, semi_axes_a, semi_axes_b, heights, rotations) -> int:
-        c = _f64(centers, 3)
-        a, b, h = _f64(semi_axes_a), _f64(semi_axes_b), _f64(heights)
-        R = _f64(rotations, 9)
+        c = self._a(centers, 3)
+        a, b, h = self._a(semi_axes_a), self._a(semi_axes_b), self._a(heights)
+        R = self._a(rotations, 9)
         if not (len(c) == len(a) == len(b) == len(h) == len(R)):
             raise ValueError("cylinder arrays must have the same length")
-        return self._add(lib().brt_scene_add_cylinders, _dp(c), _dp(a), _dp(b), _dp(h), _dp(R), len(a))
+        return self._add(self._fn("brt_scene_add_cylinders"), self._p(c), self._p(a), self._p(b), self._p(h),
+                         self._p(R), len(a))
 
     def commit(self, device: bool = False, stream=None) -> bool:
         """Builds the BVHs (blazert/scene.h:91-115). device=True builds them on the current GPU
@@ -146,27 +160,29 @@ class Scene:
                   any_hit: bool = False, normals: bool = False, hits: dict | None = None, block_size: int = 0,
                   schedule: int = capi.SCHEDULE_PERSISTENT, refill: int = 0, sort: int = capi.SORT_NONE,
                   stack_ring: int = 0, stream=None) -> dict:
-        """Batched intersect1 over n rays. org / dirs: (n,3) float64 CUDA tensors or tuples of three columns.
+        """Batched intersect1 over n rays. org / dirs: (n,3) CUDA tensors or tuples of three columns, in the
+        scene type (float64, or float32 for a Scene<float>).
 
-        Returns a dict of CUDA tensors t, u, v (float64), prim_id, geom_id (int32 holding uint32 bits) and,
+        Returns a dict of CUDA tensors t, u, v (scene type), prim_id, geom_id (int32 holding uint32 bits) and,
         with normals=True, normal (n,3). Asynchronous on `stream` (torch's current stream by default).
         With ``hits`` given (same dict layout), those tensors are used as in/out RayHits.
         """
         import torch
 
-        cols_o = _columns(org)
-        cols_d = _columns(dirs)
+        tdt = torch.float32 if self.f32 else torch.float64
+        cols_o = _columns(org, tdt)
+        cols_d = _columns(dirs, tdt)
         n = cols_o[0].numel()
         dev = cols_o[0].device
-        tmn = None if tmin is None else _scalar_col(tmin, n, dev)
-        tmx = None if tmax is None else _scalar_col(tmax, n, dev)
+        tmn = None if tmin is None else _scalar_col(tmin, n, dev, tdt)
+        tmx = None if tmax is None else _scalar_col(tmax, n, dev, tdt)
         flags = capi.FLAG_ANY_HIT if any_hit else 0
         if hits is None:
-            hits = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in ("t", "u", "v")}
+            hits = {k: torch.empty(n, dtype=tdt, device=dev) for k in ("t", "u", "v")}
             hits["prim_id"] = torch.empty(n, dtype=torch.int32, device=dev)
             hits["geom_id"] = torch.empty(n, dtype=torch.int32, device=dev)
             if normals:
-                hits["normal"] = torch.empty((3, n), dtype=torch.float64, device=dev)
+                hits["normal"] = torch.empty((3, n), dtype=tdt, device=dev)
         else:
             flags |= capi.FLAG_HITS_INOUT
         nrm = hits.get("normal")
@@ -178,7 +194,8 @@ class Scene:
         L = capi.Launch(precision, flags, block_size, schedule, refill, sort, stack_ring)
         st = stream if stream is not None else torch.cuda.current_stream(dev)
         keep = (cols_o, cols_d, tmn, tmx)  # noqa: F841 (inputs alive until the launch is enqueued)
-        check(lib().brt_intersect(self._h, C.byref(rays), C.byref(H), n, C.byref(L), C.c_void_p(st.cuda_stream)))
+        check(self._fn("brt_intersect")(self._h, C.byref(rays), C.byref(H), n, C.byref(L),
+                                        C.c_void_p(st.cuda_stream)))
         return hits
 
     # -------------------------------------------------------------------------------------------- distances
@@ -321,23 +338,24 @@ class Batch:
         self.close()
 
 
-def _columns(x):
-    """(n,3) tensor or three columns -> three contiguous float64 columns on the same device."""
+def _columns(x, dtype=None):
+    """(n,3) tensor or three columns -> three contiguous columns (float64 unless `dtype`) on the same device."""
     import torch
 
+    dtype = dtype or torch.float64
     if isinstance(x, (list, tuple)) and len(x) == 3:
-        return [torch.as_tensor(c, dtype=torch.float64).contiguous() for c in x]
-    x = torch.as_tensor(x, dtype=torch.float64)
+        return [torch.as_tensor(c, dtype=dtype).contiguous() for c in x]
+    x = torch.as_tensor(x, dtype=dtype)
     if not (x.dim() == 2 and x.shape[1] == 3):
         raise ValueError("expected an (n,3) tensor or a tuple of three columns")
     x = x.t()
     return [x[k].contiguous() for k in range(3)]
 
 
-def _scalar_col(v, n, dev):
+def _scalar_col(v, n, dev, dtype=None):
     import torch
 
-    t = torch.as_tensor(v, dtype=torch.float64, device=dev)
+    t = torch.as_tensor(v, dtype=dtype or torch.float64, device=dev)
     if t.dim() == 0:
         t = t.expand(n)
     return t.contiguous()
@@ -365,14 +383,21 @@ class RayHit:
 
 
 def intersect1(scene: Scene, ray: Ray, rayhit: RayHit) -> bool:
-    """intersect1(scene, ray, rayhit) (blazert/scene.h:161-211): one ray on the GPU, rayhit updated in place."""
-    h = capi.Hit1((C.c_double * 3)(*rayhit.normal), rayhit.uv[0], rayhit.uv[1], rayhit.hit_distance,
-                  rayhit.prim_id, rayhit.geom_id)
-    o = np.asarray(ray.origin, dtype=np.float64)
-    d = np.asarray(ray.direction, dtype=np.float64)
+    """intersect1(scene, ray, rayhit) (blazert/scene.h:161-211): one ray on the GPU, rayhit updated in place.
+    For a Scene<float> every value is rounded to float32 first, as Ray<float> / RayHit<float> hold them."""
+    f32 = scene.f32
+    ct = C.c_float if f32 else C.c_double
+    mx = float(np.finfo(np.float32).max) if f32 else DBL_MAX
+    H = capi.Hit1F if f32 else capi.Hit1
+    h = H((ct * 3)(*rayhit.normal), rayhit.uv[0], rayhit.uv[1], min(rayhit.hit_distance, mx), rayhit.prim_id,
+          rayhit.geom_id)
+    o = np.asarray(ray.origin, dtype=scene.dtype)
+    d = np.asarray(ray.direction, dtype=scene.dtype)
     res = C.c_int(0)
-    check(lib().brt_intersect1(scene.handle, _dp(o), _dp(d), float(ray.min_hit_distance), float(ray.max_hit_distance),
-                               capi.FLAG_ANY_HIT if ray.any_hit else 0, C.byref(h), C.byref(res)))
+    fn = lib().brt_intersect1_f32 if f32 else lib().brt_intersect1
+    check(fn(scene.handle, o.ctypes.data_as(C.POINTER(ct)), d.ctypes.data_as(C.POINTER(ct)),
+             float(ray.min_hit_distance), min(float(ray.max_hit_distance), mx),
+             capi.FLAG_ANY_HIT if ray.any_hit else 0, C.byref(h), C.byref(res)))
     rayhit.normal = [h.normal[0], h.normal[1], h.normal[2]]
     rayhit.uv = [h.u, h.v]
     rayhit.hit_distance = h.t

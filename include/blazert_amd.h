@@ -306,6 +306,49 @@ int brt_scene_tune_leaf(brt_scene *scene, const brt_rays *rays, const brt_hits *
                         const uint32_t *leaf_sizes, uint32_t count, uint32_t repeats, brt_tune_result *out,
                         void *stream);
 
+/* ---- blazert::Scene<float> ----
+ * A scene made by brt_scene_create_f32 is the reference's Scene<float>: FP32 geometry, FP32 builder arithmetic
+ * (binning, SAH), FP32 traversal and primitive tests, FP32 hits, bit-identical to the reference's T = float
+ * instantiation. It takes the _f32 add functions and is traversed with brt_intersect_f32 / brt_intersect1_f32;
+ * commit, upload, build statistics, tree export and destroy are shared with Scene<double>. Split precision,
+ * batches, the device build, distance queries and mirror tracing are for Scene<double> only. */
+int brt_scene_create_f32(const brt_build_options *opts, brt_scene **out);
+int brt_scene_add_mesh_f32(brt_scene *scene, const float *vertices, uint32_t num_vertices, const uint32_t *faces,
+                           uint32_t num_faces, uint32_t *geom_id);
+int brt_scene_add_spheres_f32(brt_scene *scene, const float *centers, const float *radii, uint32_t n,
+                              uint32_t *geom_id);
+int brt_scene_add_planes_f32(brt_scene *scene, const float *centers, const float *dxs, const float *dys,
+                             const float *rotations, uint32_t n, uint32_t *geom_id);
+int brt_scene_add_cylinders_f32(brt_scene *scene, const float *centers, const float *semi_axes_a,
+                                const float *semi_axes_b, const float *heights, const float *rotations, uint32_t n,
+                                uint32_t *geom_id);
+
+/* FP32 device rays / hits (Ray<float>, RayHit<float>); same conventions as brt_rays / brt_hits. */
+typedef struct brt_rays_f32 {
+  const float *ox, *oy, *oz;
+  const float *dx, *dy, *dz;
+  const float *tmin, *tmax;
+} brt_rays_f32;
+typedef struct brt_hits_f32 {
+  float *t, *u, *v;
+  uint32_t *prim_id, *geom_id;
+  float *nx, *ny, *nz;
+} brt_hits_f32;
+typedef struct brt_hit1_f32 {
+  float normal[3];
+  float u, v;
+  float t;
+  uint32_t prim_id, geom_id;
+} brt_hit1_f32;
+
+/* Batched intersect1 for a Scene<float> (blazert/scene.h:161-211 with T = float). launch->precision must be
+ * BRT_PRECISION_FP64 (0, "the scene's own type") and sort BRT_SORT_NONE. Asynchronous on `stream`. */
+int brt_intersect_f32(const brt_scene *scene, const brt_rays_f32 *rays, const brt_hits_f32 *hits, uint64_t n,
+                      const brt_launch *launch, void *stream);
+/* Single-ray intersect1 for a Scene<float>: host memory in/out, blocking. */
+int brt_intersect1_f32(const brt_scene *scene, const float org[3], const float dir[3], float tmin, float tmax,
+                       uint32_t flags, brt_hit1_f32 *hit, int *hit_out);
+
 /* Single-ray drop-in for intersect1(scene, Ray<T>{org, dir, tmin, tmax}, rayhit): host memory in/out, blocking.
  * hit is in/out exactly like RayHit<T>; *hit_out receives the bool result. */
 int brt_intersect1(const brt_scene *scene, const double org[3], const double dir[3], double tmin, double tmax,

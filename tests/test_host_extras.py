@@ -118,3 +118,31 @@ def test_uneven_batch_and_mirror_workloads():
     assert np.abs(v[-8:]).max() == 3.0
     w = W.mirror_weights(100, 3)
     assert w.shape == (3, 100) and (w >= 0).all() and (w < 1).all()
+
+
+def test_scene_f32_type_guards():
+    """Scene<float> takes the _f32 adds only, and refuses the Scene<double>-only entry points before any GPU work."""
+    import ctypes as C
+
+    from blazert_amd import Scene, capi
+    from blazert_amd.capi import lib
+    s = Scene(dtype=np.float32)
+    assert s.add_spheres(np.zeros((2, 3)), [1.0, 2.0]) == 0
+    d = np.zeros(3)
+    rc = lib().brt_scene_add_mesh(s.handle, d.ctypes.data_as(C.POINTER(C.c_double)), 1,
+                                  np.zeros(3, np.uint32).ctypes.data_as(C.POINTER(C.c_uint32)), 0, None)
+    assert rc == capi.ERR_INVALID_ARGUMENT
+    s.commit()
+    st = s.build_stats(1)
+    assert st["num_primitives"] == 2 and st["built_on_device"] == 0
+    nodes, order = s.export_tree(1)
+    assert len(nodes) >= 1 and sorted(order) == [0, 1]
+    rays = capi.Rays()
+    hits = capi.Hits()
+    assert lib().brt_intersect(s.handle, C.byref(rays), C.byref(hits), 1, None, None) == capi.ERR_UNSUPPORTED
+    assert lib().brt_scene_commit_device(s.handle, None) == capi.ERR_UNSUPPORTED
+    sd = Scene()
+    f = np.zeros(3, np.float32)
+    rc = lib().brt_scene_add_spheres_f32(sd.handle, f.ctypes.data_as(C.POINTER(C.c_float)),
+                                         f.ctypes.data_as(C.POINTER(C.c_float)), 1, None)
+    assert rc == capi.ERR_INVALID_ARGUMENT
