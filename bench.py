@@ -407,6 +407,41 @@ def distance_leg(args, dev, v, f, scene, osc) -> dict:
             "pairs_mpairs_s": round(npair / ms_p / 1e3, 2), "pairs": npair}
 
 
+def float_scene_leg(args, dev, v, f, org, dirs) -> dict:
+    """blazert::Scene<float> on the same workload: the icosphere and the 4,194,304 incoherent rays in FP32
+    (brt_intersect_f32); the first 65,536 rays are compared bit for bit with the FP32 oracle."""
+    import torch
+
+    from blazert_amd import Scene
+    from oracle import oracle as OR
+    s = Scene(dtype=np.float32)
+    s.add_mesh(v, f)
+    t0 = time.perf_counter()
+    s.commit()
+    build_s = time.perf_counter() - t0
+    s.upload()
+    o32 = np.ascontiguousarray(org, dtype=np.float32)
+    d32 = np.ascontiguousarray(dirs, dtype=np.float32)
+    O = tuple(torch.from_numpy(np.ascontiguousarray(o32[:, k])).to(dev) for k in range(3))
+    D = tuple(torch.from_numpy(np.ascontiguousarray(d32[:, k])).to(dev) for k in range(3))
+    n = len(org)
+    ms, _ = time_launch(lambda: s.intersect(O, D), args.steps, args.warmup)
+    h = s.intersect(O, D)
+    torch.cuda.synchronize()
+    ns = 1 << 16
+    osc = OR.SceneF32()
+    osc.add_mesh(v, f)
+    osc.commit()
+    ref = osc.intersect_batch(o32[:ns], d32[:ns], normals=False, trace=True, nthreads=len(os.sched_getaffinity(0)))
+    ok = ref["trace"]["overflow"] == 0
+    exact = bool(np.array_equal(h["t"][:ns].cpu().numpy()[ok].view(np.uint32), ref["t"][ok].view(np.uint32))
+                 and np.array_equal(h["prim_id"][:ns].cpu().numpy().view(np.uint32)[ok], ref["prim_id"][ok]))
+    s.close()
+    log(f"Scene<float> {n / ms / 1e3:.1f} Mrays/s ({ms:.3f} ms), exact on {ns}: {exact}")
+    return {"value": round(n / ms / 1e3, 3), "unit": "Mrays/s", "ms_per_step": round(ms, 4), "commit_s": round(build_s, 3),
+            "bit_exact_vs_f32_oracle_sample": exact, "sample": ns}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,6 +453,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-primitive split-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven multi-scene batch leg")
     ap.add_argument("--no-distance", action="store_true", help="skip the distance-query leg")
+    ap.add_argument("--no-float", action="store_true", help="skip the Scene<float> leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -537,6 +573,8 @@ def main():
     out["split_precision"]["disagreements"] = classify_disagreements(osc, org, dirs, dis_idx, dis_other)
     del h64, h32, O, D, CO, CD
     out["device_commit"] = device_commit_leg(v, f, scene, build_s)
+    if not args.no_float:
+        out["scene_float"] = float_scene_leg(args, dev, v, f, org, dirs)
     if not args.no_distance:
         out["distance_queries"] = distance_leg(args, dev, v, f, scene, osc)
     if not args.no_mixed:

@@ -1,7 +1,7 @@
 // blazert_amd/blazert.hpp -- C++17 drop-in for blazeRT's scene API on the MI355X engine.
 //
 // Mirrors the reference's public types so code written against blazert::Scene<T> compiles unchanged for
-// T = double (blazert/blazert.h:15-21, blazert/scene.h:38-116, blazert/ray.h:20-85):
+// T = double and T = float (blazert/blazert.h:15-21, blazert/scene.h:38-116, blazert/ray.h:20-85):
 //   blazert::Vec3r<T>, Vec2r<T>, Vec3ui, Mat3r<T>, Vec3rList<T>, Vec3iList, Mat3rList<T>  (blazert/datatypes.h:31-59)
 //   blazert::Ray<T>{origin, direction, min_hit_distance, max_hit_distance, cull_back_face, any_hit}
 //   blazert::RayHit<T>{normal, uv, hit_distance, prim_id, geom_id}
@@ -19,6 +19,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../blazert_amd.h"
@@ -118,14 +119,68 @@ struct RayHit {
 template <typename T>
 class BlazertScene;
 
+namespace detail {
+// The C ABI of each scene type: Scene<double> (brt_*) and Scene<float> (brt_*_f32).
+template <typename T>
+struct Api;
+template <>
+struct Api<double> {
+  using Hit1 = brt_hit1;
+  static int create(const brt_build_options *o, brt_scene **s) { return brt_scene_create(o, s); }
+  static int add_mesh(brt_scene *s, const double *v, uint32_t nv, const uint32_t *f, uint32_t nf, uint32_t *g) {
+    return brt_scene_add_mesh(s, v, nv, f, nf, g);
+  }
+  static int add_spheres(brt_scene *s, const double *c, const double *r, uint32_t n, uint32_t *g) {
+    return brt_scene_add_spheres(s, c, r, n, g);
+  }
+  static int add_planes(brt_scene *s, const double *c, const double *dx, const double *dy, const double *R,
+                        uint32_t n, uint32_t *g) {
+    return brt_scene_add_planes(s, c, dx, dy, R, n, g);
+  }
+  static int add_cylinders(brt_scene *s, const double *c, const double *a, const double *b, const double *h,
+                           const double *R, uint32_t n, uint32_t *g) {
+    return brt_scene_add_cylinders(s, c, a, b, h, R, n, g);
+  }
+  static int intersect1(const brt_scene *s, const double *o, const double *d, double tmin, double tmax,
+                        uint32_t flags, brt_hit1 *h, int *hit) {
+    return brt_intersect1(s, o, d, tmin, tmax, flags, h, hit);
+  }
+};
+template <>
+struct Api<float> {
+  using Hit1 = brt_hit1_f32;
+  static int create(const brt_build_options *o, brt_scene **s) { return brt_scene_create_f32(o, s); }
+  static int add_mesh(brt_scene *s, const float *v, uint32_t nv, const uint32_t *f, uint32_t nf, uint32_t *g) {
+    return brt_scene_add_mesh_f32(s, v, nv, f, nf, g);
+  }
+  static int add_spheres(brt_scene *s, const float *c, const float *r, uint32_t n, uint32_t *g) {
+    return brt_scene_add_spheres_f32(s, c, r, n, g);
+  }
+  static int add_planes(brt_scene *s, const float *c, const float *dx, const float *dy, const float *R, uint32_t n,
+                        uint32_t *g) {
+    return brt_scene_add_planes_f32(s, c, dx, dy, R, n, g);
+  }
+  static int add_cylinders(brt_scene *s, const float *c, const float *a, const float *b, const float *h,
+                           const float *R, uint32_t n, uint32_t *g) {
+    return brt_scene_add_cylinders_f32(s, c, a, b, h, R, n, g);
+  }
+  static int intersect1(const brt_scene *s, const float *o, const float *d, float tmin, float tmax, uint32_t flags,
+                        brt_hit1_f32 *h, int *hit) {
+    return brt_intersect1_f32(s, o, d, tmin, tmax, flags, h, hit);
+  }
+};
+}  // namespace detail
+
 template <typename T>
 bool intersect1(const BlazertScene<T> &scene, const Ray<T> &ray, RayHit<T> &rayhit);
 
-// blazert/scene.h:38-116 for T = double. add_* return the geometry id, or unsigned(-1) for a second add of a
-// geometry type or any add after commit (blazert/scene.h:230-239).
+// blazert/scene.h:38-116 for T = double or float. add_* return the geometry id, or unsigned(-1) for a second add of
+// a geometry type or any add after commit (blazert/scene.h:230-239).
 template <typename T>
 class BlazertScene {
-  static_assert(std::is_same<T, double>::value, "the MI355X engine implements Scene<double>");
+  static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
+                "the MI355X engine implements Scene<double> and Scene<float>");
+  using Api = detail::Api<T>;
 
  public:
   BVHBuildOptions<T> build_options;
@@ -134,39 +189,39 @@ class BlazertScene {
   BlazertScene() = default;
 
   unsigned int add_mesh(const Vec3rList<T> &vertices, const Vec3iList &triangles) {
-    std::vector<double> v(3 * vertices.size());
+    std::vector<T> v(3 * vertices.size());
     std::vector<uint32_t> f(3 * triangles.size());
     for (size_t i = 0; i < vertices.size(); i++)
       for (int k = 0; k < 3; k++) v[3 * i + k] = vertices[i][k];
     for (size_t i = 0; i < triangles.size(); i++)
       for (int k = 0; k < 3; k++) f[3 * i + k] = triangles[i][k];
     uint32_t gid = BRT_INVALID_ID;
-    brt_scene_add_mesh(ensure(), v.data(), (uint32_t)vertices.size(), f.data(), (uint32_t)triangles.size(), &gid);
+    Api::add_mesh(ensure(), v.data(), (uint32_t)vertices.size(), f.data(), (uint32_t)triangles.size(), &gid);
     return gid;
   }
 
   unsigned int add_spheres(const Vec3rList<T> &centers, const std::vector<T> &radii) {
-    const std::vector<double> c = flat(centers);
+    const std::vector<T> c = flat(centers);
     uint32_t gid = BRT_INVALID_ID;
-    brt_scene_add_spheres(ensure(), c.data(), radii.data(), (uint32_t)radii.size(), &gid);
+    Api::add_spheres(ensure(), c.data(), radii.data(), (uint32_t)radii.size(), &gid);
     return gid;
   }
 
   unsigned int add_planes(const Vec3rList<T> &centers, const std::vector<T> &dxs, const std::vector<T> &dys,
                           const Mat3rList<T> &rotations) {
-    const std::vector<double> c = flat(centers), r = flat(rotations);
+    const std::vector<T> c = flat(centers), r = flat(rotations);
     uint32_t gid = BRT_INVALID_ID;
-    brt_scene_add_planes(ensure(), c.data(), dxs.data(), dys.data(), r.data(), (uint32_t)dxs.size(), &gid);
+    Api::add_planes(ensure(), c.data(), dxs.data(), dys.data(), r.data(), (uint32_t)dxs.size(), &gid);
     return gid;
   }
 
   unsigned int add_cylinders(const Vec3rList<T> &centers, const std::vector<T> &semi_axes_a,
                              const std::vector<T> &semi_axes_b, const std::vector<T> &heights,
                              const Mat3rList<T> &rotations) {
-    const std::vector<double> c = flat(centers), r = flat(rotations);
+    const std::vector<T> c = flat(centers), r = flat(rotations);
     uint32_t gid = BRT_INVALID_ID;
-    brt_scene_add_cylinders(ensure(), c.data(), semi_axes_a.data(), semi_axes_b.data(), heights.data(), r.data(),
-                            (uint32_t)semi_axes_a.size(), &gid);
+    Api::add_cylinders(ensure(), c.data(), semi_axes_a.data(), semi_axes_b.data(), heights.data(), r.data(),
+                       (uint32_t)semi_axes_a.size(), &gid);
     return gid;
   }
 
@@ -190,20 +245,20 @@ class BlazertScene {
       const brt_build_options o{build_options.min_leaf_primitives, build_options.max_tree_depth,
                                 build_options.bin_size};
       brt_scene *s = nullptr;
-      if (brt_scene_create(&o, &s) != BRT_OK) throw std::runtime_error(brt_last_error());
+      if (Api::create(&o, &s) != BRT_OK) throw std::runtime_error(brt_last_error());
       handle_.reset(s);
     }
     return handle_.get();
   }
 
-  static std::vector<double> flat(const Vec3rList<T> &a) {
-    std::vector<double> o(3 * a.size());
+  static std::vector<T> flat(const Vec3rList<T> &a) {
+    std::vector<T> o(3 * a.size());
     for (size_t i = 0; i < a.size(); i++)
       for (int k = 0; k < 3; k++) o[3 * i + k] = a[i][k];
     return o;
   }
-  static std::vector<double> flat(const Mat3rList<T> &a) {
-    std::vector<double> o(9 * a.size());
+  static std::vector<T> flat(const Mat3rList<T> &a) {
+    std::vector<T> o(9 * a.size());
     for (size_t i = 0; i < a.size(); i++)
       for (int k = 0; k < 9; k++) o[9 * i + k] = a[i].m[k];
     return o;
@@ -216,9 +271,9 @@ using Scene = BlazertScene<T>;
 // blazert/scene.h:161-211: rayhit is in/out; returns whether a hit was merged into it.
 template <typename T>
 bool intersect1(const BlazertScene<T> &scene, const Ray<T> &ray, RayHit<T> &rayhit) {
-  const double o[3] = {ray.origin[0], ray.origin[1], ray.origin[2]};
-  const double d[3] = {ray.direction[0], ray.direction[1], ray.direction[2]};
-  brt_hit1 h{};
+  const T o[3] = {ray.origin[0], ray.origin[1], ray.origin[2]};
+  const T d[3] = {ray.direction[0], ray.direction[1], ray.direction[2]};
+  typename detail::Api<T>::Hit1 h{};
   for (int k = 0; k < 3; k++) h.normal[k] = rayhit.normal[k];
   h.u = rayhit.uv[0];
   h.v = rayhit.uv[1];
@@ -227,7 +282,8 @@ bool intersect1(const BlazertScene<T> &scene, const Ray<T> &ray, RayHit<T> &rayh
   h.geom_id = rayhit.geom_id;
   int hit = 0;
   const uint32_t flags = ray.any_hit == Ray<T>::AnyHit::yes ? BRT_FLAG_ANY_HIT : 0u;
-  if (brt_intersect1(scene.native(), o, d, ray.min_hit_distance, ray.max_hit_distance, flags, &h, &hit) != BRT_OK)
+  if (detail::Api<T>::intersect1(scene.native(), o, d, ray.min_hit_distance, ray.max_hit_distance, flags, &h, &hit) !=
+      BRT_OK)
     throw std::runtime_error(brt_last_error());
   for (int k = 0; k < 3; k++) rayhit.normal[k] = h.normal[k];
   rayhit.uv[0] = h.u;

@@ -9,7 +9,10 @@
 #include <cstring>
 #include <memory>
 
-using ft = double;
+#ifndef DROPIN_FT
+#define DROPIN_FT double
+#endif
+using ft = DROPIN_FT;  // the scene type: double, or float with -DDROPIN_FT=float
 
 static int semantics() {
   auto centers = std::make_unique<blazert::Vec3rList<ft>>();
