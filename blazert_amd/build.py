@@ -64,5 +64,24 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     return out
 
 
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+USER_GEOMETRY_EXAMPLE = os.path.join(os.path.dirname(HERE), "examples", "user_geometry", "user_geometry.hip")
+USER_GEOMETRY_EXAMPLE_LIB = os.path.join(os.path.dirname(HERE), "examples", "user_geometry", "libuser_geometry.so")
+
+
+def build_user_geometry(src: str, out: str, verbose: bool = False) -> str:
+    """Compiles a user-geometry translation unit (device collections + include/blazert_amd/user_geometry.h) into a
+    shared library linked against libblazert_amd.so, with the flags the exact path needs."""
+    cc = hipcc()
+    out_tmp = out + ".tmp"
+    _run([cc, *COMMON, *DEVICE, f"-I{INCLUDE}", "-shared", src, f"-L{HERE}", "-lblazert_amd",
+          f"-Wl,-rpath,{HERE}", "-o", out_tmp])
+    os.replace(out_tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_user_geometry(USER_GEOMETRY_EXAMPLE, USER_GEOMETRY_EXAMPLE_LIB, verbose=True)

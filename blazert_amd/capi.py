@@ -100,6 +100,13 @@ class TuneResult(C.Structure):
                 ("best_ms", C.c_double), ("default_ms", C.c_double)]
 
 
+class BvhView(C.Structure):  # brt_bvh_view: device view of a user collection's tree
+    _fields_ = [("nodes", C.c_void_p), ("prim_ids", C.c_void_p), ("root_min", C.c_double * 3),
+                ("root_max", C.c_double * 3), ("root_ref", C.c_uint32), ("root_cnt", C.c_uint32),
+                ("num_nodes", C.c_uint32), ("num_prims", C.c_uint32), ("depth", C.c_uint32),
+                ("nan_boxes", C.c_uint32), ("real_bytes", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 # every entry point declared in include/blazert_amd.h: name -> (restype, argtypes)
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -152,6 +159,12 @@ SIGNATURES = {
                                    C.POINTER(MirrorOut), C.c_void_p]),
     "brt_mirror_adjoint": (C.c_int, [_P, C.POINTER(Rays), C.c_uint64, C.c_uint32, C.c_void_p, C.POINTER(MirrorGrads),
                                      C.c_void_p]),
+    "brt_bvh_build": (C.c_int, [_D, _D, _D, C.c_uint32, C.POINTER(BuildOptions), C.POINTER(C.c_void_p)]),
+    "brt_bvh_build_f32": (C.c_int, [_F, _F, _F, C.c_uint32, C.POINTER(BuildOptions), C.POINTER(C.c_void_p)]),
+    "brt_bvh_device_view": (C.c_int, [_P, C.POINTER(BvhView)]),
+    "brt_bvh_build_stats": (C.c_int, [_P, C.POINTER(BuildStats)]),
+    "brt_bvh_export_tree": (C.c_int, [_P, C.c_void_p, C.c_uint32, _U, _U]),
+    "brt_bvh_destroy": (C.c_int, [_P]),
 }
 
 _lib = None

@@ -19,6 +19,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include "../../include/blazert_amd/device/exact.h"
 #include "engine.h"
 #include "f32_round.h"
 
@@ -26,7 +27,6 @@ namespace brt {
 
 namespace {
 
-constexpr double L1 = 1.0 + 4.0 * DBL_EPSILON;  // far-plane widening of blazert/bvh/intersect.h:40
 constexpr uint32_t PREC_FP64 = 0;
 constexpr uint32_t FLAG_INOUT = 1u;
 constexpr uint32_t FLAG_ANY_HIT = 2u;
@@ -36,127 +36,16 @@ constexpr uint32_t FLAG_ANY_HIT = 2u;
 template <bool SPLIT>
 constexpr int stack_cap() { return SPLIT ? 128 : 64; }
 
-// Constants of the scene type T (blazert::Scene<T>): double for the drop-in path, float for Scene<float>.
-template <class R>
-struct Real;
-template <>
-struct Real<double> {
-  static constexpr double eps = DBL_EPSILON;
-  static constexpr double max = DBL_MAX;
-};
-template <>
-struct Real<float> {
-  static constexpr float eps = FLT_EPSILON;
-  static constexpr float max = FLT_MAX;
-};
-
-template <class R>
-__device__ __forceinline__ R std_min(R a, R b) { return (b < a) ? b : a; }
-template <class R>
-__device__ __forceinline__ R std_max(R a, R b) { return (a < b) ? b : a; }
-
-// blazert/datatypes.h:83-85
-__device__ __forceinline__ double product_sign(double x, double y) {
-  const unsigned long long ux = (unsigned long long)__double_as_longlong(x);
-  const unsigned long long uy = (unsigned long long)__double_as_longlong(y);
-  return __longlong_as_double((long long)(ux ^ (uy & 0x8000000000000000ull)));
-}
-__device__ __forceinline__ float product_sign(float x, float y) {
-  return __uint_as_float(__float_as_uint(x) ^ (__float_as_uint(y) & 0x80000000u));
-}
-
-// correctly rounded in both precisions
-__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
-
-template <class R>
-struct RayT {
-  R o[3];
-  R d[3];    // normalize(raw direction)
-  R inv[3];  // 1 / raw direction
-  uint32_t sign;  // bit i: raw direction[i] < 0; bit 3: no slab term can be NaN (see make_ray)
-  R tmin, tmax;
-};
+// The reference's ray setup and exact slab test (Real, RayT, setup_ray, slab_exact, product_sign, dsqrt) are shared
+// with user-collection traversal: include/blazert_amd/device/exact.h.
+using namespace exact;
 using Ray = RayT<double>;
-constexpr uint32_t RAY_NAN_FREE = 8u;
 
-// blazert/ray.h:56-65 with normalize(v) = v * (len != 0 ? 1/len : 0)
 template <class R>
 __device__ __forceinline__ void make_ray(const RaysViewT<R> &V, uint64_t i, RayT<R> &r) {
+  const R org[3] = {V.ox[i], V.oy[i], V.oz[i]};
   const R raw[3] = {V.dx[i], V.dy[i], V.dz[i]};
-  r.o[0] = V.ox[i];
-  r.o[1] = V.oy[i];
-  r.o[2] = V.oz[i];
-  const R len = dsqrt(raw[0] * raw[0] + raw[1] * raw[1] + raw[2] * raw[2]);
-  const R il = (len != R(0)) ? R(1) / len : R(0);
-  r.sign = 0;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    r.d[k] = raw[k] * il;
-    r.inv[k] = R(1) / raw[k];
-    r.sign |= (raw[k] < R(0) ? 1u : 0u) << k;
-  }
-  r.tmin = V.tmin ? V.tmin[i] : V.tmin0;
-  r.tmax = V.tmax ? V.tmax[i] : V.tmax0;
-  // With a finite origin, finite non-zero direction components (so 1/d is finite and non-zero), a non-NaN tmin and
-  // finite node boxes (checked on the host), (b - o) * inv is never NaN: the slab chains reduce to plain max/min.
-  bool ok = r.tmin == r.tmin;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    ok = ok && fabs(r.o[k]) <= Real<R>::max && raw[k] != R(0) && fabs(raw[k]) <= Real<R>::max &&
-         fabs(r.inv[k]) <= Real<R>::max;
-  }
-  if (ok) r.sign |= RAY_NAN_FREE;
-}
-
-// ------------------------------------------------------------------------------------------------ slab tests
-// Exact slab test of one child box (blazert/bvh/intersect.h:32-51 in the scene type), split into its hit-distance
-// independent part. Returns false when the box fails for every hit distance; otherwise key is the value the current
-// hit distance h must not be below (pass iff !(h < key)); key = -inf when a NaN far-plane term removes h from the
-// reference's min chain.
-template <class R>
-__device__ __forceinline__ bool slab_exact(const R *b, const RayT<R> &r, R &key) {
-  const R L1 = R(1) + R(4) * Real<R>::eps;  // far-plane widening of blazert/bvh/intersect.h:40
-  if (r.sign & RAY_NAN_FREE) {
-    R A = r.tmin, M = std::numeric_limits<R>::infinity();
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const bool s = (r.sign >> i) & 1u;
-      const R nearp = s ? b[3 + i] : b[i];
-      const R farp = s ? b[i] : b[3 + i];
-      A = fmax(A, (nearp - r.o[i]) * r.inv[i]);
-      M = fmin(M, ((farp - r.o[i]) * r.inv[i]) * L1);
-    }
-    key = A;
-    return A <= M;
-  }
-  R acc = r.tmin;
-  R m[3];
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    const bool s = (r.sign >> i) & 1u;
-    const R nearp = s ? b[3 + i] : b[i];
-    const R farp = s ? b[i] : b[3 + i];
-    const R mn = (nearp - r.o[i]) * r.inv[i];
-    m[i] = ((farp - r.o[i]) * r.inv[i]) * L1;
-    acc = std_max(mn, acc);
-  }
-  const bool n0 = m[0] != m[0], n1 = m[1] != m[1], n2 = m[2] != m[2];
-  R M;
-  bool hdep = false;
-  if (n2) {
-    return false;
-  } else if (n1) {
-    M = m[2];
-  } else if (n0) {
-    M = std_min(m[2], m[1]);
-  } else {
-    M = fmin(fmin(m[0], m[1]), m[2]);
-    hdep = true;
-  }
-  if (!(acc <= M)) return false;
-  key = hdep ? acc : -std::numeric_limits<R>::infinity();
-  return true;
+  setup_ray(org, raw, V.tmin ? V.tmin[i] : V.tmin0, V.tmax ? V.tmax[i] : V.tmax0, r);
 }
 
 // Conservative FP32 slab test: per axis lower/upper bounds of the FP64 entry/exit distances.

@@ -10,23 +10,12 @@
 
 #include <limits>
 
+#include "../../include/blazert_amd/device/nodes.h"
+
 namespace brt {
 
-constexpr uint32_t LEAF_BIT = 0x80000000u;
-constexpr uint32_t AXIS_SHIFT = 30;           // cnt[1] high bits carry the parent's split axis
-constexpr uint32_t CNT_MASK = (1u << AXIS_SHIFT) - 1u;
-constexpr uint32_t INVALID_ID = 0xFFFFFFFFu;
-
-enum GeomType : uint32_t { TRIANGLES = 0, SPHERES = 1, PLANES = 2, CYLINDERS = 3, NUM_TYPES = 4 };
-
-// FP64 internal node: 128 B, one cache line. Child c box: b[6c+0..2] = min xyz, b[6c+3..5] = max xyz.
-struct alignas(128) Node64 {
-  double b[12];
-  uint32_t ref[2];  // child reference: internal node index, or LEAF_BIT | first primitive (leaf order)
-  uint32_t cnt[2];  // leaf primitive count (0 for an internal child); cnt[1] >> 30 = split axis
-  uint32_t pad[4];
-};
-static_assert(sizeof(Node64) == 128, "Node64 must be one 128-byte line");
+// USER_COLLECTION: a tree over a user collection's boxes (brt_bvh_build), with no primitive records
+enum GeomType : uint32_t { TRIANGLES = 0, SPHERES = 1, PLANES = 2, CYLINDERS = 3, NUM_TYPES = 4, USER_COLLECTION = 16 };
 
 // FP32 4-wide node for split-precision traversal, 128 B: the binary tree collapsed two levels at a time (an
 // internal node's internal children are replaced by their own two children), so one record holds up to four
@@ -82,15 +71,8 @@ struct alignas(128) CylRec {
 };
 static_assert(sizeof(CylRec) == 128, "CylRec size");
 
-// Scene<float> (the reference instantiated with T = float): the same children-in-parent node with FP32 boxes
-// (64 B) and FP32 primitive records, traversed and tested in FP32 throughout.
-struct alignas(64) Node32 {
-  float b[12];
-  uint32_t ref[2];
-  uint32_t cnt[2];
-};
-static_assert(sizeof(Node32) == 64, "Node32 size");
-
+// Scene<float> (the reference instantiated with T = float): Node32 records (FP32 boxes) and FP32 primitive
+// records, traversed and tested in FP32 throughout.
 struct alignas(16) TriRecF {
   using real = float;
   static constexpr uint32_t kind = TRIANGLES;

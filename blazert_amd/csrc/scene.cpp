@@ -616,6 +616,10 @@ void flatten(Geometry &g, GeomType type) {
     }
     break;
   }
+  case USER_COLLECTION:  // a user collection's tree: the primitives stay with the user
+    g.rec_size = 0;
+    g.recs.clear();
+    break;
   default: {
     g.rec_size = sizeof(CylRec);
     g.recs.assign((size_t)n * sizeof(CylRec), 0);
@@ -721,6 +725,10 @@ void flatten_f32(GeometryF &g, GeomType type) {
       r.dy = g.dys[p];
       for (int k = 0; k < 9; k++) r.R[k] = g.rot[9ull * p + k];
     });
+    break;
+  case USER_COLLECTION:
+    g.rec_size = 0;
+    g.recs.clear();
     break;
   default:
     g.rec_size = sizeof(CylRecF);
@@ -1605,6 +1613,149 @@ int brt_intersect_f32(const brt_scene *cs, const brt_rays_f32 *rays, const brt_h
     hipError_t e = launch_init_hits_f32(hv, n, st);
     if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
   }
+  return BRT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------ user collections
+// blazert::BVH<T, Collection> over a user-defined primitive type (blazert/bvh/accel.h:32-52): the reference's
+// SAHBinnedBuilder run on the collection's boxes and centers, flattened to the device node records. The primitives
+// stay with the user; device code traverses the tree with its own intersector (include/blazert_amd/user_geometry.h).
+struct brt_bvh {
+  bool f32 = false;
+  brt_build_options opts{4, 28, 64};
+  Geometry g;
+  GeometryF gF;
+  std::mutex mu;
+  int device = -1;  // device the tree was uploaded to (-1: not yet)
+  std::vector<void *> allocs;
+  brt_bvh_view view{};
+  ~brt_bvh() {
+    for (void *p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+template <class R>
+int bvh_build_t(const R *box_min, const R *box_max, const R *centers, uint32_t n, const brt_build_options *opts,
+                brt_bvh **out) {
+  if (!out) return fail(BRT_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (n && (!box_min || !box_max || !centers)) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL box or center array");
+  if (opts && (opts->bin_size < 2 || opts->min_leaf_primitives == 0))
+    return fail(BRT_ERR_INVALID_ARGUMENT, "bin_size must be >= 2 and min_leaf_primitives >= 1");
+  std::unique_ptr<brt_bvh> b(new (std::nothrow) brt_bvh());
+  if (!b) return fail(BRT_ERR_OUT_OF_MEMORY, "bvh allocation failed");
+  if (opts) b->opts = *opts;
+  b->f32 = std::is_same<R, float>::value;
+  GeometryT<R> *g;
+  if constexpr (std::is_same<R, double>::value) g = &b->g;
+  else g = &b->gF;
+  g->present = true;
+  g->geom_id = 0;
+  g->boxes.resize(n);
+  std::copy(box_min, box_min + 3ull * n, g->boxes.lo.begin());
+  std::copy(box_max, box_max + 3ull * n, g->boxes.hi.begin());
+  std::copy(centers, centers + 3ull * n, g->boxes.ctr.begin());
+  g->boxes.classify();
+  build_geometry(*g, USER_COLLECTION, b->opts);
+  *out = b.release();
+  return BRT_OK;
+}
+
+template <class R>
+int bvh_upload(brt_bvh *b, GeometryT<R> &g) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(BRT_ERR_NO_DEVICE, "no HIP device");
+  if (b->device >= 0) {
+    if (b->device != dev) return fail(BRT_ERR_STATE, "the tree was uploaded to another device");
+    return BRT_OK;
+  }
+  if (g.stats.max_tree_depth >= 64) return fail(BRT_ERR_UNSUPPORTED, "tree deeper than the 64-entry device stack");
+  brt_bvh_view v{};
+  v.real_bytes = sizeof(R);
+  v.num_prims = g.boxes.n;
+  v.depth = g.stats.max_tree_depth;
+  if (g.boxes.n) {
+    auto put = [&](const void *src, size_t bytes, const void **dst) -> hipError_t {
+      *dst = nullptr;
+      if (!bytes) return hipSuccess;
+      void *p = nullptr;
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) return e;
+      b->allocs.push_back(p);
+      *dst = p;
+      return hipMemcpy(p, src, bytes, hipMemcpyHostToDevice);
+    };
+    const void *nodes = nullptr, *ids = nullptr;
+    hipError_t e;
+    if constexpr (std::is_same<R, double>::value) e = put(g.n64.data(), g.n64.size() * sizeof(Node64), &nodes);
+    else e = put(g.n32.data(), g.n32.size() * sizeof(Node32), &nodes);
+    if (e == hipSuccess) e = put(g.order.data(), g.order.size() * 4, &ids);
+    if (e != hipSuccess) return device_error(e, "upload failed");
+    v.nodes = nodes;
+    v.prim_ids = static_cast<const uint32_t *>(ids);
+    for (int k = 0; k < 3; k++) {
+      v.root_min[k] = g.view.root_min[k];
+      v.root_max[k] = g.view.root_max[k];
+    }
+    v.root_ref = g.view.root_ref;
+    v.root_cnt = g.view.root_cnt;
+    v.num_nodes = g.view.n_nodes;
+    v.nan_boxes = g.view.nan_boxes;
+  }
+  b->view = v;
+  b->device = dev;
+  return BRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int brt_bvh_build(const double *box_min, const double *box_max, const double *centers, uint32_t n,
+                  const brt_build_options *opts, brt_bvh **out) {
+  return bvh_build_t(box_min, box_max, centers, n, opts, out);
+}
+
+int brt_bvh_build_f32(const float *box_min, const float *box_max, const float *centers, uint32_t n,
+                      const brt_build_options *opts, brt_bvh **out) {
+  return bvh_build_t(box_min, box_max, centers, n, opts, out);
+}
+
+int brt_bvh_device_view(brt_bvh *b, brt_bvh_view *view) {
+  if (!b || !view) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
+  std::lock_guard<std::mutex> lk(b->mu);
+  const int rc = b->f32 ? bvh_upload(b, b->gF) : bvh_upload(b, b->g);
+  if (rc) return rc;
+  *view = b->view;
+  return BRT_OK;
+}
+
+int brt_bvh_build_stats(const brt_bvh *b, brt_build_stats *stats) {
+  if (!b || !stats) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
+  *stats = b->f32 ? b->gF.stats : b->g.stats;
+  return BRT_OK;
+}
+
+int brt_bvh_export_tree(const brt_bvh *b, brt_tree_node *nodes, uint32_t capacity, uint32_t *prim_order,
+                        uint32_t *num_nodes) {
+  if (!b || !num_nodes) return fail(BRT_ERR_INVALID_ARGUMENT, "bad argument");
+  const std::vector<brt_tree_node> &tree = b->f32 ? b->gF.tree : b->g.tree;
+  const std::vector<uint32_t> &order = b->f32 ? b->gF.order : b->g.order;
+  *num_nodes = (uint32_t)tree.size();
+  if (nodes) {
+    if (capacity < tree.size()) return fail(BRT_ERR_INVALID_ARGUMENT, "capacity too small");
+    std::memcpy(nodes, tree.data(), tree.size() * sizeof(brt_tree_node));
+  }
+  if (prim_order && !order.empty()) std::memcpy(prim_order, order.data(), order.size() * 4);
+  return BRT_OK;
+}
+
+int brt_bvh_destroy(brt_bvh *b) {
+  delete b;
   return BRT_OK;
 }
 

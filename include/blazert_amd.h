@@ -354,6 +354,43 @@ int brt_intersect1_f32(const brt_scene *scene, const float org[3], const float d
 int brt_intersect1(const brt_scene *scene, const double org[3], const double dir[3], double tmin, double tmax,
                    uint32_t flags, brt_hit1 *hit, int *hit_out);
 
+/* ---- user-defined collections ----
+ * blazert::BVH<T, Collection> for a primitive type of the caller's (examples/geometry_template/GEOM_TEMPLATE.h:
+ * 19-137, blazert/bvh/accel.h:32-52): SAHBinnedBuilder::build (blazert/bvh/builder.h:48-82) over the collection's
+ * per-primitive boxes (get_primitive_bounding_box) and centers (get_primitive_center), 3 values per primitive, host
+ * memory, copied. The tree is the reference's for those boxes and options; the primitives stay with the caller,
+ * whose device code traverses the tree with its own intersector (include/blazert_amd/user_geometry.h, the
+ * counterpart of traverse(), blazert/bvh/accel.h:80-120). _f32 builds BVH<float, Collection>. */
+typedef struct brt_bvh brt_bvh;
+
+/* Device view of a built tree: node records (brt::Node64 for an FP64 tree, brt::Node32 for FP32; layout in
+ * include/blazert_amd/device/nodes.h), the leaf order (leaf position -> collection primitive index), the root box
+ * and the root as a child reference. nodes / prim_ids are NULL for an empty collection (num_prims == 0). */
+typedef struct brt_bvh_view {
+  const void *nodes;
+  const uint32_t *prim_ids;
+  double root_min[3], root_max[3];
+  uint32_t root_ref, root_cnt;
+  uint32_t num_nodes, num_prims;
+  uint32_t depth;
+  uint32_t nan_boxes;  /* some box coordinate is NaN */
+  uint32_t real_bytes; /* 8: FP64 tree, 4: FP32 tree */
+  uint32_t reserved;
+} brt_bvh_view;
+
+int brt_bvh_build(const double *box_min, const double *box_max, const double *centers, uint32_t n,
+                  const brt_build_options *opts, brt_bvh **out);
+int brt_bvh_build_f32(const float *box_min, const float *box_max, const float *centers, uint32_t n,
+                      const brt_build_options *opts, brt_bvh **out);
+/* Uploads the tree to the current device on the first call (BRT_ERR_STATE if it was uploaded to another one,
+ * BRT_ERR_UNSUPPORTED for a tree deeper than the 64-entry device stack) and returns its device view. */
+int brt_bvh_device_view(brt_bvh *bvh, brt_bvh_view *view);
+int brt_bvh_build_stats(const brt_bvh *bvh, brt_build_stats *stats);
+/* Reference-form tree and leaf order, as brt_scene_export_tree. */
+int brt_bvh_export_tree(const brt_bvh *bvh, brt_tree_node *nodes, uint32_t capacity, uint32_t *prim_order,
+                        uint32_t *num_nodes);
+int brt_bvh_destroy(brt_bvh *bvh);
+
 #ifdef __cplusplus
 }
 #endif

@@ -586,6 +586,24 @@ int orc_commit(orc_scene *s) {
   return 1;
 }
 
+/* blazert/bvh/builder.h:48-82 on a caller-defined collection (examples/geometry_template/GEOM_TEMPLATE.h:60-96):
+ * the builder sees only the collection's boxes and centers, given here as arrays. The result is a committed scene
+ * whose sphere slot (type 1) holds the tree; only its tree export is meaningful. */
+orc_scene *orc_collection_scene(uint32_t min_leaf_primitives, uint32_t max_tree_depth, uint32_t bin_size,
+                                const real *box_min, const real *box_max, const real *centers, uint32_t n) {
+  orc_scene *s = orc_scene_new(min_leaf_primitives, max_tree_depth, bin_size);
+  orc_coll *c = &s->c[ORC_SPHERES];
+  alloc_boxes(c, n);
+  memcpy(c->box_min, box_min, 3 * (size_t)n * sizeof(real));
+  memcpy(c->box_max, box_max, 3 * (size_t)n * sizeof(real));
+  memcpy(c->centers, centers, 3 * (size_t)n * sizeof(real));
+  c->present = 1;
+  s->geom_id[ORC_SPHERES] = s->geometries++;
+  build_type(s, ORC_SPHERES);
+  s->committed = 1;
+  return s;
+}
+
 uint32_t orc_tree_size(const orc_scene *s, int type) { return s->c[type].present ? s->c[type].n_nodes : 0; }
 
 int orc_tree_export(const orc_scene *s, int type, orc_node *nodes, uint32_t *prim_order, orc_stats *stats) {

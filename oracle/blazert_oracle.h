@@ -107,6 +107,8 @@ uint32_t orc_prim_count(const orc_scene *s, int type);
 uint32_t orc_geom_id(const orc_scene *s, int type);
 
 /* built tree export (reference pre-order layout) */
+orc_scene *orc_collection_scene(uint32_t min_leaf_primitives, uint32_t max_tree_depth, uint32_t bin_size,
+                                const orc_real *box_min, const orc_real *box_max, const orc_real *centers, uint32_t n);
 uint32_t orc_tree_size(const orc_scene *s, int type);
 int orc_tree_export(const orc_scene *s, int type, orc_node *nodes, uint32_t *prim_order, orc_stats *stats);
 

@@ -99,6 +99,8 @@ def lib():
         L.orc_tree_size.restype = C.c_uint32
         L.orc_tree_size.argtypes = [P, C.c_int]
         L.orc_tree_export.argtypes = [P, C.c_int, C.c_void_p, up, C.POINTER(Stats)]
+        L.orc_collection_scene.restype = P
+        L.orc_collection_scene.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, dp, dp, dp, C.c_uint32]
         L.orc_ray_init.argtypes = [C.POINTER(Ray), dp, dp, C.c_double, C.c_double, C.c_int]
         L.orc_intersect_node.argtypes = [dp, dp, dp, dp, C.POINTER(Ray)]
         L.orc_intersect_single.argtypes = [P, C.c_int, C.c_uint32, C.POINTER(Ray), C.POINTER(Hit)]
@@ -444,6 +446,8 @@ def lib_f32():
         L.orc_tree_size.restype = C.c_uint32
         L.orc_tree_size.argtypes = [P, C.c_int]
         L.orc_tree_export.argtypes = [P, C.c_int, C.c_void_p, up, C.POINTER(Stats)]
+        L.orc_collection_scene.restype = P
+        L.orc_collection_scene.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, fp, fp, fp, C.c_uint32]
         L.orc_ray_init.argtypes = [C.POINTER(RayF), fp, fp, C.c_float, C.c_float, C.c_int]
         L.orc_intersect_single.argtypes = [P, C.c_int, C.c_uint32, C.POINTER(RayF), C.POINTER(HitF)]
         L.orc_intersect1.argtypes = [P, C.POINTER(RayF), C.POINTER(HitF), C.POINTER(Trace)]
@@ -556,3 +560,23 @@ class SceneF32:
         if trace:
             out["trace"] = tr
         return out
+
+
+def collection_tree(box_min, box_max, centers, dtype=np.float64, min_leaf_primitives=4, max_tree_depth=28,
+                    bin_size=64):
+    """The reference builder (blazert/bvh/builder.h:48-82) on a caller-defined collection given by its boxes and
+    centers ((n, 3) arrays), in T = dtype. Returns (nodes, leaf order, stats)."""
+    f32 = np.dtype(dtype) == np.float32
+    L = lib_f32() if f32 else lib()
+    arrs = [np.ascontiguousarray(np.asarray(a, dtype).reshape(-1, 3)) for a in (box_min, box_max, centers)]
+    ptr = _fp if f32 else _dp
+    h = L.orc_collection_scene(min_leaf_primitives, max_tree_depth, bin_size, *[ptr(a) for a in arrs], len(arrs[0]))
+    try:
+        n = L.orc_tree_size(h, 1)
+        nodes = np.zeros(n, dtype=NODE_DTYPE_F32 if f32 else NODE_DTYPE)
+        order = np.zeros(max(len(arrs[0]), 1), dtype=np.uint32)
+        st = Stats()
+        L.orc_tree_export(h, 1, nodes.ctypes.data_as(C.c_void_p), _up(order), C.byref(st))
+        return nodes, order[:len(arrs[0])], {name: getattr(st, name) for name, _ in Stats._fields_}
+    finally:
+        L.orc_scene_free(h)
