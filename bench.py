@@ -442,6 +442,67 @@ def float_scene_leg(args, dev, v, f, org, dirs) -> dict:
             "bit_exact_vs_f32_oracle_sample": exact, "sample": ns}
 
 
+def user_geometry_leg(args, dev) -> dict:
+    """A user-defined collection (examples/user_geometry, include/blazert_amd/user_geometry.h): 1,048,576 spheres in
+    [-40, 40]^3 and 4,194,304 incoherent rays from inside the cloud, traced by the header's one-ray-per-thread
+    traversal with the example's sphere intersector, beside the engine's built-in sphere path on the same tree; the
+    two hit records are compared bit for bit over the whole batch."""
+    import ctypes as C
+
+    import torch
+
+    from blazert_amd import BVH, Scene, capi
+    from blazert_amd import workloads as W
+    lib = C.CDLL(os.path.join(ROOT, "examples", "user_geometry", "libuser_geometry.so"))
+    P, U, U64 = C.c_void_p, C.c_uint, C.c_uint64
+    lib.ug_spheres_intersect.argtypes = [P, P, P, U, C.POINTER(capi.Rays), C.POINTER(capi.Hits), U64, U, P]
+    c, r = W.sphere_field(1 << 20, seed=5)
+    c = np.ascontiguousarray(c * 4.0)
+    s = Scene()
+    s.add_spheres(c, r)
+    s.commit()
+    s.upload()
+    t0 = time.perf_counter()
+    bvh = BVH(box_min=c - r[:, None], box_max=c + r[:, None], centers=c)
+    bvh.device_view()
+    build_s = time.perf_counter() - t0
+    org, dirs = W.incoherent_rays(1 << 22, 6, 40.0)
+    n = len(org)
+    O = tuple(torch.from_numpy(np.ascontiguousarray(org[:, k])).to(dev) for k in range(3))
+    D = tuple(torch.from_numpy(np.ascontiguousarray(dirs[:, k])).to(dev) for k in range(3))
+    cd, rd = torch.from_numpy(c).to(dev), torch.from_numpy(np.ascontiguousarray(r)).to(dev)
+    hu = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in ("t", "u", "v")}
+    hu["prim_id"] = torch.empty(n, dtype=torch.int32, device=dev)
+    hu["geom_id"] = torch.empty(n, dtype=torch.int32, device=dev)
+    R = capi.Rays(*[x.data_ptr() for x in O], *[x.data_ptr() for x in D], None, None)
+    H = capi.Hits(hu["t"].data_ptr(), hu["u"].data_ptr(), hu["v"].data_ptr(), hu["prim_id"].data_ptr(),
+                  hu["geom_id"].data_ptr(), None, None, None)
+
+    def user():
+        rc = lib.ug_spheres_intersect(bvh.handle, cd.data_ptr(), rd.data_ptr(), len(r), C.byref(R), C.byref(H), n, 0,
+                                      C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        if rc != 0:
+            raise RuntimeError(f"ug_spheres_intersect: {rc}")
+
+    ms_user, _ = time_launch(user, args.steps, args.warmup)
+    ms_builtin, _ = time_launch(lambda: s.intersect(O, D), args.steps, args.warmup)
+    hb = s.intersect(O, D)
+    user()
+    torch.cuda.synchronize()
+    exact = bool(torch.equal(hu["t"].view(torch.int64), hb["t"].view(torch.int64))
+                 and torch.equal(hu["prim_id"], hb["prim_id"]) and torch.equal(hu["geom_id"], hb["geom_id"]))
+    hits = int((hb["prim_id"] != -1).sum())
+    s.close()
+    bvh.close()
+    log(f"user collection {n / ms_user / 1e3:.1f} Mrays/s ({ms_user:.3f} ms) vs built-in spheres "
+        f"{n / ms_builtin / 1e3:.1f} Mrays/s, identical: {exact}")
+    return {"value": round(n / ms_user / 1e3, 3), "unit": "Mrays/s", "ms_per_step": round(ms_user, 4),
+            "builtin_spheres_mrays_s": round(n / ms_builtin / 1e3, 3), "builtin_ms": round(ms_builtin, 4),
+            "identical_to_builtin_all_rays": exact, "hits": hits, "tree_build_s": round(build_s, 3),
+            "workload": "1,048,576 spheres (r in [0.1, 0.5]) in [-40, 40]^3, 4,194,304 incoherent rays from inside, "
+                        "example sphere intersector through include/blazert_amd/user_geometry.h"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -454,6 +515,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven multi-scene batch leg")
     ap.add_argument("--no-distance", action="store_true", help="skip the distance-query leg")
     ap.add_argument("--no-float", action="store_true", help="skip the Scene<float> leg")
+    ap.add_argument("--no-user", action="store_true", help="skip the user-defined collection leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -577,6 +639,8 @@ def main():
         out["scene_float"] = float_scene_leg(args, dev, v, f, org, dirs)
     if not args.no_distance:
         out["distance_queries"] = distance_leg(args, dev, v, f, scene, osc)
+    if not args.no_user:
+        out["user_geometry"] = user_geometry_leg(args, dev)
     if not args.no_mixed:
         out["mixed_split_precision"] = mixed_leg(args, dev)
     if not args.no_batch:

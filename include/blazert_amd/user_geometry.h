@@ -214,16 +214,87 @@ struct DeviceBVH {
 // Deepest traversal stack: trees are limited to depth 63 at upload and the DFS pushes at most one entry per level.
 constexpr int kUserStack = 64;
 
+// ---------------------------------------------------------------------------------------------- stacks
+// Entries are (child reference, leaf count, key): the key is the entry distance the current hit distance must not
+// be below when the entry is popped.
+template <typename T>
+struct StackArea {  // private memory
+  T key[kUserStack];
+  uint32_t ref[kUserStack], cnt[kUserStack];
+};
+
+// Every entry in private memory: usable from any kernel.
+template <typename T>
+struct PrivateStack {
+  StackArea<T> *a;
+  int sp = 0;
+  __device__ explicit PrivateStack(StackArea<T> *area) : a(area) {}
+  __device__ bool empty() const { return sp == 0; }
+  __device__ void push(uint32_t r, uint32_t c, T k) {
+    a->key[sp] = k;
+    a->ref[sp] = r;
+    a->cnt[sp] = c;
+    sp++;
+  }
+  __device__ void pop(uint32_t &r, uint32_t &c, T &k) {
+    sp--;
+    r = a->ref[sp];
+    c = a->cnt[sp];
+    k = a->key[sp];
+  }
+};
+
+// The newest RING entries of a lane in LDS (slot-major, lane-minor, so a wave's accesses spread over the banks),
+// older ones spilled to private memory, which only rays deeper than the ring touch. Entries [lo, sp) are in LDS,
+// [0, lo) in the spill area. RING is a power of two.
+template <typename T, int RING>
+struct RingStack {
+  T *lkey;  // this lane's slot 0; slot j is `lanes` elements further
+  uint32_t *lref, *lcnt;
+  uint32_t lanes;
+  StackArea<T> *spill;
+  int sp = 0, lo = 0;
+  __device__ bool empty() const { return sp == 0; }
+  __device__ void push(uint32_t r, uint32_t c, T k) {
+    if (sp - lo == RING) {
+      const uint32_t j = (uint32_t)(lo & (RING - 1)) * lanes;
+      spill->key[lo] = lkey[j];
+      spill->ref[lo] = lref[j];
+      spill->cnt[lo] = lcnt[j];
+      lo++;
+    }
+    const uint32_t j = (uint32_t)(sp & (RING - 1)) * lanes;
+    lkey[j] = k;
+    lref[j] = r;
+    lcnt[j] = c;
+    sp++;
+  }
+  __device__ void pop(uint32_t &r, uint32_t &c, T &k) {
+    sp--;
+    if (sp < lo) {
+      lo = sp;
+      r = spill->ref[sp];
+      c = spill->cnt[sp];
+      k = spill->key[sp];
+    } else {
+      const uint32_t j = (uint32_t)(sp & (RING - 1)) * lanes;
+      r = lref[j];
+      c = lcnt[j];
+      k = lkey[j];
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------------------------- traversal
-// traverse() of blazert/bvh/accel.h:80-120 with an explicit intersector. The reference pops a node, tests it against
-// [min_hit_distance, current hit distance] and pushes far then near child; here the near child is tested at its
-// parent (the reference tests it at the very next pop, with the same hit distance) and the far child is pushed
-// with the hit-independent part of its test done, so the pop-time check reproduces the reference's decision
+// traverse() of blazert/bvh/accel.h:80-120 with an explicit intersector and stack. The reference pops a node, tests
+// it against [min_hit_distance, current hit distance] and pushes far then near child; here the near child is tested
+// at its parent (the reference tests it at the very next pop, with the same hit distance) and the far child is
+// pushed with the hit-independent part of its test done, so the pop-time check reproduces the reference's decision
 // exactly. Leaves test their primitives in leaf order (blazert/bvh/intersect.h:66-78); the hit distance used for
 // later node tests is the intersector's after a leaf reported a hit.
-template <typename T, typename Collection>
-__device__ bool traverse(const DeviceBVH<T> &bvh, const Collection &collection, const Ray<T> &ray, RayHit<T> &rayhit,
-                         typename Collection::intersector &intersector) {
+template <typename T, typename Collection, typename Stack>
+__device__ bool traverse_with(const DeviceBVH<T> &bvh, const Collection &collection, const Ray<T> &ray,
+                              RayHit<T> &rayhit, typename Collection::intersector &intersector, Stack &stack) {
 #pragma clang fp contract(off)
   using brt::exact::slab_exact;
   using Node = typename DeviceBVH<T>::Node;
@@ -233,9 +304,6 @@ __device__ bool traverse(const DeviceBVH<T> &bvh, const Collection &collection, 
   if (bvh.num_prims) {
     brt::exact::RayT<T> r = ray.slab;
     if (bvh.nan_boxes) r.sign &= ~brt::exact::RAY_NAN_FREE;
-    T skey[kUserStack];
-    uint32_t sref[kUserStack], scnt[kUserStack];
-    int sp = 0;
     T key = T(0);
     bool go = slab_exact(bvh.root, r, key) && !(hit_distance < key);
     uint32_t cur = bvh.root_ref, cnt = bvh.root_cnt;
@@ -248,12 +316,7 @@ __device__ bool traverse(const DeviceBVH<T> &bvh, const Collection &collection, 
         const bool p0 = slab_exact(nd.b, r, k0);
         const bool p1 = slab_exact(nd.b + 6, r, k1);
         const uint32_t c0 = nd.cnt[0] & brt::CNT_MASK, c1 = nd.cnt[1] & brt::CNT_MASK;
-        if (nc ? p0 : p1) {
-          skey[sp] = nc ? k0 : k1;
-          sref[sp] = nc ? nd.ref[0] : nd.ref[1];
-          scnt[sp] = nc ? c0 : c1;
-          sp++;
-        }
+        if (nc ? p0 : p1) stack.push(nc ? nd.ref[0] : nd.ref[1], nc ? c0 : c1, nc ? k0 : k1);
         if ((nc ? p1 : p0) && !(hit_distance < (nc ? k1 : k0))) {
           cur = nc ? nd.ref[1] : nd.ref[0];
           cnt = nc ? c1 : c0;
@@ -273,11 +336,9 @@ __device__ bool traverse(const DeviceBVH<T> &bvh, const Collection &collection, 
         }
       }
       go = false;
-      while (sp > 0) {
-        sp--;
-        if (!(hit_distance < skey[sp])) {
-          cur = sref[sp];
-          cnt = scnt[sp];
+      while (!stack.empty()) {
+        stack.pop(cur, cnt, key);
+        if (!(hit_distance < key)) {
           go = true;
           break;
         }
@@ -287,6 +348,15 @@ __device__ bool traverse(const DeviceBVH<T> &bvh, const Collection &collection, 
   const bool hit = intersector.hit_distance < ray.max_hit_distance;
   if (hit) post_traversal(intersector, rayhit);
   return hit;
+}
+
+// The same with a private-memory stack: callable from any kernel.
+template <typename T, typename Collection>
+__device__ bool traverse(const DeviceBVH<T> &bvh, const Collection &collection, const Ray<T> &ray, RayHit<T> &rayhit,
+                         typename Collection::intersector &intersector) {
+  StackArea<T> area;
+  PrivateStack<T> stack(&area);
+  return traverse_with(bvh, collection, ray, rayhit, intersector, stack);
 }
 
 // traverse(bvh, ray, rayhit) of blazert/bvh/accel.h:60-63: a fresh intersector of the collection.
@@ -311,19 +381,27 @@ struct Soa<float> {
   using hits = brt_hits_f32;
 };
 
-// One ray per thread. Hits follow brt_hits: a hit writes the RayHit fields and geom_id; a miss writes t = max,
-// u = v = 0, ids BRT_INVALID_ID, normal 0. u / v / geom_id / normal pointers may be NULL.
+// One ray per thread, blocks of kBlock threads, the newest kRing stack entries of each ray in LDS. Hits follow
+// brt_hits: a hit writes the RayHit fields and geom_id; a miss writes t = max, u = v = 0, ids BRT_INVALID_ID,
+// normal 0. u / v / geom_id / normal pointers may be NULL.
+constexpr uint32_t kBlock = 256;
+constexpr int kRing = 8;
 template <typename T, typename Collection>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kBlock)
     intersect_kernel(DeviceBVH<T> bvh, Collection collection, typename Soa<T>::rays rays,
                      typename Soa<T>::hits hits, uint64_t n, uint32_t any_hit, uint32_t geom_id) {
+  __shared__ T lkey[kRing * kBlock];
+  __shared__ uint32_t lref[kRing * kBlock], lcnt[kRing * kBlock];
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Ray<T> ray(Vec3r<T>(rays.ox[i], rays.oy[i], rays.oz[i]), Vec3r<T>(rays.dx[i], rays.dy[i], rays.dz[i]),
                    rays.tmin ? rays.tmin[i] : T(0), rays.tmax ? rays.tmax[i] : std::numeric_limits<T>::max(),
                    Ray<T>::CullBackFace::no, any_hit ? Ray<T>::AnyHit::yes : Ray<T>::AnyHit::no);
   RayHit<T> rh;
-  const bool hit = traverse(bvh, collection, ray, rh);
+  typename Collection::intersector intersector(collection);
+  StackArea<T> spill;
+  RingStack<T, kRing> stack{lkey + threadIdx.x, lref + threadIdx.x, lcnt + threadIdx.x, kBlock, &spill};
+  const bool hit = traverse_with(bvh, collection, ray, rh, intersector, stack);
   hits.t[i] = hit ? rh.hit_distance : std::numeric_limits<T>::max();
   if (hits.u) hits.u[i] = hit ? rh.uv[0] : T(0);
   if (hits.v) hits.v[i] = hit ? rh.uv[1] : T(0);
@@ -342,9 +420,9 @@ hipError_t intersect(const DeviceBVH<T> &bvh, const Collection &collection, cons
                      const typename Soa<T>::hits &hits, uint64_t n, bool any_hit = false, unsigned int geom_id = 0,
                      hipStream_t stream = nullptr) {
   if (n == 0) return hipSuccess;
-  const uint64_t blocks = (n + 255) / 256;
+  const uint64_t blocks = (n + kBlock - 1) / kBlock;
   if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((intersect_kernel<T, Collection>), dim3((uint32_t)blocks), dim3(256), 0, stream, bvh,
+  hipLaunchKernelGGL((intersect_kernel<T, Collection>), dim3((uint32_t)blocks), dim3(kBlock), 0, stream, bvh,
                      collection, rays, hits, n, any_hit ? 1u : 0u, geom_id);
   return hipGetLastError();
 }
