@@ -104,7 +104,7 @@ class BvhView(C.Structure):  # brt_bvh_view: device view of a user collection's 
     _fields_ = [("nodes", C.c_void_p), ("prim_ids", C.c_void_p), ("root_min", C.c_double * 3),
                 ("root_max", C.c_double * 3), ("root_ref", C.c_uint32), ("root_cnt", C.c_uint32),
                 ("num_nodes", C.c_uint32), ("num_prims", C.c_uint32), ("depth", C.c_uint32),
-                ("nan_boxes", C.c_uint32), ("real_bytes", C.c_uint32), ("reserved", C.c_uint32)]
+                ("irregular_boxes", C.c_uint32), ("real_bytes", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 # every entry point declared in include/blazert_amd.h: name -> (restype, argtypes)

@@ -753,7 +753,7 @@ template <bool SPLIT, class R>
 __device__ __forceinline__ void start_ray(const BvhView &bvh, const RaysViewT<R> &rays, uint64_t i, double S,
                                           RayT<R> &r, Ray32 &q, HitT<R> &s) {
   make_ray(rays, i, r);
-  if (bvh.nan_boxes) r.sign &= ~RAY_NAN_FREE;
+  if (bvh.irregular_boxes) r.sign &= ~RAY_NAN_FREE;
   if constexpr (SPLIT) make_ray32(r, S, q);
   s.h = r.tmax;  // prepare_traversal
   s.u = R(0);

@@ -130,7 +130,7 @@ struct BvhView {
   uint32_t n_prims;
   uint32_t n_nodes;
   uint32_t depth;             // deepest level of the built tree
-  uint32_t nan_boxes;         // some node box coordinate is NaN (disables the NaN-free slab path)
+  uint32_t irregular_boxes;   // some node box has a NaN coordinate or min > max (disables the NaN-free slab path)
 };
 
 // Structure-of-arrays rays / hits in device memory (mirrors brt_rays / brt_hits of the C ABI, and their FP32

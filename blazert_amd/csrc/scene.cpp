@@ -98,6 +98,8 @@ struct BoxesT {
   // every box coordinate is finite and none is -0.0: then every min / max the builder takes has one result
   // whatever the evaluation order, which the device builder relies on
   bool device_buildable = false;
+  // every box has min <= max on each axis (then so has every node box, a union of them)
+  bool ordered = false;
   void resize(uint32_t m) {
     n = m;
     lo.assign(3ull * m, R(0));
@@ -107,6 +109,8 @@ struct BoxesT {
   void classify() {
     auto plain = [](R x) { return std::isfinite(x) && !(x == R(0) && std::signbit(x)); };
     device_buildable = std::all_of(lo.begin(), lo.end(), plain) && std::all_of(hi.begin(), hi.end(), plain);
+    ordered = true;
+    for (size_t i = 0; i < lo.size(); i++) ordered = ordered && lo[i] <= hi[i];
   }
 };
 
@@ -469,9 +473,9 @@ void flatten(Geometry &g, GeomType type) {
     if (!T[i].leaf) internal[i] = ni++;
   g.n64.assign(ni, Node64{});
   double extent = 0.0;
-  bool ok = true, nan_boxes = false;
+  bool ok = true, irregular = false;
   auto note = [&](double v) {
-    if (v != v) nan_boxes = true;
+    if (v != v) irregular = true;
     if (!std::isfinite(v)) ok = false;
     else extent = std::max(extent, std::fabs(v));
   };
@@ -494,6 +498,7 @@ void flatten(Geometry &g, GeomType type) {
         a.b[6 * c + 3 + k] = ch.max[k];
         note(ch.min[k]);
         note(ch.max[k]);
+        if (ch.min[k] > ch.max[k]) irregular = true;
       }
       child_ref(T[i].children[c], a.ref[c], a.cnt[c]);
     }
@@ -569,7 +574,7 @@ void flatten(Geometry &g, GeomType type) {
   v.n_prims = g.boxes.n;
   v.n_nodes = ni;
   v.depth = g.stats.max_tree_depth;
-  v.nan_boxes = nan_boxes ? 1u : 0u;
+  v.irregular_boxes = irregular ? 1u : 0u;
   g.extent = extent;
   g.extent_ok = ok && extent < 1e30;
 
@@ -674,9 +679,9 @@ void flatten_f32(GeometryF &g, GeomType type) {
   }
   BvhView &v = g.view;
   v = BvhView{};
-  bool nan_boxes = false;
+  bool irregular = false;
   for (const auto &x : T)
-    for (int k = 0; k < 3; k++) nan_boxes = nan_boxes || x.min[k] != x.min[k] || x.max[k] != x.max[k];
+    for (int k = 0; k < 3; k++) irregular = irregular || x.min[k] != x.min[k] || x.max[k] != x.max[k] || x.min[k] > x.max[k];
   for (int k = 0; k < 3; k++) {
     v.root_min[k] = T[0].min[k];
     v.root_max[k] = T[0].max[k];
@@ -687,7 +692,7 @@ void flatten_f32(GeometryF &g, GeomType type) {
   v.n_prims = g.boxes.n;
   v.n_nodes = ni;
   v.depth = g.stats.max_tree_depth;
-  v.nan_boxes = nan_boxes ? 1u : 0u;
+  v.irregular_boxes = irregular ? 1u : 0u;
   g.extent_ok = false;  // no split precision for FP32 scenes
   const uint32_t n = g.boxes.n;
   auto fill = [&](auto *r, auto &&one) {
@@ -1059,7 +1064,7 @@ int commit_on_device(brt_scene *s, hipStream_t st) {
     v.n_prims = g.boxes.n;
     v.n_nodes = r.num_n64;
     v.depth = r.max_depth;
-    v.nan_boxes = 0;
+    v.irregular_boxes = g.boxes.ordered ? 0u : 1u;  // finite boxes (device_buildable): only the order can fail
     g.view = v;
     g.extent = extent;
     g.extent_ok = extent < 1e30;
@@ -1704,7 +1709,7 @@ int bvh_upload(brt_bvh *b, GeometryT<R> &g) {
     v.root_ref = g.view.root_ref;
     v.root_cnt = g.view.root_cnt;
     v.num_nodes = g.view.n_nodes;
-    v.nan_boxes = g.view.nan_boxes;
+    v.irregular_boxes = g.view.irregular_boxes;
   }
   b->view = v;
   b->device = dev;

@@ -78,8 +78,8 @@ __device__ __forceinline__ void setup_ray(const R org[3], const R raw[3], R tmin
   r.tmin = tmin;
   r.tmax = tmax;
   // With a finite origin, finite non-zero direction components (so 1/d is finite and non-zero), a non-NaN tmin and
-  // node boxes free of NaN (the caller clears the flag otherwise), (b - o) * inv is never NaN: the slab chains
-  // reduce to plain max/min.
+  // node boxes free of NaN (the caller clears the flag otherwise, also for boxes with min > max), (b - o) * inv is
+  // never NaN: the slab chains reduce to plain max/min.
   bool ok = r.tmin == r.tmin;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -89,23 +89,59 @@ __device__ __forceinline__ void setup_ray(const R org[3], const R raw[3], R tmin
   if (ok) r.sign |= RAY_NAN_FREE;
 }
 
+// max / min of two values known not to be NaN. fmax / fmin in IEEE mode first quiet every operand the compiler
+// cannot prove canonical, which it cannot for values computed in an earlier basic block: one extra v_max per
+// operand. On non-NaN operands the bare instructions return what fmax / fmin return.
+__device__ __forceinline__ double max_nn(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double min_nn(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float max_nn(float a, float b) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float min_nn(float a, float b) {
+  float r;
+  asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // Exact slab test of one child box (blazert/bvh/intersect.h:32-51 in the scene type), split into its hit-distance
 // independent part. Returns false when the box fails for every hit distance; otherwise key is the value the current
 // hit distance h must not be below (pass iff !(h < key)); key = -inf when a NaN far-plane term removes h from the
 // reference's min chain. b = {min x, y, z, max x, y, z}.
+//
+// RAY_NAN_FREE requires, besides the ray's own conditions, a tree whose boxes all have min <= max and no NaN (the
+// caller clears the bit otherwise). Then the reference's near-plane distance (b_near - o) * inv, with the plane
+// chosen by the sign of the direction, is the smaller of the two planes' distances and the far one the larger:
+// rounding is monotone, so for inv > 0 (b_min - o) * inv <= (b_max - o) * inv, and the order flips for inv < 0.
+// min / max of the two computed distances thus pick exactly the reference's values (a zero's sign aside, which no
+// comparison sees), without selecting the planes by sign first.
 template <class R>
 __device__ __forceinline__ bool slab_exact(const R *b, const RayT<R> &r, R &key) {
 #pragma clang fp contract(off)
   const R L1 = R(1) + R(4) * Real<R>::eps;  // far-plane widening of blazert/bvh/intersect.h:40
-  if (r.sign & RAY_NAN_FREE) {
-    R A = r.tmin, M = std::numeric_limits<R>::infinity();
+  // both planes' distances; the reference's near / far terms are these values, chosen by the direction's sign
+  R t0[3], t1[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const bool s = (r.sign >> i) & 1u;
-      const R nearp = s ? b[3 + i] : b[i];
-      const R farp = s ? b[i] : b[3 + i];
-      A = fmax(A, (nearp - r.o[i]) * r.inv[i]);
-      M = fmin(M, ((farp - r.o[i]) * r.inv[i]) * L1);
+  for (int i = 0; i < 3; i++) {
+    t0[i] = (b[i] - r.o[i]) * r.inv[i];
+    t1[i] = (b[3 + i] - r.o[i]) * r.inv[i];
+  }
+  if (r.sign & RAY_NAN_FREE) {
+    R A = max_nn(r.tmin, min_nn(t0[0], t1[0]));
+    R M = max_nn(t0[0], t1[0]) * L1;  // the reference's min chain starts from +inf
+#pragma unroll
+    for (int i = 1; i < 3; i++) {
+      A = max_nn(A, min_nn(t0[i], t1[i]));
+      M = min_nn(M, max_nn(t0[i], t1[i]) * L1);
     }
     key = A;
     return A <= M;
@@ -115,10 +151,8 @@ __device__ __forceinline__ bool slab_exact(const R *b, const RayT<R> &r, R &key)
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const bool s = (r.sign >> i) & 1u;
-    const R nearp = s ? b[3 + i] : b[i];
-    const R farp = s ? b[i] : b[3 + i];
-    const R mn = (nearp - r.o[i]) * r.inv[i];
-    m[i] = ((farp - r.o[i]) * r.inv[i]) * L1;
+    const R mn = s ? t1[i] : t0[i];
+    m[i] = (s ? t0[i] : t1[i]) * L1;
     acc = std_max(mn, acc);
   }
   const bool n0 = m[0] != m[0], n1 = m[1] != m[1], n2 = m[2] != m[2];

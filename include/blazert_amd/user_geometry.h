@@ -191,7 +191,7 @@ struct DeviceBVH {
   T root[6] = {};                      // root box: min xyz, max xyz
   uint32_t root_ref = 0, root_cnt = 0;
   uint32_t num_prims = 0;
-  uint32_t nan_boxes = 0;
+  uint32_t irregular_boxes = 0;  // some box has a NaN coordinate or min > max
 
   // from the C ABI's view; throws if the view's real type is not T
   static DeviceBVH from_view(const brt_bvh_view &v) {
@@ -206,7 +206,7 @@ struct DeviceBVH {
     b.root_ref = v.root_ref;
     b.root_cnt = v.root_cnt;
     b.num_prims = v.num_prims;
-    b.nan_boxes = v.nan_boxes;
+    b.irregular_boxes = v.irregular_boxes;
     return b;
   }
 };
@@ -303,7 +303,7 @@ __device__ bool traverse_with(const DeviceBVH<T> &bvh, const Collection &collect
   prepare_traversal(intersector, ray);
   if (bvh.num_prims) {
     brt::exact::RayT<T> r = ray.slab;
-    if (bvh.nan_boxes) r.sign &= ~brt::exact::RAY_NAN_FREE;
+    if (bvh.irregular_boxes) r.sign &= ~brt::exact::RAY_NAN_FREE;
     T key = T(0);
     bool go = slab_exact(bvh.root, r, key) && !(hit_distance < key);
     uint32_t cur = bvh.root_ref, cnt = bvh.root_cnt;
