@@ -119,9 +119,18 @@ __device__ __forceinline__ bool isect_tri(const Rec &T, const RayT<typename Rec:
   const R ry = r.d[2] * c0 - r.d[0] * c2;
   const R rz = r.d[0] * c1 - r.d[1] * c0;
   const R *e1 = T.e1, *e2 = T.e2;
-  const R nx = e1[1] * e2[2] - e1[2] * e2[1];
-  const R ny = e1[2] * e2[0] - e1[0] * e2[2];
-  const R nz = e1[0] * e2[1] - e1[1] * e2[0];
+  R nx, ny, nz;  // cross(e1, e2)
+  if constexpr (Rec::has_normal) {
+    nx = T.n[0];
+    ny = T.n[1];
+    nz = T.n[2];
+  } else {
+    R n[3];
+    tri_normal(e1, e2, n);
+    nx = n[0];
+    ny = n[1];
+    nz = n[2];
+  }
   const R det = nx * r.d[0] + ny * r.d[1] + nz * r.d[2];
   const R abs_det = fabs(det);
   const R u = product_sign(rx * e2[0] + ry * e2[1] + rz * e2[2], det);

@@ -14,6 +14,14 @@
 
 namespace brt {
 
+// n = cross(e1, e2) of a triangle record, component order of the reference's test (trimesh.h:160)
+template <class R>
+inline __host__ __device__ void tri_normal(const R *e1, const R *e2, R *n) {
+  n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+  n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+  n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+}
+
 // USER_COLLECTION: a tree over a user collection's boxes (brt_bvh_build), with no primitive records
 enum GeomType : uint32_t { TRIANGLES = 0, SPHERES = 1, PLANES = 2, CYLINDERS = 3, NUM_TYPES = 4, USER_COLLECTION = 16 };
 
@@ -30,10 +38,13 @@ struct alignas(128) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 
-// Triangle record in leaf order: v0, e1 = v0 - v1, e2 = v2 - v0 (blazert/primitives/trimesh.h:26-27).
+// Triangle record in leaf order: v0, e1 = v0 - v1, e2 = v2 - v0 (blazert/primitives/trimesh.h:26-27). The FP64
+// record does not carry n = cross(e1, e2): the 16 more bytes per record cost more time (measured) than forming n
+// in every test, as the reference does.
 struct alignas(16) TriRec {
   using real = double;
   static constexpr uint32_t kind = TRIANGLES;
+  static constexpr bool has_normal = false;
   double v0[3];
   double e1[3];
   double e2[3];
@@ -73,13 +84,16 @@ static_assert(sizeof(CylRec) == 128, "CylRec size");
 
 // Scene<float> (the reference instantiated with T = float): Node32 records (FP32 boxes) and FP32 primitive
 // records, traversed and tested in FP32 throughout.
+// The FP32 record keeps n = cross(e1, e2) in what would be padding, formed at build time with the expression of the
+// reference's test (trimesh.h:160), so its bits are the ones the test would form.
 struct alignas(16) TriRecF {
   using real = float;
   static constexpr uint32_t kind = TRIANGLES;
+  static constexpr bool has_normal = true;
   float v0[3];
   float e1[3];
   float e2[3];
-  float pad[3];
+  float n[3];
 };
 static_assert(sizeof(TriRecF) == 48, "TriRecF size");
 

@@ -711,6 +711,7 @@ void flatten_f32(GeometryF &g, GeomType type) {
         r.e1[k] = v0[k] - v1[k];  // Triangle::c = a - b
         r.e2[k] = v2[k] - v0[k];  // Triangle::b = c - a
       }
+      tri_normal(r.e1, r.e2, r.n);
     });
     break;
   case SPHERES:
