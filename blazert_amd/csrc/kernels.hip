@@ -615,14 +615,53 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
                                           bool any_hit, HitT<typename Rec::real> &s, Trav &t,
                                           RingStack<SPLIT, typename Rec::real> &st) {
   using R = typename Rec::real;
-  if (!(t.cur & LEAF_BIT)) {
+  using Node = typename std::conditional<SPLIT, Node4,
+                                         typename std::conditional<sizeof(R) == 8, Node64, Node32>::type>::type;
+  // One fetch per lane, issued together: the internal record for lanes at an inner node, the primitive record for
+  // lanes at a leaf. Written as separate branches, each would wait for its own loads, and a wave holding both kinds
+  // of lanes would pay two memory round trips per step. The chunks both records have are loaded from a per-lane
+  // address in one set of instructions; the rest of the longer one follows in a short branch (from the same lines,
+  // already on their way), and the compute branches below run without waiting in between.
+  constexpr int NODE_Q = SPLIT ? 8 : (sizeof(R) == 8 ? 7 : 4);  // 16-byte chunks of the record used (Node64: 112 B)
+  constexpr int REC_Q = (int)(sizeof(Rec) / 16);
+  constexpr int COMMON = NODE_Q < REC_Q ? NODE_Q : REC_Q;
+  constexpr int QMAX = NODE_Q < REC_Q ? REC_Q : NODE_Q;
+  static_assert(sizeof(Rec) % 16 == 0 && sizeof(Node) >= 16 * NODE_Q, "record chunking");
+  const bool at_leaf = (t.cur & LEAF_BIT) != 0;
+  const bool fetch_rec = at_leaf && t.pos < t.cnt;
+  const uint32_t p = (t.cur & ~LEAF_BIT) + t.pos;
+  const Node *nodes;
+  if constexpr (SPLIT) nodes = reinterpret_cast<const Node *>(bvh.nodes4);
+  else if constexpr (sizeof(R) == 8) nodes = reinterpret_cast<const Node *>(bvh.nodes64);
+  else nodes = reinterpret_cast<const Node *>(bvh.nodes32);
+  using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
+  const u32x4 *src = at_leaf ? reinterpret_cast<const u32x4 *>(reinterpret_cast<const Rec *>(bvh.prims) + p)
+                             : reinterpret_cast<const u32x4 *>(nodes + t.cur);
+  u32x4 chunk[QMAX];
+  if (!at_leaf || fetch_rec) {
+#pragma unroll
+    for (int k = 0; k < COMMON; k++) chunk[k] = src[k];
+  }
+  if (!at_leaf) {
+#pragma unroll
+    for (int k = COMMON; k < NODE_Q; k++) chunk[k] = src[k];
+  } else if (fetch_rec) {
+#pragma unroll
+    for (int k = COMMON; k < REC_Q; k++) chunk[k] = src[k];
+  }
+  // an empty asm taking every chunk as an operand: the loads cannot be sunk into the compute branches below, so all
+  // of them are issued first and waited for once
+#pragma unroll
+  for (int k = 0; k < QMAX; k++) asm volatile("" : "+v"(chunk[k]));
+  if (!at_leaf) {
     bool near_ok, far_ok;
     uint32_t nref, ncnt, fref, fcnt;
     StackKey<SPLIT, R> fkey;
+    Node nd;
+    __builtin_memcpy(&nd, chunk, 16 * NODE_Q);
     if constexpr (SPLIT) {
       // up to four children per record, visited nearest entry first (the FP32 path needs no particular order:
       // it only has to test every primitive the FP64 path tests)
-      const Node4 nd = bvh.nodes4[t.cur];
       float k[4];
       uint32_t rf[4], cn[4];
 #pragma unroll
@@ -654,11 +693,6 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
       nref = ncnt = fref = fcnt = 0;
       fkey = 0.0f;
     } else {
-      using Node = typename std::conditional<sizeof(R) == 8, Node64, Node32>::type;
-      const Node *nodes;
-      if constexpr (sizeof(R) == 8) nodes = bvh.nodes64;
-      else nodes = bvh.nodes32;
-      const Node nd = nodes[t.cur];
       const uint32_t axis = nd.cnt[1] >> AXIS_SHIFT;
       const uint32_t nc = (r.sign >> axis) & 1u;
       R k0 = R(0), k1 = R(0);
@@ -684,10 +718,9 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
     // one primitive per step: a leaf's tests (blazert/bvh/intersect.h:66-78, in order, each against the hit
     // distance the previous one left) are spread over consecutive steps, so lanes at leaves and lanes at inner
     // nodes run branches of similar length instead of inner-node lanes idling through a whole leaf
-    if (t.pos < t.cnt) {
-      const Rec *recs = reinterpret_cast<const Rec *>(bvh.prims);
-      const uint32_t p = (t.cur & ~LEAF_BIT) + t.pos;
-      const Rec rec = recs[p];
+    if (fetch_rec) {
+      Rec rec;
+      __builtin_memcpy(&rec, chunk, sizeof(Rec));
       if (isect(rec, r, s, p)) {
         if (SPLIT) t.h32 = f32_up(s.h);
         if (any_hit) {
