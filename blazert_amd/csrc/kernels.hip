@@ -489,6 +489,10 @@ constexpr uint32_t ring_default() { return SPLIT ? 16u : 8u; }
 template <bool SPLIT, class R>
 using StackKey = typename std::conditional<SPLIT, float, R>::type;
 
+// LDS pointers of the ring stack carry the LDS address space explicitly: as generic pointers the compiler had to
+// resolve their address space at run time in places (flat accesses), which costs time on the coherent sets.
+#define BRT_LDS __attribute__((address_space(3)))
+
 template <bool SPLIT, class R = double>
 struct SpillArea {  // private (scratch) memory for entries pushed deeper than the ring
   using Key = StackKey<SPLIT, R>;
@@ -500,8 +504,8 @@ struct SpillArea {  // private (scratch) memory for entries pushed deeper than t
 template <bool SPLIT, class R = double>
 struct RingStack {
   using Key = StackKey<SPLIT, R>;
-  Key *lkey;  // this lane's slot 0 in LDS; slot j is at [64 * j]
-  uint32_t *lref, *lcnt;
+  BRT_LDS Key *lkey;  // this lane's slot 0 in LDS; slot j is at [64 * j]
+  BRT_LDS uint32_t *lref, *lcnt;
   SpillArea<SPLIT, R> *spill;
   int sp, lo;
   int ring;  // LDS slots per lane (power of two)
@@ -552,9 +556,11 @@ __device__ __forceinline__ void ring_bind(RingStack<SPLIT, R> &st, unsigned char
                                           uint32_t ring) {
   using Key = StackKey<SPLIT, R>;
   const uint32_t waves = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  Key *keys = reinterpret_cast<Key *>(smem);
-  uint32_t *refs = reinterpret_cast<uint32_t *>(keys + (size_t)waves * ring * 64);
-  uint32_t *cnts = refs + (size_t)waves * ring * 64;
+  BRT_LDS unsigned char *lds = (BRT_LDS unsigned char *)smem;
+  const size_t slots = (size_t)waves * ring * 64;
+  BRT_LDS Key *keys = (BRT_LDS Key *)lds;
+  BRT_LDS uint32_t *refs = (BRT_LDS uint32_t *)(lds + slots * sizeof(Key));
+  BRT_LDS uint32_t *cnts = (BRT_LDS uint32_t *)(lds + slots * (sizeof(Key) + 4));
   st.lkey = keys + (size_t)wave * ring * 64 + lane;
   st.lref = refs + (size_t)wave * ring * 64 + lane;
   st.lcnt = cnts + (size_t)wave * ring * 64 + lane;

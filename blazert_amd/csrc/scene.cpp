@@ -902,6 +902,12 @@ hipError_t new_device_scene(int dev, std::unique_ptr<DeviceScene> &out) {
   return hipSuccess;
 }
 
+// Condition of the split-precision path: finite coordinates below 1e30 (its error bound).
+int split_supported(const Geometry &g) {
+  if (!g.extent_ok) return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
+  return BRT_OK;
+}
+
 int device_error(hipError_t e, const char *what) {
   return fail(e == hipErrorOutOfMemory ? BRT_ERR_OUT_OF_MEMORY : BRT_ERR_DEVICE,
               std::string(what) + ": " + hipGetErrorString(e));
@@ -1509,8 +1515,8 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
   for (uint32_t t = 0; t < NUM_TYPES; t++) {
     const Geometry &g = s->geo[t];
     if (!g.present || g.boxes.n == 0) continue;
-    if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT && !g.extent_ok)
-      return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
+    if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT)
+      if (int rc = split_supported(g)) return rc;
     TraceLaunch cfg{L.precision, L.flags, L.block_size, first, L.schedule, L.refill, nullptr, order};
     cfg.ring = L.stack_ring;
     if (L.schedule == BRT_SCHEDULE_PERSISTENT)
@@ -1549,8 +1555,8 @@ int brt_traverse(const brt_scene *cs, uint32_t geom_type, const brt_rays *rays, 
   if (L.flags & ~BRT_FLAG_ANY_HIT) return fail(BRT_ERR_INVALID_ARGUMENT, "traverse takes only BRT_FLAG_ANY_HIT");
   const Geometry &g = s->geo[geom_type];
   if (g.boxes.n == 0) return BRT_OK;
-  if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT && !g.extent_ok)
-    return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
+  if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT)
+    if (int rc = split_supported(g)) return rc;
   if (int rc = upload(s)) return rc;
   RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
   HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz, nullptr};
@@ -1843,8 +1849,8 @@ int brt_batch_create(const brt_batch_item *items, uint32_t count, const brt_laun
     for (uint32_t t = 0; t < NUM_TYPES; t++) {
       const Geometry &g = s->geo[t];
       if (!g.present || g.boxes.n == 0) continue;
-      if (b->L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT && !g.extent_ok)
-        return fail(BRT_ERR_UNSUPPORTED, "split precision needs finite scene coordinates below 1e30");
+      if (b->L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT)
+        if (int rc = split_supported(g)) return rc;
       BatchJob j{};
       j.bvh = s->dev->views[t];
       j.rays = RaysView{it.rays.ox, it.rays.oy, it.rays.oz, it.rays.dx, it.rays.dy, it.rays.dz, it.rays.tmin,
