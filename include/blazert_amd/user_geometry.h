@@ -246,11 +246,15 @@ struct PrivateStack {
 
 // The newest RING entries of a lane in LDS (slot-major, lane-minor, so a wave's accesses spread over the banks),
 // older ones spilled to private memory, which only rays deeper than the ring touch. Entries [lo, sp) are in LDS,
-// [0, lo) in the spill area. RING is a power of two.
+// [0, lo) in the spill area. RING is a power of two. The LDS pointers carry the LDS address space: as generic
+// pointers the compiler merges the LDS and spill reads of a pop into flat loads.
+template <typename T>
+using LdsPtr = __attribute__((address_space(3))) T *;
+
 template <typename T, int RING>
 struct RingStack {
-  T *lkey;  // this lane's slot 0; slot j is `lanes` elements further
-  uint32_t *lref, *lcnt;
+  LdsPtr<T> lkey;  // this lane's slot 0; slot j is `lanes` elements further
+  LdsPtr<uint32_t> lref, lcnt;
   uint32_t lanes;
   StackArea<T> *spill;
   int sp = 0, lo = 0;
@@ -400,7 +404,8 @@ __global__ void __launch_bounds__(kBlock)
   RayHit<T> rh;
   typename Collection::intersector intersector(collection);
   StackArea<T> spill;
-  RingStack<T, kRing> stack{lkey + threadIdx.x, lref + threadIdx.x, lcnt + threadIdx.x, kBlock, &spill};
+  RingStack<T, kRing> stack{(LdsPtr<T>)(lkey + threadIdx.x), (LdsPtr<uint32_t>)(lref + threadIdx.x),
+                            (LdsPtr<uint32_t>)(lcnt + threadIdx.x), kBlock, &spill};
   const bool hit = traverse_with(bvh, collection, ray, rh, intersector, stack);
   hits.t[i] = hit ? rh.hit_distance : std::numeric_limits<T>::max();
   if (hits.u) hits.u[i] = hit ? rh.uv[0] : T(0);
