@@ -41,11 +41,18 @@ constexpr int stack_cap() { return SPLIT ? 128 : 64; }
 using namespace exact;
 using Ray = RayT<double>;
 
+// A pointer into device memory viewed in the global address space. Pointers loaded from memory (the batch kernel's
+// job table) are generic, and accesses through generic pointers are flat accesses; through this view they are global.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 template <class R>
 __device__ __forceinline__ void make_ray(const RaysViewT<R> &V, uint64_t i, RayT<R> &r) {
-  const R org[3] = {V.ox[i], V.oy[i], V.oz[i]};
-  const R raw[3] = {V.dx[i], V.dy[i], V.dz[i]};
-  setup_ray(org, raw, V.tmin ? V.tmin[i] : V.tmin0, V.tmax ? V.tmax[i] : V.tmax0, r);
+  const R org[3] = {gmem(V.ox)[i], gmem(V.oy)[i], gmem(V.oz)[i]};
+  const R raw[3] = {gmem(V.dx)[i], gmem(V.dy)[i], gmem(V.dz)[i]};
+  setup_ray(org, raw, V.tmin ? gmem(V.tmin)[i] : V.tmin0, V.tmax ? gmem(V.tmax)[i] : V.tmax0, r);
 }
 
 // Conservative FP32 slab test: per axis lower/upper bounds of the FP64 entry/exit distances.
@@ -640,9 +647,12 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
   if constexpr (SPLIT) nodes = reinterpret_cast<const Node *>(bvh.nodes4);
   else if constexpr (sizeof(R) == 8) nodes = reinterpret_cast<const Node *>(bvh.nodes64);
   else nodes = reinterpret_cast<const Node *>(bvh.nodes32);
+  // The loads go through a pointer in the global address space: the batch kernel's tree pointers come from its job
+  // table in memory, and as generic pointers every access through them would be a flat access.
   using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
-  const u32x4 *src = at_leaf ? reinterpret_cast<const u32x4 *>(reinterpret_cast<const Rec *>(bvh.prims) + p)
-                             : reinterpret_cast<const u32x4 *>(nodes + t.cur);
+  using GlobalChunk = const __attribute__((address_space(1))) u32x4;
+  GlobalChunk *src = at_leaf ? (GlobalChunk *)(reinterpret_cast<const Rec *>(bvh.prims) + p)
+                             : (GlobalChunk *)(nodes + t.cur);
   u32x4 chunk[QMAX];
   if (!at_leaf || fetch_rec) {
 #pragma unroll
@@ -766,33 +776,33 @@ __device__ __forceinline__ void finish_ray(const BvhView &bvh, const HitsViewT<t
   bool write = false;
   if (hit) {
     if (is_tri) write = true;
-    else write = s.h < ((first && !(flags & FLAG_INOUT)) ? Real<R>::max : hits.t[i]);
+    else write = s.h < ((first && !(flags & FLAG_INOUT)) ? Real<R>::max : gmem(hits.t)[i]);
   }
   if (write) {
-    const uint32_t pid = bvh.prim_ids[s.leaf_idx];
-    if (hits.merged) hits.merged[i] = 1u;
-    hits.t[i] = s.h;
-    if (hits.u) hits.u[i] = s.u;
-    if (hits.v) hits.v[i] = s.v;
-    hits.prim_id[i] = pid;
-    if (hits.geom_id) hits.geom_id[i] = bvh.geom_id;
+    const uint32_t pid = gmem(bvh.prim_ids)[s.leaf_idx];
+    if (hits.merged) gmem(hits.merged)[i] = 1u;
+    gmem(hits.t)[i] = s.h;
+    if (hits.u) gmem(hits.u)[i] = s.u;
+    if (hits.v) gmem(hits.v)[i] = s.v;
+    gmem(hits.prim_id)[i] = pid;
+    if (hits.geom_id) gmem(hits.geom_id)[i] = bvh.geom_id;
     if (hits.nx) {
       R nrm[3];
       post_normal<Rec>(bvh, r, s, pid, nrm);
-      hits.nx[i] = nrm[0];
-      hits.ny[i] = nrm[1];
-      hits.nz[i] = nrm[2];
+      gmem(hits.nx)[i] = nrm[0];
+      gmem(hits.ny)[i] = nrm[1];
+      gmem(hits.nz)[i] = nrm[2];
     }
   } else if (first && !(flags & FLAG_INOUT)) {
-    hits.t[i] = Real<R>::max;
-    if (hits.u) hits.u[i] = R(0);
-    if (hits.v) hits.v[i] = R(0);
-    hits.prim_id[i] = INVALID_ID;
-    if (hits.geom_id) hits.geom_id[i] = INVALID_ID;
+    gmem(hits.t)[i] = Real<R>::max;
+    if (hits.u) gmem(hits.u)[i] = R(0);
+    if (hits.v) gmem(hits.v)[i] = R(0);
+    gmem(hits.prim_id)[i] = INVALID_ID;
+    if (hits.geom_id) gmem(hits.geom_id)[i] = INVALID_ID;
     if (hits.nx) {
-      hits.nx[i] = R(0);
-      hits.ny[i] = R(0);
-      hits.nz[i] = R(0);
+      gmem(hits.nx)[i] = R(0);
+      gmem(hits.ny)[i] = R(0);
+      gmem(hits.nz)[i] = R(0);
     }
   }
 }
@@ -966,6 +976,7 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
   bool exhausted = false;  // wave-uniform
   uint64_t idx = 0;
   uint32_t j = 0;
+  BvhView bv{};  // this lane's job's tree, kept across steps (the step reads only its node and record pointers)
   WavePool pool;
   for (;;) {
     if (!exhausted) {
@@ -981,18 +992,19 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
           const uint64_t local = g - J.offset;
           idx = J.order ? (uint64_t)J.order[local] : local;
           has_ray = true;
-          start_ray<SPLIT>(J.bvh, J.rays, idx, J.extent, r, q, s);
-          trav_begin<SPLIT>(J.bvh, r, q, s, t);
+          bv = J.bvh;
+          start_ray<SPLIT>(bv, J.rays, idx, J.extent, r, q, s);
+          trav_begin<SPLIT>(bv, r, q, s, t);
           st.reset();
         }
       }
     }
     if (!__ballot(has_ray)) break;
     if (has_ray) {
-      const BatchJob &J = jobs[j];
-      if (t.active) trav_step<Rec, SPLIT>(J.bvh, r, q, any_hit, s, t, st);
+      if (t.active) trav_step<Rec, SPLIT>(bv, r, q, any_hit, s, t, st);
       if (!t.active) {
-        finish_ray<Rec>(J.bvh, J.hits, idx, r, s, flags, J.first, J.is_tri);
+        const BatchJob &J = jobs[j];
+        finish_ray<Rec>(bv, J.hits, idx, r, s, flags, J.first, J.is_tri);
         has_ray = false;
       }
     }
