@@ -21,6 +21,7 @@
 #include <float.h>
 #include <stdint.h>
 
+#include "../../include/blazert_amd/user_geometry.h"  // RingStack: LDS stack ring with a private spill area
 #include "engine.h"
 
 namespace brt {
@@ -309,14 +310,17 @@ __device__ __forceinline__ double box_gap(const double *lo, const double *hi, co
   return __builtin_sqrt(s);
 }
 
-constexpr int CLOSEST_STACK = 64;  // trees are at most 63 levels deep on the device
+// Traversal stack of the closest-surface query: the newest CLOSEST_RING entries of each point in LDS, deeper ones
+// in a private spill area (trees are at most 63 levels deep on the device, so 64 entries always suffice).
+constexpr int CLOSEST_RING = 8;
+constexpr uint32_t CLOSEST_BLOCK = 128;
+using ClosestStack = blazert_amd::RingStack<double, CLOSEST_RING>;
 
 // Depth-first search of one BVH, nearer child first. `bound` is the best distance of earlier geometry types.
 template <int TYPE>
-__device__ void closest_in_bvh(const BvhView &bvh, const double *p, double margin, double bound, Best &best) {
-  uint32_t stack_ref[CLOSEST_STACK], stack_cnt[CLOSEST_STACK];
-  double stack_key[CLOSEST_STACK];
-  int sp = 0;
+__device__ void closest_in_bvh(const BvhView &bvh, const double *p, double margin, double bound, Best &best,
+                               ClosestStack &st) {
+  st.sp = st.lo = 0;
   auto limit = [&]() { return best.d < bound ? best.d : bound; };
   if (box_gap(bvh.root_min, bvh.root_max, p) - margin > limit()) return;
   uint32_t ref = bvh.root_ref, cnt = bvh.root_cnt;
@@ -337,10 +341,7 @@ __device__ void closest_in_bvh(const BvhView &bvh, const double *p, double margi
       const bool ok0 = !(g0 - margin > lim), ok1 = !(g1 - margin > lim);
       if (ok0 && ok1) {
         const bool first0 = !(g1 < g0);
-        stack_ref[sp] = first0 ? r1 : r0;
-        stack_cnt[sp] = first0 ? c1 : c0;
-        stack_key[sp] = first0 ? g1 : g0;
-        sp++;
+        st.push(first0 ? r1 : r0, first0 ? c1 : c0, first0 ? g1 : g0);
         ref = first0 ? r0 : r1;
         cnt = first0 ? c0 : c1;
         continue;
@@ -353,11 +354,10 @@ __device__ void closest_in_bvh(const BvhView &bvh, const double *p, double margi
     }
     // pop the next entry still within reach
     bool found = false;
-    while (sp > 0) {
-      sp--;
-      if (!(stack_key[sp] - margin > limit())) {
-        ref = stack_ref[sp];
-        cnt = stack_cnt[sp];
+    while (!st.empty()) {
+      double key;
+      st.pop(ref, cnt, key);
+      if (!(key - margin > limit())) {
         found = true;
         break;
       }
@@ -378,8 +378,13 @@ __device__ void closest_scan(const BvhView &bvh, const double *p, Best &best) {
 __global__ void k_closest_surface(ClosestViews V, const double *__restrict__ px, const double *__restrict__ py,
                                   const double *__restrict__ pz, uint64_t n, double *__restrict__ dist,
                                   uint32_t *__restrict__ prim, uint32_t *__restrict__ geom) {
+  __shared__ double lkey[CLOSEST_RING * CLOSEST_BLOCK];
+  __shared__ uint32_t lref[CLOSEST_RING * CLOSEST_BLOCK], lcnt[CLOSEST_RING * CLOSEST_BLOCK];
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  blazert_amd::StackArea<double> spill;
+  ClosestStack st{(blazert_amd::LdsPtr<double>)(lkey + threadIdx.x), (blazert_amd::LdsPtr<uint32_t>)(lref + threadIdx.x),
+                  (blazert_amd::LdsPtr<uint32_t>)(lcnt + threadIdx.x), CLOSEST_BLOCK, &spill};
   const double p[3] = {px[i], py[i], pz[i]};
   const double pm = fmax(fabs(p[0]), fmax(fabs(p[1]), fabs(p[2])));
   double gd = __builtin_inf();
@@ -389,8 +394,8 @@ __global__ void k_closest_surface(ClosestViews V, const double *__restrict__ px,
     if (!V.present[t] || bvh.n_prims == 0) continue;
     Best b{__builtin_inf(), INVALID_ID};
     const double margin = 1e-10 * (pm + V.extent[t]);
-    if (t == TRIANGLES) closest_in_bvh<TRIANGLES>(bvh, p, margin, gd, b);
-    else if (t == SPHERES) closest_in_bvh<SPHERES>(bvh, p, margin, gd, b);
+    if (t == TRIANGLES) closest_in_bvh<TRIANGLES>(bvh, p, margin, gd, b, st);
+    else if (t == SPHERES) closest_in_bvh<SPHERES>(bvh, p, margin, gd, b, st);
     else if (t == PLANES) closest_scan<PlaRec>(bvh, p, b);
     else closest_scan<CylRec>(bvh, p, b);
     if (b.d < gd) {  // a later type replaces only when strictly closer
@@ -429,7 +434,8 @@ hipError_t launch_distance_pairs(GeomType type, const BvhView &bvh, const uint32
 hipError_t launch_closest_surface(const ClosestViews &views, const double *px, const double *py, const double *pz,
                                   uint64_t n, double *dist, uint32_t *prim, uint32_t *geom, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  k_closest_surface<<<(uint32_t)((n + 127) / 128), 128, 0, stream>>>(views, px, py, pz, n, dist, prim, geom);
+  k_closest_surface<<<(uint32_t)((n + CLOSEST_BLOCK - 1) / CLOSEST_BLOCK), CLOSEST_BLOCK, 0, stream>>>(
+      views, px, py, pz, n, dist, prim, geom);
   return hipGetLastError();
 }
 
