@@ -280,6 +280,17 @@ __global__ void k_distance_pairs(BvhView bvh, const uint32_t *__restrict__ inv, 
   out[i] = d;
 }
 
+__global__ void k_leaf_vertices(const uint32_t *__restrict__ order, const uint32_t *__restrict__ F,
+                                const double *__restrict__ V, uint32_t n, double *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t *f = F + 3ull * order[i];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[9ull * i + 3 * c + k] = V[3ull * f[c] + k];
+}
+
 __global__ void k_inverse_order(const uint32_t *__restrict__ order, uint32_t n, uint32_t *__restrict__ inv) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) inv[order[i]] = i;
@@ -318,8 +329,8 @@ using ClosestStack = blazert_amd::RingStack<double, CLOSEST_RING>;
 
 // Depth-first search of one BVH, nearer child first. `bound` is the best distance of earlier geometry types.
 template <int TYPE>
-__device__ void closest_in_bvh(const BvhView &bvh, const double *p, double margin, double bound, Best &best,
-                               ClosestStack &st) {
+__device__ void closest_in_bvh(const BvhView &bvh, const double *leaf_vertices, const double *p, double margin,
+                               double bound, Best &best, ClosestStack &st) {
   st.sp = st.lo = 0;
   auto limit = [&]() { return best.d < bound ? best.d : bound; };
   if (box_gap(bvh.root_min, bvh.root_max, p) - margin > limit()) return;
@@ -328,8 +339,10 @@ __device__ void closest_in_bvh(const BvhView &bvh, const double *p, double margi
     if (ref & LEAF_BIT) {
       const uint32_t first = ref & ~LEAF_BIT;
       for (uint32_t k = 0; k < cnt; k++) {
+        // triangles read their vertices from the leaf-order copy: one gather instead of id -> face -> vertices
         const uint32_t id = bvh.prim_ids[first + k];
-        const double d = (TYPE == TRIANGLES) ? tri_distance_by_id(bvh, id, p)
+        const double *lv = leaf_vertices + 9ull * (first + k);
+        const double d = (TYPE == TRIANGLES) ? triangle_distance(lv, lv + 3, lv + 6, p)
                                              : sphere_distance(reinterpret_cast<const SphRec *>(bvh.prims)[first + k], p);
         consider(best, d, id);
       }
@@ -394,8 +407,8 @@ __global__ void k_closest_surface(ClosestViews V, const double *__restrict__ px,
     if (!V.present[t] || bvh.n_prims == 0) continue;
     Best b{__builtin_inf(), INVALID_ID};
     const double margin = 1e-10 * (pm + V.extent[t]);
-    if (t == TRIANGLES) closest_in_bvh<TRIANGLES>(bvh, p, margin, gd, b, st);
-    else if (t == SPHERES) closest_in_bvh<SPHERES>(bvh, p, margin, gd, b, st);
+    if (t == TRIANGLES) closest_in_bvh<TRIANGLES>(bvh, V.leaf_vertices, p, margin, gd, b, st);
+    else if (t == SPHERES) closest_in_bvh<SPHERES>(bvh, nullptr, p, margin, gd, b, st);
     else if (t == PLANES) closest_scan<PlaRec>(bvh, p, b);
     else closest_scan<CylRec>(bvh, p, b);
     if (b.d < gd) {  // a later type replaces only when strictly closer
@@ -414,6 +427,13 @@ __global__ void k_closest_surface(ClosestViews V, const double *__restrict__ px,
 hipError_t launch_inverse_order(const uint32_t *order, uint32_t n, uint32_t *inv, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   k_inverse_order<<<(n + 255) / 256, 256, 0, stream>>>(order, n, inv);
+  return hipGetLastError();
+}
+
+hipError_t launch_leaf_vertices(const uint32_t *order, const uint32_t *faces, const double *vertices, uint32_t n,
+                                double *out, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  k_leaf_vertices<<<(n + 255) / 256, 256, 0, stream>>>(order, faces, vertices, n, out);
   return hipGetLastError();
 }
 

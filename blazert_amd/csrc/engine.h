@@ -105,7 +105,11 @@ struct ClosestViews {
   BvhView view[NUM_TYPES];
   double extent[NUM_TYPES];
   uint32_t present[NUM_TYPES];
+  const double *leaf_vertices;  // triangles in leaf order: v0, v1, v2 (9 doubles each)
 };
+// Gathers the triangles' vertices in leaf order (9 doubles per triangle) for the closest-surface query.
+hipError_t launch_leaf_vertices(const uint32_t *order, const uint32_t *faces, const double *vertices, uint32_t n,
+                                double *out, hipStream_t stream);
 hipError_t launch_closest_surface(const ClosestViews &views, const double *px, const double *py, const double *pz,
                                   uint64_t n, double *dist, uint32_t *prim, uint32_t *geom, hipStream_t stream);
 

@@ -794,6 +794,7 @@ struct DeviceScene {
   std::atomic<uint32_t> next_counter{0};
   std::mutex inv_mu;
   uint32_t *inv_order[NUM_TYPES]{};  // primitive id -> leaf position, made on the first distance query
+  double *leaf_vertices = nullptr;    // triangles' vertices in leaf order, made on the first closest-surface query
   std::mutex sort_mu;
   std::vector<std::pair<hipStream_t, std::unique_ptr<SortWorkspace>>> sorts;  // one ray-sort workspace per stream
   ~DeviceScene() {
@@ -1390,6 +1391,7 @@ int brt_distance_to_surface(const brt_scene *cs, uint32_t type, const brt_points
       if (e != hipSuccess) return device_error(e, "inverse order allocation");
       s->dev->allocs.push_back(p);
       e = launch_inverse_order(v.prim_ids, g.boxes.n, static_cast<uint32_t *>(p), st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);  // complete before a query on another stream reads it
       if (e != hipSuccess) return device_error(e, "launch failed");
       slot = static_cast<uint32_t *>(p);
     }
@@ -1417,6 +1419,23 @@ int brt_closest_surface(const brt_scene *cs, const brt_points *points, uint64_t 
     if (!cv.present[t]) continue;
     cv.view[t] = s->dev->views[t];
     cv.extent[t] = g.extent_ok ? g.extent : INFINITY;
+  }
+  if (cv.present[TRIANGLES]) {
+    std::lock_guard<std::mutex> lk(s->dev->inv_mu);
+    if (!s->dev->leaf_vertices) {
+      const Geometry &g = s->geo[TRIANGLES];
+      const BvhView &v = s->dev->views[TRIANGLES];
+      void *p = nullptr;
+      hipError_t e = hipMalloc(&p, (size_t)g.boxes.n * 9 * sizeof(double));
+      if (e != hipSuccess) return device_error(e, "leaf-order vertex allocation");
+      s->dev->allocs.push_back(p);
+      const hipStream_t st = static_cast<hipStream_t>(stream);
+      e = launch_leaf_vertices(v.prim_ids, v.faces, v.vertices, g.boxes.n, static_cast<double *>(p), st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);  // complete before a query on another stream reads it
+      if (e != hipSuccess) return device_error(e, "launch failed");
+      s->dev->leaf_vertices = static_cast<double *>(p);
+    }
+    cv.leaf_vertices = s->dev->leaf_vertices;
   }
   hipError_t e = launch_closest_surface(cv, points->x, points->y, points->z, n, distance, prim_id, geom_id,
                                         static_cast<hipStream_t>(stream));

@@ -202,7 +202,9 @@ int brt_distance_to_surface(const brt_scene *scene, uint32_t geom_type, const br
 /* Closest surface of the whole scene for n device points: the smallest distance_to_surface over every primitive,
  * ties to the earlier geometry type (triangles, spheres, planes, cylinders) and then the lower primitive id; +inf
  * and BRT_INVALID_ID when no primitive gives a distance. Extension: the reference has no scene-level distance query
- * (docs/source/roadmap.rst). Asynchronous on `stream`. */
+ * (docs/source/roadmap.rst). Asynchronous on `stream`, except that the first call on a scene with triangles
+ * allocates (hipMalloc, 72 bytes per triangle) and fills a leaf-order copy of the triangles' vertices, then waits
+ * for it; do not graph-capture that first call. */
 int brt_closest_surface(const brt_scene *scene, const brt_points *points, uint64_t n, double *distance,
                         uint32_t *prim_id, uint32_t *geom_id, void *stream);
 
