@@ -259,6 +259,30 @@ def test_any_hit_matches_oracle(dev, mixed):
     assert_bit_exact(g, ref, ref["trace"]["overflow"] == 0)
 
 
+def test_inverted_boxes_take_the_exact_general_path(dev, oracle_mod):
+    """Spheres with a negative radius have inverted boxes (center - r > center + r). The slab test's min / max form
+    is exact only for boxes with min <= max, so the host flags such trees and the kernel takes the sign-selected
+    general form: hits must still equal the oracle bit for bit, in both precisions' hit ids."""
+    c, r = W.sphere_field(4096, seed=41)
+    r = np.where(W.Rng(42).uniform(len(r)) < 0.3, -r, r)
+    p, o = make_pair(oracle_mod, lambda s: s.add_spheres(c, r))
+    org, dirs = W.incoherent_rays(1 << 15, 43, 12.0)
+    ref = o.intersect_batch(org, dirs, trace=True)
+    ok = ref["trace"]["overflow"] == 0
+    g = gpu_intersect(p, org, dirs, dev)
+    assert (ref["prim_id"] != INVALID).sum() > 1000
+    assert_bit_exact(g, ref, ok)
+    g32 = gpu_intersect(p, org, dirs, dev, precision=1)
+    assert (g32["prim_id"][ok] == ref["prim_id"][ok]).mean() > 0.999
+    # the same scene built by the device builder (finite boxes: it builds them; the host flags the order)
+    from blazert_amd import Scene
+    d = Scene()
+    d.add_spheres(c, r)
+    d.commit(device=True)
+    assert d.build_stats(1)["built_on_device"] == 1
+    assert_bit_exact(gpu_intersect(d, org, dirs, dev), ref, ok)
+
+
 def test_degenerate_and_axis_parallel_rays(dev, oracle_mod):
     """Zero direction components (1/d = inf), origins on box planes (0*inf = NaN slab terms), -0.0, tmin/tmax."""
     v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1], [-1, 0, 0], [0, -1, 0]], float)

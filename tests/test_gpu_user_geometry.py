@@ -225,6 +225,19 @@ def test_user_boxes_bruteforce(dev, plugin):
     _assert_same(got, ref)
 
 
+def test_irregular_boxes_flag(dev):
+    """brt_bvh_view.irregular_boxes marks trees with an inverted (min > max) or NaN box, which the device traversal
+    then tests with the sign-selected general slab form."""
+    lo, hi, ctr = _boxes(100, 44)
+    assert BVH(box_min=lo, box_max=hi, centers=ctr).device_view().irregular_boxes == 0
+    lo2, hi2 = lo.copy(), hi.copy()
+    lo2[7], hi2[7] = hi[7], lo[7]
+    assert BVH(box_min=lo2, box_max=hi2, centers=ctr).device_view().irregular_boxes == 1
+    lo3 = lo.copy()
+    lo3[3, 1] = np.nan
+    assert BVH(box_min=lo3, box_max=hi, centers=ctr).device_view().irregular_boxes == 1
+
+
 def test_empty_collection(dev, plugin):
     z = np.zeros((0, 3))
     bvh = BVH(box_min=z, box_max=z, centers=z)
