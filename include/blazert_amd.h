@@ -375,7 +375,7 @@ typedef struct brt_bvh_view {
   uint32_t root_ref, root_cnt;
   uint32_t num_nodes, num_prims;
   uint32_t depth;
-  uint32_t irregular_boxes; /* some box has a NaN coordinate or min > max */
+  uint32_t irregular_boxes; /* some node box of the tree has a NaN coordinate or min > max */
   uint32_t real_bytes; /* 8: FP64 tree, 4: FP32 tree */
   uint32_t reserved;
 } brt_bvh_view;

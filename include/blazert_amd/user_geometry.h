@@ -191,7 +191,7 @@ struct DeviceBVH {
   T root[6] = {};                      // root box: min xyz, max xyz
   uint32_t root_ref = 0, root_cnt = 0;
   uint32_t num_prims = 0;
-  uint32_t irregular_boxes = 0;  // some box has a NaN coordinate or min > max
+  uint32_t irregular_boxes = 0;  // some node box has a NaN coordinate or min > max
 
   // from the C ABI's view; throws if the view's real type is not T
   static DeviceBVH from_view(const brt_bvh_view &v) {

@@ -226,16 +226,17 @@ def test_user_boxes_bruteforce(dev, plugin):
 
 
 def test_irregular_boxes_flag(dev):
-    """brt_bvh_view.irregular_boxes marks trees with an inverted (min > max) or NaN box, which the device traversal
-    then tests with the sign-selected general slab form."""
+    """brt_bvh_view.irregular_boxes marks trees with an inverted (min > max) or NaN node box, which the device
+    traversal then tests with the sign-selected general slab form. A node box is the union of its primitives' boxes,
+    so one primitive per leaf makes a primitive's box a node box."""
     lo, hi, ctr = _boxes(100, 44)
-    assert BVH(box_min=lo, box_max=hi, centers=ctr).device_view().irregular_boxes == 0
+    assert BVH(box_min=lo, box_max=hi, centers=ctr, min_leaf_primitives=1).device_view().irregular_boxes == 0
     lo2, hi2 = lo.copy(), hi.copy()
     lo2[7], hi2[7] = hi[7], lo[7]
-    assert BVH(box_min=lo2, box_max=hi2, centers=ctr).device_view().irregular_boxes == 1
+    assert BVH(box_min=lo2, box_max=hi2, centers=ctr, min_leaf_primitives=1).device_view().irregular_boxes == 1
     lo3 = lo.copy()
     lo3[3, 1] = np.nan
-    assert BVH(box_min=lo3, box_max=hi, centers=ctr).device_view().irregular_boxes == 1
+    assert BVH(box_min=lo3, box_max=hi, centers=ctr, min_leaf_primitives=1).device_view().irregular_boxes == 1
 
 
 def test_empty_collection(dev, plugin):
