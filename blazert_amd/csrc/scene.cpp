@@ -567,6 +567,7 @@ void flatten(Geometry &g, GeomType type) {
     v.root_max32[k] = f32_up(T[0].max[k]);
     note(T[0].min[k]);
     note(T[0].max[k]);
+    if (T[0].min[k] > T[0].max[k]) irregular = true;  // a root leaf's box is not among the children checked above
   }
   child_ref(0, v.root_ref, v.root_cnt);
   v.root_ref4 = (root4 == INVALID_ID) ? v.root_ref : root4;

@@ -281,6 +281,12 @@ def test_inverted_boxes_take_the_exact_general_path(dev, oracle_mod):
     d.commit(device=True)
     assert d.build_stats(1)["built_on_device"] == 1
     assert_bit_exact(gpu_intersect(d, org, dirs, dev), ref, ok)
+    # a root that is a leaf: one or two spheres with negative radii, so the root box itself is inverted
+    for cs, rs in (([[0.0, 0.0, 0.0]], [-1.0]), ([[0.0, 0.0, 0.0], [0.5, 0.0, 0.0]], [-1.0, -0.75])):
+        p1, o1 = make_pair(oracle_mod, lambda s: s.add_spheres(np.array(cs), np.array(rs)))
+        org1, dirs1 = W.incoherent_rays(4096, 45, 3.0)
+        ref1 = o1.intersect_batch(org1, dirs1)
+        assert_bit_exact(gpu_intersect(p1, org1, dirs1, dev), ref1)
 
 
 def test_degenerate_and_axis_parallel_rays(dev, oracle_mod):
