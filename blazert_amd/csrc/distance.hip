@@ -379,8 +379,16 @@ __device__ void closest_in_bvh(const BvhView &bvh, const double *leaf_vertices, 
   }
 }
 
+__device__ __forceinline__ double rec_distance(const SphRec &r, const double *p) { return sphere_distance(r, p); }
 __device__ __forceinline__ double rec_distance(const PlaRec &r, const double *p) { return plane_distance(r, p); }
 __device__ __forceinline__ double rec_distance(const CylRec &r, const double *p) { return cylinder_distance(r, p); }
+
+__device__ void closest_scan_triangles(const BvhView &bvh, const double *leaf_vertices, const double *p, Best &best) {
+  for (uint32_t k = 0; k < bvh.n_prims; k++) {
+    const double *lv = leaf_vertices + 9ull * k;
+    consider(best, triangle_distance(lv, lv + 3, lv + 6, p), bvh.prim_ids[k]);
+  }
+}
 
 template <class Rec>
 __device__ void closest_scan(const BvhView &bvh, const double *p, Best &best) {
@@ -407,8 +415,12 @@ __global__ void k_closest_surface(ClosestViews V, const double *__restrict__ px,
     if (!V.present[t] || bvh.n_prims == 0) continue;
     Best b{__builtin_inf(), INVALID_ID};
     const double margin = 1e-10 * (pm + V.extent[t]);
-    if (t == TRIANGLES) closest_in_bvh<TRIANGLES>(bvh, V.leaf_vertices, p, margin, gd, b, st);
-    else if (t == SPHERES) closest_in_bvh<SPHERES>(bvh, nullptr, p, margin, gd, b, st);
+    // box pruning needs boxes that contain their primitives: a tree with an inverted or NaN node box (a sphere with a
+    // negative radius) is scanned in full instead
+    if (t == TRIANGLES && !bvh.irregular_boxes) closest_in_bvh<TRIANGLES>(bvh, V.leaf_vertices, p, margin, gd, b, st);
+    else if (t == TRIANGLES) closest_scan_triangles(bvh, V.leaf_vertices, p, b);
+    else if (t == SPHERES && !bvh.irregular_boxes) closest_in_bvh<SPHERES>(bvh, nullptr, p, margin, gd, b, st);
+    else if (t == SPHERES) closest_scan<SphRec>(bvh, p, b);
     else if (t == PLANES) closest_scan<PlaRec>(bvh, p, b);
     else closest_scan<CylRec>(bvh, p, b);
     if (b.d < gd) {  // a later type replaces only when strictly closer

@@ -130,6 +130,29 @@ def test_closest_surface_icosphere(dev, oracle_mod, device_commit):
     assert np.isfinite(d).all()
 
 
+def test_closest_surface_inverted_sphere_boxes(dev, oracle_mod):
+    """Spheres with negative radii have inverted boxes, which box pruning cannot use: such a tree is scanned in full,
+    and the result must still be the brute-force minimum."""
+    from blazert_amd import Scene
+    c, r = W.sphere_field(3000, seed=46)
+    r = np.where(W.Rng(47).uniform(len(r)) < 0.3, -r, r)
+    V, F = W.icosphere(3)
+
+    def fill(s):
+        s.add_mesh(V * 4.0, F)
+        s.add_spheres(c, r)
+    g = Scene()
+    fill(g)
+    g.commit()
+    o = oracle_mod.Scene()
+    fill(o)
+    o.commit()
+    pts = W.incoherent_rays(2048, 48, 12.0)[0]
+    print("closest surface: inverted sphere boxes, 2048 points", flush=True)
+    d = _closest_check(g, o, pts, dev)
+    assert np.isfinite(d).all()
+
+
 def test_closest_surface_ties_and_empty(dev, oracle_mod):
     from blazert_amd import Scene
 
