@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(64) k_small(const SmallRoot *__restrict__ root
                                               const double *__restrict__ lo, const double *__restrict__ hi,
                                               const double *__restrict__ ctr, uint32_t nb, uint32_t min_leaf,
                                               uint32_t max_depth, brt_tree_node *__restrict__ tmp,
-                                              uint32_t *__restrict__ tmp_depth, uint32_t *__restrict__ sizes,
+                                              uint32_t *__restrict__ sizes,
                                               BuildStatsDev *__restrict__ stats) {
   __shared__ uint32_t e_idx[SMALL];
   __shared__ double e_lo[3][SMALL], e_hi[3][SMALL], e_ctr[3][SMALL];
@@ -501,7 +501,6 @@ __global__ void __launch_bounds__(64) k_small(const SmallRoot *__restrict__ root
     perm[k] = (uint16_t)k;
   }
   brt_tree_node *out = tmp + 2ull * R.begin;
-  uint32_t *out_depth = tmp_depth + 2ull * R.begin;
   BuildStatsDev st{0, 0, 0, 0, 0, 0xFFFFFFFFu};
   if (lane == 0) {
     stk[0][0] = 0;
@@ -549,7 +548,6 @@ __global__ void __launch_bounds__(64) k_small(const SmallRoot *__restrict__ root
       st.min_prims = n < st.min_prims ? n : st.min_prims;
       if (lane == 0) {
         out[self] = node;
-        out_depth[self] = d;
         if (parent != INVALID_ID) out[parent].children[1] = self;
       }
       continue;
@@ -636,7 +634,6 @@ __global__ void __launch_bounds__(64) k_small(const SmallRoot *__restrict__ root
     st.branches++;
     if (lane == 0) {
       out[self] = node;
-      out_depth[self] = d;
       if (parent != INVALID_ID) out[parent].children[1] = self;
       stk[sp][0] = mid;
       stk[sp][1] = e;
@@ -665,22 +662,19 @@ __global__ void __launch_bounds__(64) k_small(const SmallRoot *__restrict__ root
 // ------------------------------------------------------------------------------------------------ assembly
 struct TopPut {
   brt_tree_node node;
-  uint32_t pre, depth;
+  uint32_t pre, pad;
 };
 
-__global__ void k_put_top(const TopPut *__restrict__ tops, uint32_t m, brt_tree_node *__restrict__ tree,
-                          uint32_t *__restrict__ depth) {
+__global__ void k_put_top(const TopPut *__restrict__ tops, uint32_t m, brt_tree_node *__restrict__ tree) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   tree[tops[i].pre] = tops[i].node;
-  depth[tops[i].pre] = tops[i].depth;
 }
 
 // Moves each small subtree from its scratch slot to its pre-order position (base[s]), making child indices global.
 __global__ void k_reloc(const SmallRoot *__restrict__ roots, const uint32_t *__restrict__ sizes,
                         const uint32_t *__restrict__ bases, const brt_tree_node *__restrict__ tmp,
-                        const uint32_t *__restrict__ tmp_depth, brt_tree_node *__restrict__ tree,
-                        uint32_t *__restrict__ depth) {
+                        brt_tree_node *__restrict__ tree) {
   const uint32_t s = blockIdx.x, base = bases[s];
   const size_t from = 2ull * roots[s].begin;
   for (uint32_t j = threadIdx.x; j < sizes[s]; j += blockDim.x) {
@@ -690,42 +684,69 @@ __global__ void k_reloc(const SmallRoot *__restrict__ roots, const uint32_t *__r
       nd.children[1] += base;
     }
     tree[base + j] = nd;
-    depth[base + j] = tmp_depth[from + j];
   }
 }
 
 // ------------------------------------------------------------------------------------------------ flatten
-// Every internal node becomes one Node64 (numbered in pre-order among internal nodes); every internal node at even
-// depth becomes one Node4 (its slots: leaf children, and the two children of internal children), numbered as the
-// host's depth-first allocation does: a Node4's internal slots get consecutive indices when it is processed, and
-// Node4s are processed in pre-order, so slot r of node P gets 1 + (internal slots of all Node4s before P) + r.
-__global__ void k_flags(const brt_tree_node *__restrict__ T, const uint32_t *__restrict__ depth, uint32_t M,
+// Every internal node becomes one Node64 (numbered in pre-order among internal nodes). The root and every internal
+// slot of a record (collapse4) become one Node4 each, numbered as the host's depth-first allocation does: a Node4's
+// internal slots get consecutive indices when it is processed, and Node4s are processed in pre-order, so slot r of
+// node P gets 1 + (internal slots of all Node4s before P) + r.
+
+// is_root[i] = 1 when internal node i roots a Node4. A node's status depends on the records above it, so each
+// thread walks from the root down to its node: in pre-order the left subtree of v is [v + 1, right child), and on
+// the way the current record's slots tell whether the next node starts a record or lies inside the current one.
+__global__ void k_roots4(const brt_tree_node *__restrict__ T, uint32_t M, uint32_t *__restrict__ is_root) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  uint32_t root = 0;
+  if (i > 0 && !T[i].leaf) {
+    uint32_t slot[4];
+    int m = collapse4(T, 0, slot);
+    uint32_t v = 0;
+    // every step descends one level towards i, which lies below v, so v never is a leaf here and the loop ends
+    // within the tree's depth (at most 63); the bound and the leaf test only guard the reads
+    for (int level = 0; level < 64 && !T[v].leaf; level++) {
+      const uint32_t right = T[v].children[1];
+      v = (i >= right) ? right : T[v].children[0];
+      bool is_slot = false;
+      for (int c = 0; c < m; c++) is_slot = is_slot || slot[c] == v;
+      if (v == i) {
+        root = is_slot ? 1u : 0u;
+        break;
+      }
+      if (is_slot) m = collapse4(T, v, slot);
+    }
+  } else if (i == 0) {
+    root = T[0].leaf ? 0u : 1u;
+  }
+  is_root[i] = root;
+}
+
+__global__ void k_flags(const brt_tree_node *__restrict__ T, const uint32_t *__restrict__ is_root, uint32_t M,
                         uint32_t *__restrict__ internal, uint32_t *__restrict__ islots) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
-  const bool in = !T[i].leaf;
-  internal[i] = in ? 1u : 0u;
+  internal[i] = T[i].leaf ? 0u : 1u;
   uint32_t s = 0;
-  if (in && !(depth[i] & 1u))
-    for (int c = 0; c < 2; c++) {
-      const brt_tree_node &ch = T[T[i].children[c]];
-      if (!ch.leaf) s += (T[ch.children[0]].leaf ? 0u : 1u) + (T[ch.children[1]].leaf ? 0u : 1u);
-    }
+  if (is_root[i]) {
+    uint32_t slot[4];
+    const int m = collapse4(T, i, slot);
+    for (int c = 0; c < m; c++) s += T[slot[c]].leaf ? 0u : 1u;
+  }
   islots[i] = s;
 }
 
-__global__ void k_assign4(const brt_tree_node *__restrict__ T, const uint32_t *__restrict__ depth, uint32_t M,
+__global__ void k_assign4(const brt_tree_node *__restrict__ T, const uint32_t *__restrict__ is_root, uint32_t M,
                           const uint32_t *__restrict__ S, uint32_t *__restrict__ k4) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M || T[i].leaf || (depth[i] & 1u)) return;
+  if (i >= M || !is_root[i]) return;
   if (i == 0) k4[0] = 0;
+  uint32_t slot[4];
+  const int m = collapse4(T, i, slot);
   uint32_t r = 0;
-  for (int c = 0; c < 2; c++) {
-    const brt_tree_node &ch = T[T[i].children[c]];
-    if (ch.leaf) continue;
-    for (int g = 0; g < 2; g++)
-      if (!T[ch.children[g]].leaf) k4[ch.children[g]] = 1u + S[i] + r++;
-  }
+  for (int c = 0; c < m; c++)
+    if (!T[slot[c]].leaf) k4[slot[c]] = 1u + S[i] + r++;
 }
 
 __global__ void k_node64(const brt_tree_node *__restrict__ T, uint32_t M, const uint32_t *__restrict__ ni,
@@ -748,21 +769,12 @@ __global__ void k_node64(const brt_tree_node *__restrict__ T, uint32_t M, const 
   out[ni[i]] = a;
 }
 
-__global__ void k_node4(const brt_tree_node *__restrict__ T, const uint32_t *__restrict__ depth, uint32_t M,
+__global__ void k_node4(const brt_tree_node *__restrict__ T, const uint32_t *__restrict__ is_root, uint32_t M,
                         const uint32_t *__restrict__ k4, Node4 *__restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M || T[i].leaf || (depth[i] & 1u)) return;
+  if (i >= M || !is_root[i]) return;
   uint32_t slot[4];
-  int m = 0;
-  for (int c = 0; c < 2; c++) {
-    const uint32_t ch = T[i].children[c];
-    if (T[ch].leaf) {
-      slot[m++] = ch;
-    } else {
-      slot[m++] = T[ch].children[0];
-      slot[m++] = T[ch].children[1];
-    }
-  }
+  const int m = collapse4(T, i, slot);
   Node4 nd;
   for (int c = 0; c < 4; c++) {
     for (int k = 0; k < 3; k++) {
@@ -1021,7 +1033,6 @@ hipError_t device_build(const DeviceBuildIn &in, DeviceBuildOut &out, hipStream_
   // ---- small subtrees
   const uint32_t ns = (uint32_t)small.size();
   brt_tree_node *tmp = A.get<brt_tree_node>(2ull * n);
-  uint32_t *tmp_depth = A.get<uint32_t>(2ull * n);
   uint32_t *sizes = A.get<uint32_t>(ns), *bases = A.get<uint32_t>(ns);
   SmallRoot *d_small = A.get<SmallRoot>(ns);
   BuildStatsDev *d_stats = A.get<BuildStatsDev>(1);
@@ -1030,8 +1041,8 @@ hipError_t device_build(const DeviceBuildIn &in, DeviceBuildOut &out, hipStream_
   BRT_CHECK(hipMemcpyAsync(d_stats, &init, sizeof init, hipMemcpyHostToDevice, st));
   BRT_CHECK(hipMemcpyAsync(d_small, small.data(), ns * sizeof(SmallRoot), hipMemcpyHostToDevice, st));
   if (ns)
-    BRT_LAUNCH(k_small<<<ns, 64, 0, st>>>(d_small, idx, lo, hi, ctr, nb, in.min_leaf, in.max_depth, tmp, tmp_depth,
-                                          sizes, d_stats));
+    BRT_LAUNCH(k_small<<<ns, 64, 0, st>>>(d_small, idx, lo, hi, ctr, nb, in.min_leaf, in.max_depth, tmp, sizes,
+                                          d_stats));
   std::vector<uint32_t> hsizes(ns), hbases(ns);
   BuildStatsDev ds{};
   BRT_CHECK(hipMemcpyAsync(hsizes.data(), sizes, ns * 4, hipMemcpyDeviceToHost, st));
@@ -1075,23 +1086,23 @@ hipError_t device_build(const DeviceBuildIn &in, DeviceBuildOut &out, hipStream_
     p.node.prim_begin = h.leaf ? h.begin : 0u;
     p.node.prim_count = h.leaf ? h.n : 0u;
     p.pre = pre_top[t];
-    p.depth = h.depth;
+    p.pad = 0;
   }
   brt_tree_node *tree = A.get<brt_tree_node>(M);
-  uint32_t *depth = A.get<uint32_t>(M);
   const TopPut *d_puts = nullptr;
   BRT_CHECK(A.err);
   BRT_CHECK(to_device(A, puts.data(), puts.size(), &d_puts, st));
   BRT_CHECK(hipMemcpyAsync(bases, hbases.data(), ns * 4, hipMemcpyHostToDevice, st));
   if (!puts.empty())
-    BRT_LAUNCH(k_put_top<<<blocks_for(puts.size(), 64), 64, 0, st>>>(d_puts, (uint32_t)puts.size(), tree, depth));
-  if (ns) BRT_LAUNCH(k_reloc<<<ns, 256, 0, st>>>(d_small, sizes, bases, tmp, tmp_depth, tree, depth));
+    BRT_LAUNCH(k_put_top<<<blocks_for(puts.size(), 64), 64, 0, st>>>(d_puts, (uint32_t)puts.size(), tree));
+  if (ns) BRT_LAUNCH(k_reloc<<<ns, 256, 0, st>>>(d_small, sizes, bases, tmp, tree));
 
   // ---- flatten
   uint32_t *internal = A.get<uint32_t>(M), *islots = A.get<uint32_t>(M), *ni = A.get<uint32_t>(M),
-           *S = A.get<uint32_t>(M), *k4 = A.get<uint32_t>(M);
+           *S = A.get<uint32_t>(M), *k4 = A.get<uint32_t>(M), *is_root = A.get<uint32_t>(M);
   BRT_CHECK(A.err);
-  BRT_LAUNCH(k_flags<<<blocks_for(M, 256), 256, 0, st>>>(tree, depth, M, internal, islots));
+  BRT_LAUNCH(k_roots4<<<blocks_for(M, 256), 256, 0, st>>>(tree, M, is_root));
+  BRT_LAUNCH(k_flags<<<blocks_for(M, 256), 256, 0, st>>>(tree, is_root, M, internal, islots));
   size_t tb1 = 0, tb2 = 0;
   BRT_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb1, internal, ni, (int)M, st));
   BRT_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, islots, S, (int)M, st));
@@ -1119,9 +1130,9 @@ hipError_t device_build(const DeviceBuildIn &in, DeviceBuildOut &out, hipStream_
   }
   unsigned char *recs = A.get<unsigned char>((size_t)n * rec_size);
   BRT_CHECK(A.err);
-  BRT_LAUNCH(k_assign4<<<blocks_for(M, 256), 256, 0, st>>>(tree, depth, M, S, k4));
+  BRT_LAUNCH(k_assign4<<<blocks_for(M, 256), 256, 0, st>>>(tree, is_root, M, S, k4));
   BRT_LAUNCH(k_node64<<<blocks_for(M, 256), 256, 0, st>>>(tree, M, ni, nodes64));
-  BRT_LAUNCH(k_node4<<<blocks_for(M, 256), 256, 0, st>>>(tree, depth, M, k4, nodes4));
+  BRT_LAUNCH(k_node4<<<blocks_for(M, 256), 256, 0, st>>>(tree, is_root, M, k4, nodes4));
   const uint32_t g = blocks_for(n, 256);
   switch (in.type) {
   case TRIANGLES:

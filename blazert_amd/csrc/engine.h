@@ -7,6 +7,39 @@
 
 namespace brt {
 
+// Slots of the Node4 record rooted at binary internal node b (pre-order tree T): start from b's two children and,
+// while fewer than four slots are filled, replace the internal slot of largest surface area (the first of equal
+// ones) by its two children in place, so the slots stay in the binary tree's left-to-right order. The internal slots
+// are the roots of further records. Shared by the host flatten and the device build, which give identical records.
+// A record advances a root-to-leaf path by at least one binary level and pushes at most three stack entries.
+__host__ __device__ inline int collapse4(const brt_tree_node *T, uint32_t b, uint32_t slot[4]) {
+#pragma clang fp contract(off)
+  slot[0] = T[b].children[0];
+  slot[1] = T[b].children[1];
+  int m = 2;
+  while (m < 4) {
+    int best = -1;
+    double area = 0.0;
+    for (int c = 0; c < m; c++) {
+      const brt_tree_node &x = T[slot[c]];
+      if (x.leaf) continue;
+      const double dx = x.max[0] - x.min[0], dy = x.max[1] - x.min[1], dz = x.max[2] - x.min[2];
+      const double a = dx * dy + dy * dz + dz * dx;
+      if (best < 0 || a > area) {
+        best = c;
+        area = a;
+      }
+    }
+    if (best < 0) break;
+    const uint32_t e = slot[best];
+    for (int c = m; c > best + 1; c--) slot[c] = slot[c - 1];
+    slot[best] = T[e].children[0];
+    slot[best + 1] = T[e].children[1];
+    m++;
+  }
+  return m;
+}
+
 constexpr uint32_t SCHEDULE_PERSISTENT = 0;  // persistent waves with per-lane ray refill (default)
 constexpr uint32_t SCHEDULE_PER_THREAD = 1;  // one ray per thread, grid-stride
 

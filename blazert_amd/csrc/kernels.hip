@@ -31,10 +31,10 @@ constexpr uint32_t PREC_FP64 = 0;
 constexpr uint32_t FLAG_INOUT = 1u;
 constexpr uint32_t FLAG_ANY_HIT = 2u;
 // Deepest stack a traversal can build: the binary FP64 DFS pushes at most one entry per level (trees are limited
-// to depth 63 at upload); the 4-wide FP32 DFS at most three per Node4 level, and a Node4 level spans two binary
-// levels, so at most 3 * 32 = 96.
+// to depth 63 at upload); the 4-wide FP32 DFS at most three per Node4 record, and a record spans at least one
+// binary level on every path, so at most 3 * 63 = 189.
 template <bool SPLIT>
-constexpr int stack_cap() { return SPLIT ? 128 : 64; }
+constexpr int stack_cap() { return SPLIT ? 192 : 64; }
 
 // The reference's ray setup and exact slab test (Real, RayT, setup_ray, slab_exact, product_sign, dsqrt) are shared
 // with user-collection traversal: include/blazert_amd/device/exact.h.

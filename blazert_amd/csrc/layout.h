@@ -25,9 +25,9 @@ inline __host__ __device__ void tri_normal(const R *e1, const R *e2, R *n) {
 // USER_COLLECTION: a tree over a user collection's boxes (brt_bvh_build), with no primitive records
 enum GeomType : uint32_t { TRIANGLES = 0, SPHERES = 1, PLANES = 2, CYLINDERS = 3, NUM_TYPES = 4, USER_COLLECTION = 16 };
 
-// FP32 4-wide node for split-precision traversal, 128 B: the binary tree collapsed two levels at a time (an
-// internal node's internal children are replaced by their own two children), so one record holds up to four
-// child boxes, rounded outward (min down, max up). Planes are child-minor: lo[axis][child]. A child reference is a
+// FP32 4-wide node for split-precision traversal, 128 B: the binary tree collapsed greedily (starting from a node's
+// two children, the internal slot of largest surface area is replaced by its own two children until four slots are
+// filled or all are leaves), so one record holds up to four child boxes, rounded outward (min down, max up). Planes are child-minor: lo[axis][child]. A child reference is a
 // Node4 index or LEAF_BIT | first primitive with the leaf count beside it; unused slots have cnt = EMPTY_SLOT.
 constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
 struct alignas(128) Node4 {

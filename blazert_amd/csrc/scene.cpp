@@ -504,8 +504,8 @@ void flatten(Geometry &g, GeomType type) {
     }
     a.cnt[1] |= T[i].axis << AXIS_SHIFT;
   }
-  // 4-wide FP32 records: every Node4 stands for one binary internal node; its slots are the node's children, with
-  // each internal child replaced by that child's two children. Depth-first allocation keeps subtrees together.
+  // 4-wide FP32 records: every Node4 stands for one binary internal node, with up to four slots taken from its
+  // subtree by collapse4 (largest internal slot opened first). Depth-first allocation keeps subtrees together.
   g.n4.clear();
   uint32_t root4 = INVALID_ID;
   if (!T.empty() && !T[0].leaf) {
@@ -517,16 +517,7 @@ void flatten(Geometry &g, GeomType type) {
       const auto [bi, k4] = work.back();
       work.pop_back();
       uint32_t slot[4];
-      int m = 0;
-      for (int c = 0; c < 2; c++) {
-        const uint32_t ch = T[bi].children[c];
-        if (T[ch].leaf) {
-          slot[m++] = ch;
-        } else {
-          slot[m++] = T[ch].children[0];
-          slot[m++] = T[ch].children[1];
-        }
-      }
+      const int m = collapse4(T.data(), bi, slot);
       Node4 nd{};
       for (int c = 0; c < 4; c++) {
         for (int k = 0; k < 3; k++) {
