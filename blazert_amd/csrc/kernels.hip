@@ -502,10 +502,12 @@ struct Trav {
 // steps push up to three entries at a time.
 template <bool SPLIT>
 constexpr uint32_t ring_default() { return SPLIT ? 16u : 8u; }
-// Default refill threshold (idle lanes of a wave before it takes new rays), measured on the 1.31M-triangle
-// incoherent set: 32 for the exact path (16: 3% slower, 40: 5% slower), 16 for the 4-wide split path (32: 7% slower).
-template <bool SPLIT>
-constexpr uint32_t refill_default() { return SPLIT ? 16u : 32u; }
+// Default refill threshold (idle lanes of a wave before it takes new rays). Exact path: 32 for triangles (1.31M-
+// triangle incoherent set: 16 is 3% slower, 40 5% slower), 16 for the other types, whose longer primitive tests
+// favour earlier refills (mixed scene: spheres 3% and cylinders 5% faster than at 32). 4-wide split path: 16 for
+// every type (icosphere: 32 is 7% slower).
+template <class Rec, bool SPLIT>
+constexpr uint32_t refill_default() { return (SPLIT || Rec::kind != TRIANGLES) ? 16u : 32u; }
 
 // Stack keys: FP32 entry distances on the split path, otherwise the scene type's.
 template <bool SPLIT, class R>
@@ -1095,7 +1097,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real>
     const uint64_t need = (n + block - 1) / block;
     if (blocks > need) blocks = need;
     if (blocks == 0) return hipSuccess;
-    const uint32_t refill = cfg.refill ? cfg.refill : refill_default<SPLIT>();
+    const uint32_t refill = cfg.refill ? cfg.refill : refill_default<Rec, SPLIT>();
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
     hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
                        hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill, cfg.order, ring);
@@ -1127,7 +1129,7 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
   const uint64_t need = (total + block - 1) / block;
   if (blocks > need) blocks = need;
   if (blocks == 0) return hipSuccess;
-  const uint32_t refill = cfg.refill ? cfg.refill : refill_default<SPLIT>();
+  const uint32_t refill = cfg.refill ? cfg.refill : refill_default<Rec, SPLIT>();
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
   hipLaunchKernelGGL((k_trace_batch<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, jobs, njobs, total,
                      cfg.flags, cfg.counter, refill, ring);

@@ -114,7 +114,7 @@ typedef struct brt_launch {
   uint32_t block_size; /* threads per block: 64, 128, 192 or 256; 0 = library default (256) */
   uint32_t schedule;   /* BRT_SCHEDULE_* */
   uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default (32 for
-                          the exact paths, 16 for split precision) */
+                          triangles on the exact paths, 16 otherwise) */
   uint32_t sort;       /* BRT_SORT_*: trace rays in a sorted order (results are written in input order) */
   uint32_t stack_ring; /* traversal-stack entries per lane kept in LDS: 4, 8 or 16; 0 = 8 (FP64) or 16 (split).
                           Deeper entries spill to private memory; a deeper ring spills less but takes more LDS */
