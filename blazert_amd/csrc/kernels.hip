@@ -318,8 +318,10 @@ __device__ bool cyl_eval(const Rec &Cy, const RayT<typename Rec::real> &r, typen
     const R B = a * a * k * y0 + b * b * j * x0;
     const R C = a * a * k * k + b * b * j * j - j * j * y0 * y0 + 2 * j * k * x0 * y0 - k * k * x0 * x0;
     if (C < 0) return true;  // reference returns false immediately
-    const R s0 = R(1) / A * (-B + a * b * dsqrt(C));
-    const R s1 = R(-1) / A * (B + a * b * dsqrt(C));
+    // R(-1) / A is -(R(1) / A) exactly (round to nearest is symmetric under negation): one division for both roots
+    const R inv_a = R(1) / A;
+    const R s0 = inv_a * (-B + a * b * dsqrt(C));
+    const R s1 = -inv_a * (B + a * b * dsqrt(C));
     const R i0z = org[2] + dir[2] * s0;
     const R i1z = org[2] + dir[2] * s1;
     if ((s1 > 0) && (!ordered || s1 < s0) && (i1z <= zt) && (i1z >= zb)) {
@@ -386,8 +388,9 @@ __device__ bool cyl_eval(const Rec &Cy, const RayT<typename Rec::real> &r, typen
       const R B = a * a * k * y0 + b * b * j * x0;
       const R C = a * a * k * k + b * b * j * j - j * j * y0 * y0 + 2 * j * k * x0 * y0 - k * k * x0 * x0;
       if (C < 0) return false;
-      const R s0 = R(1) / A * (-B + a * b * dsqrt(C));
-      const R s1 = R(-1) / A * (B + a * b * dsqrt(C));
+      const R inv_a = R(1) / A;
+      const R s0 = inv_a * (-B + a * b * dsqrt(C));
+      const R s1 = -inv_a * (B + a * b * dsqrt(C));
       const R i0z = org[2] + dir[2] * s0;
       const R i1z = org[2] + dir[2] * s1;
       if ((s1 > 0) && (i1z <= zt) && (i1z >= zb)) {
