@@ -205,3 +205,87 @@ def test_icosphere8_full(dev):
     td = time.perf_counter() - t0
     print(f"commit 1.31M triangles: host {th * 1e3:.1f} ms, device {td * 1e3:.1f} ms", flush=True)
     _assert_same_build(h, d, [0])
+
+
+NODE4 = np.dtype([("lo", np.float32, (3, 4)), ("hi", np.float32, (3, 4)), ("ref", np.uint32, 4),
+                  ("cnt", np.uint32, 4)])
+LEAF_BIT = 0x80000000
+EMPTY_SLOT = 0xFFFFFFFF
+
+
+def _f32_down(x):
+    f = np.float32(x)
+    return np.nextafter(f, np.float32(-np.inf)) if float(f) > x else f
+
+
+def _f32_up(x):
+    f = np.float32(x)
+    return np.nextafter(f, np.float32(np.inf)) if float(f) < x else f
+
+
+def _expected_node4(T):
+    """Records of the 4-wide split-precision tree as documented: from a node's two children, the internal slot of
+    largest surface area (the first of equal ones) is replaced in place by its two children until four slots are
+    filled; records are numbered depth first, each record's internal slots consecutively when it is written."""
+    def area(i):
+        d = T["max"][i] - T["min"][i]
+        return d[0] * d[1] + d[1] * d[2] + d[2] * d[0]
+
+    def slots(b):
+        sl = [int(T["children"][b][0]), int(T["children"][b][1])]
+        while len(sl) < 4:
+            cand = [(c, area(x)) for c, x in enumerate(sl) if not T["leaf"][x]]
+            if not cand:
+                break
+            best = cand[0]
+            for c, a in cand[1:]:
+                if a > best[1]:
+                    best = (c, a)
+            e = sl[best[0]]
+            sl[best[0]:best[0] + 1] = [int(T["children"][e][0]), int(T["children"][e][1])]
+        return sl
+
+    recs, work = [None], [(0, 0)]
+    while work:
+        b, k = work.pop()
+        sl, refs, cnts, inner = slots(b), [], [], []
+        for x in sl:
+            if T["leaf"][x]:
+                refs.append(LEAF_BIT | int(T["prim_begin"][x]))
+                cnts.append(int(T["prim_count"][x]))
+            else:
+                refs.append(len(recs))
+                cnts.append(0)
+                recs.append(None)
+                inner.append((x, refs[-1]))
+        recs[k] = (sl, refs, cnts)
+        work.extend(reversed(inner))
+    return recs
+
+
+@pytest.mark.parametrize("leaf", [1, 4])
+def test_node4_records_follow_the_greedy_collapse(dev, leaf):
+    """Both builders' 4-wide records against a restatement of the collapse rule: slots, references, counts, boxes
+    rounded outward to FP32 and the empty-slot encoding, record for record; the record count is below the
+    two-levels-at-a-time count (one record per internal node at even depth)."""
+    V, F = W.icosphere(5)
+    h, d = _pair(_mesh(V, F), min_leaf_primitives=leaf)
+    T, _ = h.export_tree(0)
+    want = _expected_node4(T)
+    depth = np.zeros(len(T), dtype=np.int64)
+    for i in range(len(T)):
+        if not T["leaf"][i]:
+            depth[T["children"][i]] = depth[i] + 1
+    assert len(want) < int(np.sum((T["leaf"] == 0) & (depth % 2 == 0)))
+    for s in (h, d):
+        got = s.read_device_array(0, capi.ARRAY_NODES4).view(NODE4)
+        assert len(got) == len(want)
+        for k, (sl, refs, cnts) in enumerate(want):
+            r = got[k]
+            assert list(r["ref"][:len(sl)]) == refs and list(r["cnt"][:len(sl)]) == cnts, k
+            for c, x in enumerate(sl):
+                for a in range(3):
+                    assert r["lo"][a][c] == _f32_down(T["min"][x][a]) and r["hi"][a][c] == _f32_up(T["max"][x][a])
+            for c in range(len(sl), 4):
+                assert r["cnt"][c] == EMPTY_SLOT and r["ref"][c] == 0
+                assert np.all(r["lo"][:, c] == np.inf) and np.all(r["hi"][:, c] == -np.inf)
