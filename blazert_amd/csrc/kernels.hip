@@ -868,12 +868,15 @@ __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysViewT<typename R
 
 // Persistent waves: every lane owns one ray at a time; when at least `refill` lanes of a wave are idle, they start
 // the next indices of the wave's pool. Skewed per-ray work (many rays miss the root box, a few visit hundreds of
-// nodes) then no longer idles the wave until its slowest ray ends. The pool is a range of POOL_CHUNK indices (or
-// the number of idle lanes, if larger) claimed from the launch's counter with one atomic: every claim on that one
-// address serialises across the GPU, and claiming per refill made the counter a bottleneck at small refill
-// thresholds. Every lane leaves the loop once the counter has passed n, the pool is empty and its own ray is done,
-// so the grid always drains; at the end a wave holds at most POOL_CHUNK unstarted rays (one per lane).
+// nodes) then no longer idles the wave until its slowest ray ends. The pool is a range of indices claimed from the
+// launch's counter with one atomic: every claim on that one address serialises across the GPU, and claiming per
+// refill made the counter a bottleneck at small refill thresholds. A claim takes POOL_BIG indices while the wave's
+// previous claim ended more than two big pools per wave before n, and POOL_CHUNK (or the number of idle lanes, if
+// larger) after that, so the final claims stay small. Every lane leaves the loop once the counter has passed n, the
+// pool is empty and its own ray is done, so the grid always drains; at the end a wave holds at most POOL_CHUNK
+// unstarted rays (one per lane).
 constexpr uint32_t POOL_CHUNK = 64;
+constexpr uint32_t POOL_BIG = 128;
 
 // Wave-uniform pool of claimed, not yet started ray indices [next, end).
 struct WavePool {
@@ -888,7 +891,9 @@ __device__ __forceinline__ void pool_refill(WavePool &P, unsigned long long *cou
                                             uint64_t &idx, bool &exhausted) {
   got = false;
   if (P.next == P.end && !P.last) {
-    const uint32_t want = nidle > POOL_CHUNK ? nidle : POOL_CHUNK;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64u);
+    const uint32_t chunk = (P.end + 2ull * POOL_BIG * waves < n) ? POOL_BIG : POOL_CHUNK;
+    const uint32_t want = nidle > chunk ? nidle : chunk;
     unsigned long long base = 0;
     const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
     if (lane == leader) base = atomicAdd(counter, (unsigned long long)want);
