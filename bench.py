@@ -51,28 +51,13 @@ def time_launch(fn, steps, warmup):
 
 
 def classify_disagreements(osc, org, dirs, idx, other=None, cyl_geom=None) -> dict:
-    """Re-runs the oracle (with its traversal trace) on the rays where the two precisions disagree and counts how
-    many are ties (a runner-up within 1e-12 relative in the reference traversal, or a split-precision hit at the
-    same distance to 1e-12), order-dependent cylinder rays (either side's hit is a cylinder: the reference accepts a
-    cylinder without a distance check, so the last one visited wins, and bounds a rotated cylinder by a box that
-    can miss part of it) or reference stack overflows. `other` holds the split path's t and geom_id on those rays;
-    cyl_geom is the geometry id of the scene's cylinders (geometry ids follow the add order)."""
-    if len(idx) == 0:
-        return {"disagree": 0, "tie": 0, "cylinder": 0, "overflow": 0, "unexplained": 0}
-    ref = osc.intersect_batch(org[idx], dirs[idx], normals=False, trace=True)
-    tr = ref["trace"]
-    tie = tr["tie"] != 0
-    cyl = tr["multi_cyl"] != 0
-    if cyl_geom is not None:
-        cyl |= ref["geom_id"] == cyl_geom
-    if other is not None:
-        ot, og = other
-        tie |= np.abs(ot - ref["t"]) <= 1e-12 * np.abs(ref["t"])
-        if cyl_geom is not None:
-            cyl |= og == cyl_geom
-    ovf = tr["overflow"] != 0
-    return {"disagree": int(len(idx)), "tie": int(tie.sum()), "cylinder": int((cyl & ~tie).sum()),
-            "overflow": int((ovf & ~tie & ~cyl).sum()), "unexplained": int((~tie & ~cyl & ~ovf).sum())}
+    """Counts of the split/FP64 disagreements by class (tie, cylinder, reference overflow, unexplained), from the
+    oracle's classifier. `other` holds the split path's t and geom_id on the disagreeing rays."""
+    from oracle import oracle as OR
+    ot, og = other if other is not None else (np.full(len(idx), np.nan), np.full(len(idx), 0xFFFFFFFF, np.uint32))
+    c = OR.classify_split_disagreements(osc, org, dirs, idx, ot, og, cyl_geom=cyl_geom)
+    c.pop("unexplained_idx")
+    return c
 
 
 def split_agreement(h64, h32) -> tuple:

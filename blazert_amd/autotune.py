@@ -15,7 +15,7 @@ import os
 
 from . import capi
 from .capi import check, lib
-from .scene import Scene, _alloc_hits, _columns, _hits_struct
+from .scene import Scene, _alloc_hits, _columns, _hits_struct, _OnStream
 
 LAUNCH_KEYS = ("block_size", "refill", "stack_ring", "sort")
 
@@ -72,7 +72,10 @@ def _result(r: capi.TuneResult) -> dict:
 
 
 def _rays(org, dirs):
-    co, cd = _columns(org), _columns(dirs)
+    co = _columns(org, what="origins")
+    cd = _columns(dirs, dev=co[0].device, what="directions")
+    if cd[0].numel() != co[0].numel():
+        raise ValueError(f"directions: expected {co[0].numel()} rays, got {cd[0].numel()}")
     return co, cd, capi.Rays(*[c.data_ptr() for c in co], *[c.data_ptr() for c in cd], None, None)
 
 
@@ -80,8 +83,6 @@ def tune_launch(scene: Scene, org, dirs, *, precision: int = capi.PRECISION_FP64
                 repeats: int = 3, use_cache: bool = True, stream=None) -> dict:
     """Fastest launch parameters for this scene and ray set (org / dirs: device (n,3) tensors or columns, a
     representative sample of the workload). Returns a dict whose LAUNCH_KEYS entries are Scene.intersect keywords."""
-    import torch
-
     co, cd, rays = _rays(org, dirs)
     n = co[0].numel()
     flags = capi.FLAG_ANY_HIT if any_hit else 0
@@ -91,10 +92,11 @@ def tune_launch(scene: Scene, org, dirs, *, precision: int = capi.PRECISION_FP64
         if hit is not None:
             return hit
     hits = _alloc_hits(n, co[0].device, False)
-    st = stream if stream is not None else torch.cuda.current_stream(co[0].device)
+    on = _OnStream(stream, co[0].device)
     r = capi.TuneResult()
     check(lib().brt_autotune_launch(scene.handle, C.byref(rays), C.byref(_hits_struct(hits)), n, precision, flags,
-                                    repeats, C.byref(r), C.c_void_p(st.cuda_stream)))
+                                    repeats, C.byref(r), on.handle))
+    on.hold(co, cd, hits)
     out = _result(r)
     if use_cache:
         _store(key, out)
@@ -115,16 +117,16 @@ def tune_leaf(scene: Scene, org, dirs, *, precision: int = capi.PRECISION_FP64, 
               repeats: int = 3, stream=None) -> dict:
     """Rebuilds the scene with each leaf size, times it on the rays and leaves it built with the fastest."""
     import numpy as np
-    import torch
 
     co, cd, rays = _rays(org, dirs)
     n = co[0].numel()
     hits = _alloc_hits(n, co[0].device, False)
     sizes = np.ascontiguousarray(leaf_sizes, dtype=np.uint32)
-    st = stream if stream is not None else torch.cuda.current_stream(co[0].device)
+    on = _OnStream(stream, co[0].device)
     r = capi.TuneResult()
     check(lib().brt_scene_tune_leaf(scene.handle, C.byref(rays), C.byref(_hits_struct(hits)), n, precision,
                                     sizes.ctypes.data_as(C.POINTER(C.c_uint32)), len(sizes), repeats, C.byref(r),
-                                    C.c_void_p(st.cuda_stream)))
+                                    on.handle))
+    on.hold(co, cd, hits)
     scene.has_been_committed = True
     return _result(r)

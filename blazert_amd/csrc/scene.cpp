@@ -901,6 +901,14 @@ int split_supported(const Geometry &g) {
   return BRT_OK;
 }
 
+// Precision a geometry type is traced in. The split path's conservative FP32 boxes bound the primitives only where
+// the tree's boxes do; a tree with an inverted or NaN box (a negative sphere radius, NaN input) culls primitives
+// its boxes do not contain, which the split path would reach. Such a type is traced by the exact FP64 path, so a
+// split-precision request returns the FP64 result for it.
+uint32_t trace_precision(uint32_t requested, const BvhView &v) {
+  return (requested == BRT_PRECISION_FP32_TRAV_FP64_HIT && v.irregular_boxes) ? BRT_PRECISION_FP64 : requested;
+}
+
 int device_error(hipError_t e, const char *what) {
   return fail(e == hipErrorOutOfMemory ? BRT_ERR_OUT_OF_MEMORY : BRT_ERR_DEVICE,
               std::string(what) + ": " + hipGetErrorString(e));
@@ -1528,7 +1536,8 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
     if (!g.present || g.boxes.n == 0) continue;
     if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT)
       if (int rc = split_supported(g)) return rc;
-    TraceLaunch cfg{L.precision, L.flags, L.block_size, first, L.schedule, L.refill, nullptr, order};
+    TraceLaunch cfg{trace_precision(L.precision, s->dev->views[t]), L.flags, L.block_size, first, L.schedule,
+                    L.refill, nullptr, order};
     cfg.ring = L.stack_ring;
     if (L.schedule == BRT_SCHEDULE_PERSISTENT)
       cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
@@ -1571,8 +1580,8 @@ int brt_traverse(const brt_scene *cs, uint32_t geom_type, const brt_rays *rays, 
   if (int rc = upload(s)) return rc;
   RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
   HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz, nullptr};
-  TraceLaunch cfg{L.precision, L.flags | BRT_FLAG_HITS_INOUT, L.block_size, true, L.schedule, L.refill, nullptr,
-                  nullptr};
+  TraceLaunch cfg{trace_precision(L.precision, s->dev->views[geom_type]), L.flags | BRT_FLAG_HITS_INOUT,
+                  L.block_size, true, L.schedule, L.refill, nullptr, nullptr};
   cfg.ring = L.stack_ring;
   cfg.overwrite = true;
   if (L.schedule == BRT_SCHEDULE_PERSISTENT)
@@ -1860,8 +1869,12 @@ int brt_batch_create(const brt_batch_item *items, uint32_t count, const brt_laun
     for (uint32_t t = 0; t < NUM_TYPES; t++) {
       const Geometry &g = s->geo[t];
       if (!g.present || g.boxes.n == 0) continue;
-      if (b->L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT)
+      if (b->L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT) {
         if (int rc = split_supported(g)) return rc;
+        if (s->dev->views[t].irregular_boxes)
+          return fail(BRT_ERR_UNSUPPORTED, "split-precision batches need trees without inverted or NaN boxes "
+                                           "(such a scene is traced by the FP64 path in brt_intersect)");
+      }
       BatchJob j{};
       j.bvh = s->dev->views[t];
       j.rays = RaysView{it.rays.ox, it.rays.oy, it.rays.oz, it.rays.dx, it.rays.dy, it.rays.dz, it.rays.tmin,

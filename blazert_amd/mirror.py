@@ -13,14 +13,24 @@ import ctypes as C
 
 from . import capi
 from .capi import check, lib
-from .scene import Scene, _columns, _scalar_col
+from .scene import Scene, _columns, _OnStream, _scalar_col
 
 DEFAULT_BOUNCE_TMIN = 1e-9
 
 
-def _stream(stream, dev):
+def _rays(org, dirs):
+    co = _columns(org, what="origins")
+    dev = co[0].device
+    cd = _columns(dirs, dev=dev, what="directions")
+    if cd[0].numel() != co[0].numel():
+        raise ValueError(f"directions: expected {co[0].numel()} rays, got {cd[0].numel()}")
+    return co, cd, dev
+
+
+def _cuda_on(x, dev, what):
     import torch
-    return stream if stream is not None else torch.cuda.current_stream(dev)
+    if not torch.is_tensor(x) or not x.is_cuda or x.device != dev:
+        raise ValueError(f"{what}: expected a CUDA tensor on {dev}")
 
 
 def mirror_trace(scene: Scene, org, dirs, bounces: int = 3, bounce_tmin: float = DEFAULT_BOUNCE_TMIN, tmin=None,
@@ -34,11 +44,10 @@ def mirror_trace(scene: Scene, org, dirs, bounces: int = 3, bounce_tmin: float =
     """
     import torch
 
-    co, cd = _columns(org), _columns(dirs)
+    co, cd, dev = _rays(org, dirs)
     n = co[0].numel()
-    dev = co[0].device
-    tmn = None if tmin is None else _scalar_col(tmin, n, dev)
-    tmx = None if tmax is None else _scalar_col(tmax, n, dev)
+    tmn = None if tmin is None else _scalar_col(tmin, n, dev, what="tmin")
+    tmx = None if tmax is None else _scalar_col(tmax, n, dev, what="tmax")
     B = int(bounces)
     out = {"t": torch.empty((B, n), dtype=torch.float64, device=dev),
            "prim_id": torch.empty((B, n), dtype=torch.int32, device=dev),
@@ -55,10 +64,10 @@ def mirror_trace(scene: Scene, org, dirs, bounces: int = 3, bounce_tmin: float =
                         P[0].data_ptr(), P[1].data_ptr(), P[2].data_ptr(), D[0].data_ptr(), D[1].data_ptr(),
                         D[2].data_ptr())
     L = capi.Launch(capi.PRECISION_FP64, 0, block_size, schedule, refill, capi.SORT_NONE)
-    st = _stream(stream, dev)
+    on = _OnStream(stream, dev)
     check(lib().brt_mirror_trace(scene.handle, C.byref(rays), n, B, float(bounce_tmin), C.byref(L), C.byref(mo),
-                                 C.c_void_p(st.cuda_stream)))
-    out["_keep"] = (co, cd, tmn, tmx)
+                                 on.handle))
+    on.hold(co, cd, tmn, tmx, out)
     return out
 
 
@@ -69,9 +78,12 @@ def mirror_adjoint(scene: Scene, org, dirs, prim_id, grad_t=None, grad_position=
     accumulated into when given."""
     import torch
 
-    co, cd = _columns(org), _columns(dirs)
+    co, cd, dev = _rays(org, dirs)
     n = co[0].numel()
-    dev = co[0].device
+    _cuda_on(prim_id, dev, "prim_id")
+    for name, g in (("grad_t", grad_t), ("grad_position", grad_position), ("grad_vertices", grad_vertices)):
+        if g is not None:
+            _cuda_on(g, dev, name)
     prim = prim_id.contiguous()
     if prim.dim() != 2 or prim.shape[1] != n or prim.dtype not in (torch.int32, torch.uint32):
         raise ValueError("prim_id must be a (bounces, n) int32 tensor from mirror_trace")
@@ -95,9 +107,9 @@ def mirror_adjoint(scene: Scene, org, dirs, prim_id, grad_t=None, grad_position=
                          *([go[k].data_ptr() for k in range(3)] if want_rays else [None] * 3),
                          *([gd[k].data_ptr() for k in range(3)] if want_rays else [None] * 3))
     rays = capi.Rays(*[c.data_ptr() for c in co], *[c.data_ptr() for c in cd], None, None)
-    st = _stream(stream, dev)
-    check(lib().brt_mirror_adjoint(scene.handle, C.byref(rays), n, B, prim.data_ptr(), C.byref(g),
-                                   C.c_void_p(st.cuda_stream)))
+    on = _OnStream(stream, dev)
+    check(lib().brt_mirror_adjoint(scene.handle, C.byref(rays), n, B, prim.data_ptr(), C.byref(g), on.handle))
+    on.hold(co, cd, prim, gt, gp, grad_vertices, go, gd)
     return grad_vertices, (go.t() if want_rays else None), (gd.t() if want_rays else None)
 
 

@@ -170,12 +170,14 @@ class Scene:
         import torch
 
         tdt = torch.float32 if self.f32 else torch.float64
-        cols_o = _columns(org, tdt)
-        cols_d = _columns(dirs, tdt)
-        n = cols_o[0].numel()
+        cols_o = _columns(org, tdt, what="origins")
         dev = cols_o[0].device
-        tmn = None if tmin is None else _scalar_col(tmin, n, dev, tdt)
-        tmx = None if tmax is None else _scalar_col(tmax, n, dev, tdt)
+        cols_d = _columns(dirs, tdt, dev, what="directions")
+        n = cols_o[0].numel()
+        if cols_d[0].numel() != n:
+            raise ValueError(f"directions: expected {n} rays, got {cols_d[0].numel()}")
+        tmn = None if tmin is None else _scalar_col(tmin, n, dev, tdt, "tmin")
+        tmx = None if tmax is None else _scalar_col(tmax, n, dev, tdt, "tmax")
         flags = capi.FLAG_ANY_HIT if any_hit else 0
         if hits is None:
             hits = {k: torch.empty(n, dtype=tdt, device=dev) for k in ("t", "u", "v")}
@@ -184,6 +186,7 @@ class Scene:
             if normals:
                 hits["normal"] = torch.empty((3, n), dtype=tdt, device=dev)
         else:
+            _check_hits(hits, n, dev, tdt)
             flags |= capi.FLAG_HITS_INOUT
         nrm = hits.get("normal")
         rays = capi.Rays(*[c.data_ptr() for c in cols_o], *[c.data_ptr() for c in cols_d],
@@ -192,10 +195,9 @@ class Scene:
                       hits["geom_id"].data_ptr(), *([nrm[k].data_ptr() for k in range(3)] if nrm is not None
                                                     else [None, None, None]))
         L = capi.Launch(precision, flags, block_size, schedule, refill, sort, stack_ring)
-        st = stream if stream is not None else torch.cuda.current_stream(dev)
-        keep = (cols_o, cols_d, tmn, tmx)  # noqa: F841 (inputs alive until the launch is enqueued)
-        check(self._fn("brt_intersect")(self._h, C.byref(rays), C.byref(H), n, C.byref(L),
-                                        C.c_void_p(st.cuda_stream)))
+        on = _OnStream(stream, dev)
+        check(self._fn("brt_intersect")(self._h, C.byref(rays), C.byref(H), n, C.byref(L), on.handle))
+        on.hold(cols_o, cols_d, tmn, tmx, hits)
         return hits
 
     # -------------------------------------------------------------------------------------------- distances
@@ -203,32 +205,37 @@ class Scene:
         """distance_to_surface(primitive prim_ids[i], point i) for device points ((n,3) or three columns) and int32
         ids of one geometry type; FP64 tensor, NaN for ids out of range (brt_distance_to_surface)."""
         import torch
-        cols = _columns(points)
+        cols = _columns(points, what="points")
         n = cols[0].numel()
-        ids = torch.as_tensor(prim_ids).to(device=cols[0].device, dtype=torch.int32).contiguous()
-        if ids.numel() != n:
-            raise ValueError("one primitive id per point")
-        out = torch.empty(n, dtype=torch.float64, device=cols[0].device)
-        st = stream if stream is not None else torch.cuda.current_stream(cols[0].device)
+        dev = cols[0].device
+        if not torch.is_tensor(prim_ids) or not prim_ids.is_cuda or prim_ids.device != dev:
+            raise ValueError(f"prim_ids: expected a CUDA tensor on {dev}")
+        if prim_ids.dtype not in (torch.int32, torch.int64, torch.uint32) or prim_ids.numel() != n:
+            raise ValueError(f"prim_ids: expected {n} integer ids")
+        ids = prim_ids.to(torch.int32).reshape(-1).contiguous()
+        out = torch.empty(n, dtype=torch.float64, device=dev)
+        on = _OnStream(stream, dev)
         pts = capi.Points(*[c.data_ptr() for c in cols])
         check(lib().brt_distance_to_surface(self._h, geom_type, C.byref(pts), C.c_void_p(ids.data_ptr()), n,
-                                            C.c_void_p(out.data_ptr()), C.c_void_p(st.cuda_stream)))
+                                            C.c_void_p(out.data_ptr()), on.handle))
+        on.hold(cols, ids, out)
         return out
 
     def closest_surface(self, points, stream=None) -> dict:
         """Closest surface of the scene for each device point: distance, prim_id, geom_id (brt_closest_surface)."""
         import torch
-        cols = _columns(points)
+        cols = _columns(points, what="points")
         n = cols[0].numel()
         dev = cols[0].device
         out = {"distance": torch.empty(n, dtype=torch.float64, device=dev),
                "prim_id": torch.empty(n, dtype=torch.int32, device=dev),
                "geom_id": torch.empty(n, dtype=torch.int32, device=dev)}
-        st = stream if stream is not None else torch.cuda.current_stream(dev)
+        on = _OnStream(stream, dev)
         pts = capi.Points(*[c.data_ptr() for c in cols])
         check(lib().brt_closest_surface(self._h, C.byref(pts), n, *[C.c_void_p(out[k].data_ptr())
                                                                    for k in ("distance", "prim_id", "geom_id")],
-                                        C.c_void_p(st.cuda_stream)))
+                                        on.handle))
+        on.hold(cols, out)
         return out
 
     def traverse(self, geom_type: int, org, dirs, hits: dict, tmin=None, tmax=None, **kw) -> dict:
@@ -241,18 +248,24 @@ def _traverse(scene, geom_type, org, dirs, hits, tmin=None, tmax=None, *, precis
               any_hit=False, stream=None):
     import torch
 
-    cols_o, cols_d = _columns(org), _columns(dirs)
-    n = cols_o[0].numel()
+    if scene.f32:
+        raise ValueError("traverse is available for Scene<double> only")
+    cols_o = _columns(org, what="origins")
     dev = cols_o[0].device
-    tmn = None if tmin is None else _scalar_col(tmin, n, dev)
-    tmx = None if tmax is None else _scalar_col(tmax, n, dev)
+    cols_d = _columns(dirs, dev=dev, what="directions")
+    n = cols_o[0].numel()
+    if cols_d[0].numel() != n:
+        raise ValueError(f"directions: expected {n} rays, got {cols_d[0].numel()}")
+    _check_hits(hits, n, dev, torch.float64)
+    tmn = None if tmin is None else _scalar_col(tmin, n, dev, what="tmin")
+    tmx = None if tmax is None else _scalar_col(tmax, n, dev, what="tmax")
     rays = capi.Rays(*[c.data_ptr() for c in cols_o], *[c.data_ptr() for c in cols_d],
                      tmn.data_ptr() if tmn is not None else None, tmx.data_ptr() if tmx is not None else None)
     L = capi.Launch(precision, capi.FLAG_ANY_HIT if any_hit else 0, 0, capi.SCHEDULE_PERSISTENT, 0, capi.SORT_NONE)
-    st = stream if stream is not None else torch.cuda.current_stream(dev)
-    keep = (cols_o, cols_d, tmn, tmx)  # noqa: F841
+    on = _OnStream(stream, dev)
     check(lib().brt_traverse(scene.handle, geom_type, C.byref(rays), C.byref(_hits_struct(hits)), n, C.byref(L),
-                             C.c_void_p(st.cuda_stream)))
+                             on.handle))
+    on.hold(cols_o, cols_d, tmn, tmx, hits)
     return hits
 
 
@@ -284,13 +297,22 @@ class Batch:
         self.scenes = []
         self.hits = []
         arr = (capi.BatchItem * max(1, len(items)))()
+        import torch
+
+        devices = set()
         for i, it in enumerate(items):
             scene, org, dirs = it[0], it[1], it[2]
-            co, cd = _columns(org), _columns(dirs)
-            n = co[0].numel()
+            if scene.f32:
+                raise ValueError("batches hold Scene<double> items only")
+            co = _columns(org, what=f"item {i} origins")
             dev = co[0].device
-            tmn = _scalar_col(it[3], n, dev) if len(it) > 3 and it[3] is not None else None
-            tmx = _scalar_col(it[4], n, dev) if len(it) > 4 and it[4] is not None else None
+            cd = _columns(dirs, dev=dev, what=f"item {i} directions")
+            n = co[0].numel()
+            if cd[0].numel() != n:
+                raise ValueError(f"item {i}: {n} origins but {cd[0].numel()} directions")
+            devices.add(dev)
+            tmn = _scalar_col(it[3], n, dev, what="tmin") if len(it) > 3 and it[3] is not None else None
+            tmx = _scalar_col(it[4], n, dev, what="tmax") if len(it) > 4 and it[4] is not None else None
             h = _alloc_hits(n, dev, normals)
             self._keep.extend([co, cd, tmn, tmx])
             self.scenes.append(scene)
@@ -301,6 +323,12 @@ class Batch:
                                     tmx.data_ptr() if tmx is not None else None)
             arr[i].hits = _hits_struct(h)
             arr[i].n = n
+        if len(devices) > 1:
+            raise ValueError("all batch items must be on one GPU")
+        # the batch's columns and hit tensors live as long as the batch; they were written on the current stream,
+        # and launches, captures and replays may run on any stream, so finish those writes here once
+        for d in devices:
+            torch.cuda.current_stream(d).synchronize()
         L = capi.Launch(precision, capi.FLAG_ANY_HIT if any_hit else 0, block_size, capi.SCHEDULE_PERSISTENT, refill,
                         capi.SORT_NONE, stack_ring)
         h = C.c_void_p()
@@ -338,27 +366,110 @@ class Batch:
         self.close()
 
 
-def _columns(x, dtype=None):
-    """(n,3) tensor or three columns -> three contiguous columns (float64 unless `dtype`) on the same device."""
+def _columns(x, dtype=None, dev=None, what="rays"):
+    """(n,3) tensor or three columns -> three contiguous columns of `dtype` (float64 unless given), all on one GPU.
+
+    The columns are handed to kernels as raw device pointers, so anything else is refused before a launch: host
+    data (numpy arrays, CPU tensors), columns on different devices or of different lengths, a wrong shape, or a
+    device other than `dev` when given. A floating tensor of another precision is converted (a copy on the same
+    device); an integer or complex one is refused."""
     import torch
 
     dtype = dtype or torch.float64
     if isinstance(x, (list, tuple)) and len(x) == 3:
-        return [torch.as_tensor(c, dtype=dtype).contiguous() for c in x]
-    x = torch.as_tensor(x, dtype=dtype)
-    if not (x.dim() == 2 and x.shape[1] == 3):
-        raise ValueError("expected an (n,3) tensor or a tuple of three columns")
-    x = x.t()
-    return [x[k].contiguous() for k in range(3)]
+        cols = list(x)
+    elif torch.is_tensor(x) and x.dim() == 2 and x.shape[1] == 3:
+        cols = [x[:, k] for k in range(3)]
+    else:
+        raise ValueError(f"{what}: expected an (n,3) CUDA tensor or a tuple of three CUDA columns")
+    for c in cols:
+        if not torch.is_tensor(c):
+            raise ValueError(f"{what}: expected CUDA tensors (host arrays cannot be read by the kernels)")
+        if not c.is_floating_point():
+            raise ValueError(f"{what}: expected floating-point tensors, got {c.dtype}")
+        if c.dim() != 1:
+            raise ValueError(f"{what}: columns must be 1-D")
+    if len({c.numel() for c in cols}) != 1:
+        raise ValueError(f"{what}: the three columns differ in length")
+    if not all(c.is_cuda for c in cols):
+        raise ValueError(f"{what}: expected CUDA tensors (host arrays cannot be read by the kernels)")
+    d = cols[0].device if dev is None else torch.device(dev)
+    if any(c.device != d for c in cols):
+        raise ValueError(f"{what}: all tensors must be on {d}")
+    return [c.to(dtype).contiguous() for c in cols]
 
 
-def _scalar_col(v, n, dev, dtype=None):
+def _scalar_col(v, n, dev, dtype=None, what="tmin/tmax"):
+    """A per-ray FP column of length n on `dev`: a Python scalar is broadcast; a tensor must already be a CUDA tensor
+    on `dev` with n elements (a host array cannot be read by the kernels)."""
     import torch
 
-    t = torch.as_tensor(v, dtype=dtype or torch.float64, device=dev)
-    if t.dim() == 0:
-        t = t.expand(n)
-    return t.contiguous()
+    dtype = dtype or torch.float64
+    if not torch.is_tensor(v):
+        if np.ndim(v) != 0:
+            raise ValueError(f"{what}: give a scalar or a CUDA tensor with one value per ray")
+        return torch.full((n,), float(v), dtype=dtype, device=dev)
+    if not v.is_cuda or v.device != torch.device(dev):
+        raise ValueError(f"{what}: expected a CUDA tensor on {dev}")
+    if v.dim() == 0:
+        return v.to(dtype).expand(n).contiguous()
+    if v.numel() != n:
+        raise ValueError(f"{what}: expected {n} values, got {v.numel()}")
+    return v.to(dtype).reshape(-1).contiguous()
+
+
+def _check_hits(hits, n, dev, fdtype):
+    """A caller's in/out hit record: CUDA tensors on `dev`, contiguous, t/u/v of the scene type with n elements,
+    prim_id/geom_id int32 with n elements, and an optional (3, n) normal. Raises ValueError otherwise; nothing is
+    converted, since the kernels write into these tensors."""
+    import torch
+
+    spec = {"t": (fdtype, (n,)), "u": (fdtype, (n,)), "v": (fdtype, (n,)), "prim_id": (torch.int32, (n,)),
+            "geom_id": (torch.int32, (n,))}
+    if hits.get("normal") is not None:
+        spec["normal"] = (fdtype, (3, n))
+    for k, (dt, shape) in spec.items():
+        h = hits.get(k)
+        if not torch.is_tensor(h):
+            raise ValueError(f"hits[{k!r}]: expected a CUDA tensor on {dev}")
+        if h.dtype != dt or tuple(h.shape) != shape or not h.is_contiguous():
+            raise ValueError(f"hits[{k!r}]: expected a contiguous {dt} tensor of shape {shape}, got {h.dtype} "
+                             f"{tuple(h.shape)}")
+    for k in spec:
+        if not hits[k].is_cuda or hits[k].device != torch.device(dev):
+            raise ValueError(f"hits[{k!r}]: expected a CUDA tensor on {dev}")
+
+
+class _OnStream:
+    """Stream discipline of one asynchronous call. Temporaries (converted columns, broadcast tmin/tmax, new hit
+    tensors) are written on torch's current stream; the kernels run on `stream`. So `stream` first waits for the
+    current stream, and every tensor the kernels touch is recorded on `stream`, which keeps the caching allocator
+    from handing its memory to another tensor before the kernels are done with it."""
+
+    def __init__(self, stream, dev):
+        import torch
+
+        cur = torch.cuda.current_stream(dev)
+        self.stream = stream if stream is not None else cur
+        if self.stream != cur:
+            self.stream.wait_stream(cur)
+
+    @property
+    def handle(self):
+        return C.c_void_p(self.stream.cuda_stream)
+
+    def hold(self, *objs):
+        import torch
+
+        for o in objs:
+            if o is None:
+                continue
+            if isinstance(o, dict):
+                self.hold(*o.values())
+            elif isinstance(o, (list, tuple)):
+                self.hold(*o)
+            elif torch.is_tensor(o) and o.is_cuda:
+                o.record_stream(self.stream)
 
 
 # ------------------------------------------------------------------------------------------------ drop-in

@@ -604,6 +604,15 @@ orc_scene *orc_collection_scene(uint32_t min_leaf_primitives, uint32_t max_tree_
   return s;
 }
 
+/* Test hook: partition_idx on a caller's index range with the predicate centers[3 * i + axis] < cut. Returns the
+ * number of indices in the first part; idx is permuted in place. */
+uint32_t orc_partition_test(const real *centers, uint32_t *idx, uint32_t n, uint32_t axis, real cut) {
+  orc_coll c;
+  memset(&c, 0, sizeof c);
+  c.centers = (real *)centers;
+  return partition_idx(&c, idx, idx + n, axis, cut);
+}
+
 uint32_t orc_tree_size(const orc_scene *s, int type) { return s->c[type].present ? s->c[type].n_nodes : 0; }
 
 int orc_tree_export(const orc_scene *s, int type, orc_node *nodes, uint32_t *prim_order, orc_stats *stats) {
@@ -1170,4 +1179,30 @@ int orc_bruteforce_type(const orc_scene *s, int type, const orc_ray *ray, orc_hi
     }
   }
   return found;
+}
+
+/* orc_bruteforce_type over SoA rays (OpenMP over rays): t and user prim id of the closest accepted primitive of one
+ * type per ray, DBL_MAX / 0xFFFFFFFF where none accepts. */
+uint64_t orc_bruteforce_batch(const orc_scene *s, int type, uint64_t n, const real *ox, const real *oy, const real *oz,
+                              const real *dx, const real *dy, const real *dz, const real *tmin, const real *tmax,
+                              real *t, uint32_t *prim_id, int nthreads) {
+  uint64_t hits = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : hits)
+#endif
+  for (int64_t r = 0; r < (int64_t)n; r++) {
+    orc_ray ray;
+    const real o[3] = {ox[r], oy[r], oz[r]}, d[3] = {dx[r], dy[r], dz[r]};
+    orc_ray_init(&ray, o, d, tmin ? tmin[r] : R_(0.0), tmax ? tmax[r] : REAL_MAX, 0);
+    orc_hit h;
+    memset(&h, 0, sizeof h);
+    h.t = REAL_MAX;
+    h.prim_id = h.geom_id = 0xFFFFFFFFu;
+    const int hit = orc_bruteforce_type(s, type, &ray, &h);
+    hits += (uint64_t)hit;
+    t[r] = hit ? h.t : REAL_MAX;
+    prim_id[r] = hit ? h.prim_id : 0xFFFFFFFFu;
+  }
+  return hits;
 }

@@ -109,6 +109,8 @@ uint32_t orc_geom_id(const orc_scene *s, int type);
 /* built tree export (reference pre-order layout) */
 orc_scene *orc_collection_scene(uint32_t min_leaf_primitives, uint32_t max_tree_depth, uint32_t bin_size,
                                 const orc_real *box_min, const orc_real *box_max, const orc_real *centers, uint32_t n);
+/* test hook: the builder's std::partition restatement on idx[0..n) with centers[3 * i + axis] < cut */
+uint32_t orc_partition_test(const orc_real *centers, uint32_t *idx, uint32_t n, uint32_t axis, orc_real cut);
 uint32_t orc_tree_size(const orc_scene *s, int type);
 int orc_tree_export(const orc_scene *s, int type, orc_node *nodes, uint32_t *prim_order, orc_stats *stats);
 
@@ -133,6 +135,9 @@ uint64_t orc_intersect_batch(const orc_scene *s, uint64_t n, const orc_real *ox,
 
 /* brute-force closest candidate per type (no BVH): every primitive tested from a fresh intersector */
 int orc_bruteforce_type(const orc_scene *s, int type, const orc_ray *ray, orc_hit *hit);
+uint64_t orc_bruteforce_batch(const orc_scene *s, int type, uint64_t n, const orc_real *ox, const orc_real *oy,
+                              const orc_real *oz, const orc_real *dx, const orc_real *dy, const orc_real *dz,
+                              const orc_real *tmin, const orc_real *tmax, orc_real *t, uint32_t *prim_id, int nthreads);
 
 /* ---- multi-bounce mirror tracing (mirror_oracle.c; no counterpart in the reference) ----
  * Arrays of per-bounce outputs and upstream gradients are bounces * n long, indexed [b * n + i]; per-ray gradient

@@ -110,6 +110,8 @@ def lib():
         L.orc_intersect_batch.argtypes = [P, C.c_uint64] + [dp] * 8 + [dp, dp, dp, up, up, dp, dp, dp, C.c_void_p,
                                                                         C.c_int, C.c_int]
         L.orc_bruteforce_type.argtypes = [P, C.c_int, C.POINTER(Ray), C.POINTER(Hit)]
+        L.orc_bruteforce_batch.restype = C.c_uint64
+        L.orc_bruteforce_batch.argtypes = [P, C.c_int, C.c_uint64] + [dp] * 8 + [dp, up, C.c_int]
         L.orc_mirror_trace.restype = None
         L.orc_mirror_trace.argtypes = [P, dp, up, C.c_uint64] + [dp] * 8 + [C.c_uint32, C.c_double, dp, dp, dp, up] + \
             [dp] * 6 + [C.c_int]
@@ -117,6 +119,8 @@ def lib():
         L.orc_mirror_loss.argtypes = [dp, up, C.c_uint64] + [dp] * 6 + [C.c_uint32, up] + [dp] * 5 + [C.c_int]
         L.orc_prim_params.argtypes = [P, C.c_int, C.c_uint32, dp]
         L.orc_prim_count.restype = C.c_uint32
+        L.orc_geom_id.restype = C.c_uint32
+        L.orc_geom_id.argtypes = [P, C.c_int]
         L.orc_prim_count.argtypes = [P, C.c_int]
         for name, nargs in (("orc_sphere_distance", [dp, C.c_double, dp]),
                             ("orc_plane_distance", [dp, C.c_double, C.c_double, dp, dp]),
@@ -260,6 +264,36 @@ class Scene:
         hit = lib().orc_bruteforce_type(self.h, type_, C.byref(ray), C.byref(h))
         return bool(hit), h
 
+    def bruteforce_batch(self, type_, org, dirs, tmin=None, tmax=None, nthreads=0):
+        """Closest accepted primitive of one type per ray, every primitive tested (no BVH): (t, user prim id)."""
+        org = np.ascontiguousarray(org, dtype=np.float64).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        n = len(org)
+        cols = [np.ascontiguousarray(org[:, k]) for k in range(3)] + [np.ascontiguousarray(dirs[:, k])
+                                                                       for k in range(3)]
+        tmn = None if tmin is None else np.ascontiguousarray(np.broadcast_to(tmin, (n,)), dtype=np.float64)
+        tmx = None if tmax is None else np.ascontiguousarray(np.broadcast_to(tmax, (n,)), dtype=np.float64)
+        t = np.zeros(n)
+        prim = np.zeros(n, dtype=np.uint32)
+        lib().orc_bruteforce_batch(self.h, type_, n, *[_dp(c) for c in cols], _dp(tmn), _dp(tmx), _dp(t), _up(prim),
+                                   int(nthreads))
+        return t, prim
+
+    def bruteforce_merged(self, org, dirs, types=(TRIANGLES, SPHERES, PLANES), tmin=None, tmax=None, nthreads=0):
+        """blazert/scene.h:161-211 merge over per-type brute-force hits, in the reference's type order: the first type
+        present is copied, later ones replace it only when strictly closer. Returns (t, prim_id, geom_id)."""
+        n = len(np.asarray(org).reshape(-1, 3))
+        t = np.full(n, np.finfo(np.float64).max)
+        prim = np.full(n, INVALID, np.uint32)
+        geom = np.full(n, INVALID, np.uint32)
+        for ty in types:
+            if lib().orc_prim_count(self.h, ty) == 0:
+                continue
+            tt, pp = self.bruteforce_batch(ty, org, dirs, tmin, tmax, nthreads)
+            take = (pp != INVALID) & (tt < t)
+            t[take], prim[take], geom[take] = tt[take], pp[take], lib().orc_geom_id(self.h, ty)
+        return t, prim, geom
+
     def intersect_batch(self, org, dirs, tmin=None, tmax=None, normals=True, trace=False, nthreads=0, any_hit=False):
         """Batched intersect1 on the CPU. org/dirs: (n,3) float64. Returns a dict of numpy arrays."""
         org = np.ascontiguousarray(org, dtype=np.float64).reshape(-1, 3)
@@ -304,6 +338,41 @@ class Scene:
         lib().orc_closest_batch(self.h, n, _dp(px), _dp(py), _dp(pz), _dp(out["distance"]), _up(out["prim_id"]),
                                 _up(out["geom_id"]), int(nthreads))
         return out
+
+
+def classify_split_disagreements(scene: "Scene", org, dirs, idx, split_t, split_geom, cyl_geom=None, tmin=None,
+                                 tmax=None) -> dict:
+    """Explains the rays where the split-precision path (FP32 traversal, FP64 hit) and the FP64 reference disagree.
+
+    The split path tests every primitive the reference's FP64 traversal tests, plus possibly some in nodes the
+    reference culls; the primitive tests, hit distances and merge are FP64. So a disagreement can only be:
+    * a tie: a runner-up within 1e-12 relative of the reference's t in its traversal, or a split hit at the same
+      distance to 1e-12 (a candidate the reference never tested, within the -4 eps barycentric slack);
+    * a cylinder: either side's hit is a cylinder, or two cylinders accept the ray. The reference accepts a cylinder
+      without a distance check (last visited wins, blazert/primitives/cylinder.h:181-495), and bounds a rotated
+      cylinder by a box from 6 axis points that can miss part of it (cylinder.h:115-147);
+    * a reference stack overflow (28 slots, undefined behaviour in the reference).
+    idx are the disagreeing ray indices into org/dirs; split_t / split_geom the split path's t and geom_id on them;
+    cyl_geom the geometry id of the scene's cylinders. Returns counts and the indices nothing explains."""
+    idx = np.asarray(idx, dtype=np.int64)
+    if len(idx) == 0:
+        return {"disagree": 0, "tie": 0, "cylinder": 0, "overflow": 0, "unexplained": 0,
+                "unexplained_idx": np.zeros(0, np.int64)}
+    sel = lambda a: None if a is None else np.broadcast_to(np.asarray(a, np.float64), (len(org),))[idx]
+    ref = scene.intersect_batch(np.asarray(org)[idx], np.asarray(dirs)[idx], tmin=sel(tmin), tmax=sel(tmax),
+                                normals=False, trace=True)
+    tr = ref["trace"]
+    ot = np.asarray(split_t, np.float64)
+    og = np.asarray(split_geom).view(np.uint32) if np.asarray(split_geom).dtype.itemsize == 4 else \
+        np.asarray(split_geom, np.uint32)
+    tie = (tr["tie"] != 0) | (np.abs(ot - ref["t"]) <= 1e-12 * np.abs(ref["t"]))
+    cyl = tr["multi_cyl"] != 0
+    if cyl_geom is not None:
+        cyl |= (ref["geom_id"] == cyl_geom) | (og == cyl_geom)
+    ovf = tr["overflow"] != 0
+    un = ~tie & ~cyl & ~ovf
+    return {"disagree": int(len(idx)), "tie": int(tie.sum()), "cylinder": int((cyl & ~tie).sum()),
+            "overflow": int((ovf & ~tie & ~cyl).sum()), "unexplained": int(un.sum()), "unexplained_idx": idx[un]}
 
 
 def _v3(v):
