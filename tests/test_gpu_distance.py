@@ -84,7 +84,7 @@ def test_cylinder_known_answers_on_gpu(dev, oracle_mod):
         g.add_cylinders(np.array([ctr]), [a], [b], [h], np.array([R]))
         g.commit()
         pts = np.array([c["point"] for c in cases])
-        got = g.distance_to_surface(3, torch.from_numpy(pts).to(dev), torch.zeros(len(cases), dtype=torch.int32))
+        got = g.distance_to_surface(3, torch.from_numpy(pts).to(dev), torch.zeros(len(cases), dtype=torch.int32, device=dev))
         got = got.cpu().numpy()
         for c, d in zip(cases, got):
             exp = c["distance"]
