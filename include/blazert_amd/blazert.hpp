@@ -7,14 +7,17 @@
 //   blazert::RayHit<T>{normal, uv, hit_distance, prim_id, geom_id}
 //   blazert::Scene<T>::add_mesh / add_spheres / add_planes / add_cylinders / commit, build_options
 //   bool blazert::intersect1(const Scene<T>&, const Ray<T>&, RayHit<T>&)
-// The reference lets users substitute their own vector types (blazert/datatypes.h:28); these are small
-// aggregate types with operator[] (and operator() for matrices), which is all the scene API needs.
+// The reference's vector types are blaze's (blazert/datatypes.h:31-59); these are small aggregates with operator[]
+// (operator() for matrices) and the blaze arithmetic its callers use (+ - * /, dot, cross, norm, normalize, ...).
 // Geometry is copied into the engine at add time. intersect1 is one ray per call (a GPU round trip); batches
 // of device-resident rays go through brt_intersect() in blazert_amd.h.
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdint>
+#include <ostream>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -35,6 +38,11 @@ struct Vec3r {
   T &operator[](size_t i) { return v[i]; }
   const T &operator[](size_t i) const { return v[i]; }
   static constexpr size_t size() { return 3; }
+  Vec3r &operator+=(const Vec3r &b) { return *this = *this + b; }
+  Vec3r &operator-=(const Vec3r &b) { return *this = *this - b; }
+  Vec3r &operator*=(T s) { return *this = *this * s; }
+  Vec3r &operator/=(T s) { return *this = *this / s; }
+  Vec3r &operator*=(const Vec3r &b) { return *this = *this * b; }
 };
 
 template <typename T>
@@ -42,8 +50,10 @@ struct Vec2r {
   std::array<T, 2> v{};
   Vec2r() = default;
   Vec2r(T x, T y) : v{x, y} {}
+  explicit Vec2r(T s) : v{s, s} {}
   T &operator[](size_t i) { return v[i]; }
   const T &operator[](size_t i) const { return v[i]; }
+  static constexpr size_t size() { return 2; }
 };
 
 struct Vec3ui {
@@ -52,6 +62,7 @@ struct Vec3ui {
   Vec3ui(unsigned int a, unsigned int b, unsigned int c) : v{a, b, c} {}
   unsigned int &operator[](size_t i) { return v[i]; }
   const unsigned int &operator[](size_t i) const { return v[i]; }
+  static constexpr size_t size() { return 3; }
 };
 
 // Row-major 3x3 (blaze::rowMajor in the reference), constructible from nested initializer lists.
@@ -70,7 +81,122 @@ struct Mat3r {
   static Mat3r identity() { return Mat3r{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}; }
   T &operator()(size_t i, size_t j) { return m[3 * i + j]; }
   const T &operator()(size_t i, size_t j) const { return m[3 * i + j]; }
+  static constexpr size_t rows() { return 3; }
+  static constexpr size_t columns() { return 3; }
 };
+
+// Vector and matrix arithmetic of the blaze types the reference's callers use (blaze 3.7 semantics, evaluated in
+// the expression order the engine and its oracle assume; compile with -ffp-contract=off for the same bits):
+// componentwise + - and * between vectors, scalar * / (a division multiplies by the reciprocal), dot with the
+// sum ((a0 b0 + a1 b1) + a2 b2), cross, norm / length = sqrt(dot(a, a)), normalize(v) = v * (1 / norm(v)) (a zero
+// vector stays zero), componentwise min / max / abs, trans, matrix-vector and matrix-matrix products (row sums
+// left to right).
+template <typename T>
+inline Vec3r<T> operator+(const Vec3r<T> &a, const Vec3r<T> &b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+template <typename T>
+inline Vec3r<T> operator-(const Vec3r<T> &a, const Vec3r<T> &b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+template <typename T>
+inline Vec3r<T> operator-(const Vec3r<T> &a) { return {-a[0], -a[1], -a[2]}; }
+template <typename T>
+inline Vec3r<T> operator*(const Vec3r<T> &a, const Vec3r<T> &b) { return {a[0] * b[0], a[1] * b[1], a[2] * b[2]}; }
+template <typename T>
+inline Vec3r<T> operator/(const Vec3r<T> &a, const Vec3r<T> &b) { return {a[0] / b[0], a[1] / b[1], a[2] / b[2]}; }
+template <typename T>
+inline Vec3r<T> operator*(const Vec3r<T> &a, T s) { return {a[0] * s, a[1] * s, a[2] * s}; }
+template <typename T>
+inline Vec3r<T> operator*(T s, const Vec3r<T> &a) { return {s * a[0], s * a[1], s * a[2]}; }
+template <typename T>
+inline Vec3r<T> operator/(const Vec3r<T> &a, T s) { return a * (T(1) / s); }
+// scalar / vector: componentwise (1 / d in the reference's Ray constructor, blazert/ray.h:60)
+template <typename T>
+inline Vec3r<T> operator/(T s, const Vec3r<T> &a) { return {s / a[0], s / a[1], s / a[2]}; }
+template <typename T>
+inline bool operator==(const Vec3r<T> &a, const Vec3r<T> &b) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
+template <typename T>
+inline bool operator!=(const Vec3r<T> &a, const Vec3r<T> &b) { return !(a == b); }
+
+template <typename T>
+inline Vec2r<T> operator+(const Vec2r<T> &a, const Vec2r<T> &b) { return {a[0] + b[0], a[1] + b[1]}; }
+template <typename T>
+inline Vec2r<T> operator-(const Vec2r<T> &a, const Vec2r<T> &b) { return {a[0] - b[0], a[1] - b[1]}; }
+template <typename T>
+inline Vec2r<T> operator*(const Vec2r<T> &a, T s) { return {a[0] * s, a[1] * s}; }
+template <typename T>
+inline Vec2r<T> operator*(T s, const Vec2r<T> &a) { return {s * a[0], s * a[1]}; }
+
+template <typename T>
+inline T dot(const Vec3r<T> &a, const Vec3r<T> &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+template <typename T>
+inline Vec3r<T> cross(const Vec3r<T> &a, const Vec3r<T> &b) {
+  return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+template <typename T>
+inline T sqrNorm(const Vec3r<T> &a) { return dot(a, a); }
+template <typename T>
+inline T norm(const Vec3r<T> &a) { return std::sqrt(dot(a, a)); }
+template <typename T>
+inline T sqrLength(const Vec3r<T> &a) { return sqrNorm(a); }
+template <typename T>
+inline T length(const Vec3r<T> &a) { return norm(a); }
+template <typename T>
+inline Vec3r<T> normalize(const Vec3r<T> &a) {
+  const T len = norm(a);
+  return a * (len != T(0) ? T(1) / len : T(0));
+}
+template <typename T>
+inline Vec3r<T> min(const Vec3r<T> &a, const Vec3r<T> &b) {
+  return {std::min(a[0], b[0]), std::min(a[1], b[1]), std::min(a[2], b[2])};
+}
+template <typename T>
+inline Vec3r<T> max(const Vec3r<T> &a, const Vec3r<T> &b) {
+  return {std::max(a[0], b[0]), std::max(a[1], b[1]), std::max(a[2], b[2])};
+}
+template <typename T>
+inline Vec3r<T> abs(const Vec3r<T> &a) { return {std::abs(a[0]), std::abs(a[1]), std::abs(a[2])}; }
+template <typename T>
+inline T min(const Vec3r<T> &a) { return std::min(std::min(a[0], a[1]), a[2]); }
+template <typename T>
+inline T max(const Vec3r<T> &a) { return std::max(std::max(a[0], a[1]), a[2]); }
+
+template <typename T>
+inline Mat3r<T> trans(const Mat3r<T> &a) {
+  Mat3r<T> t;
+  for (size_t i = 0; i < 3; i++)
+    for (size_t j = 0; j < 3; j++) t(i, j) = a(j, i);
+  return t;
+}
+template <typename T>
+inline Vec3r<T> operator*(const Mat3r<T> &A, const Vec3r<T> &x) {
+  return {A(0, 0) * x[0] + A(0, 1) * x[1] + A(0, 2) * x[2], A(1, 0) * x[0] + A(1, 1) * x[1] + A(1, 2) * x[2],
+          A(2, 0) * x[0] + A(2, 1) * x[1] + A(2, 2) * x[2]};
+}
+template <typename T>
+inline Mat3r<T> operator*(const Mat3r<T> &A, const Mat3r<T> &B) {
+  Mat3r<T> C;
+  for (size_t i = 0; i < 3; i++)
+    for (size_t j = 0; j < 3; j++) C(i, j) = A(i, 0) * B(0, j) + A(i, 1) * B(1, j) + A(i, 2) * B(2, j);
+  return C;
+}
+template <typename T>
+inline Mat3r<T> operator*(const Mat3r<T> &A, T s) {
+  Mat3r<T> C;
+  for (size_t k = 0; k < 9; k++) C.m[k] = A.m[k] * s;
+  return C;
+}
+template <typename T>
+inline Mat3r<T> operator*(T s, const Mat3r<T> &A) { return A * s; }
+template <typename T>
+inline Mat3r<T> operator+(const Mat3r<T> &A, const Mat3r<T> &B) {
+  Mat3r<T> C;
+  for (size_t k = 0; k < 9; k++) C.m[k] = A.m[k] + B.m[k];
+  return C;
+}
+
+// blaze prints a column vector one element per line in parentheses
+template <typename T>
+inline std::ostream &operator<<(std::ostream &os, const Vec3r<T> &a) {
+  return os << "( " << a[0] << " )\n( " << a[1] << " )\n( " << a[2] << " )\n";
+}
 
 template <typename T>
 using Vec3rList = std::vector<Vec3r<T>>;
@@ -86,14 +212,18 @@ struct BVHBuildOptions {
   unsigned int bin_size = 64;
 };
 
-// blazert/ray.h:20-66: direction is normalised by the engine; 1/direction and the sign use the raw argument.
+// blazert/ray.h:20-66: direction is the normalised argument; direction_inv and direction_sign are taken from the
+// raw argument, as in the reference's constructor (its parameter shadows the member). The raw argument is kept for
+// the engine, which forms the same three fields on the device.
 template <typename T>
 class Ray {
  public:
   enum class CullBackFace { yes, no };
   enum class AnyHit { yes, no };
   const Vec3r<T> origin;
-  const Vec3r<T> direction;  // as given (the engine normalises it exactly like the reference constructor)
+  const Vec3r<T> direction;
+  const Vec3r<T> direction_inv;
+  const Vec3ui direction_sign;
   T min_hit_distance;
   T max_hit_distance;
   CullBackFace cull_back_face;
@@ -102,8 +232,14 @@ class Ray {
   Ray(const Vec3r<T> &origin, const Vec3r<T> &direction, T min_hit_distance = T(0.),
       T max_hit_distance = std::numeric_limits<T>::max(), CullBackFace cull_back_face = CullBackFace::no,
       AnyHit any_hit = AnyHit::no)
-      : origin(origin), direction(direction), min_hit_distance(min_hit_distance), max_hit_distance(max_hit_distance),
-        cull_back_face(cull_back_face), any_hit(any_hit) {}
+      : origin(origin), direction(normalize(direction)), direction_inv(T(1) / direction),
+        direction_sign(direction[0] < T(0) ? 1u : 0u, direction[1] < T(0) ? 1u : 0u, direction[2] < T(0) ? 1u : 0u),
+        min_hit_distance(min_hit_distance), max_hit_distance(max_hit_distance), cull_back_face(cull_back_face),
+        any_hit(any_hit), raw_direction_(direction) {}
+  const Vec3r<T> &raw_direction() const { return raw_direction_; }
+
+ private:
+  Vec3r<T> raw_direction_;
 };
 
 // blazert/ray.h:72-85
@@ -272,7 +408,8 @@ using Scene = BlazertScene<T>;
 template <typename T>
 bool intersect1(const BlazertScene<T> &scene, const Ray<T> &ray, RayHit<T> &rayhit) {
   const T o[3] = {ray.origin[0], ray.origin[1], ray.origin[2]};
-  const T d[3] = {ray.direction[0], ray.direction[1], ray.direction[2]};
+  const Vec3r<T> &raw = ray.raw_direction();
+  const T d[3] = {raw[0], raw[1], raw[2]};
   typename detail::Api<T>::Hit1 h{};
   for (int k = 0; k < 3; k++) h.normal[k] = rayhit.normal[k];
   h.u = rayhit.uv[0];
