@@ -272,7 +272,9 @@ def adjoint_leg(args, dev) -> dict:
     touched = np.unique(F[pid[valid]].reshape(-1))
     verts = np.concatenate([rng.choice(touched[touched < len(V) - 8], 60, replace=False), len(V) - 1 - np.arange(4)])
     scale = 1e-3 * np.abs(gv_h).max()
-    fd_err, fd_rich_err, fd_over = 0.0, 0.0, 0
+    fd_err, fd_rich_err, fd_over, fd_rich_over = 0.0, 0.0, 0, 0
+    fd_unexplained = 0  # above the bar by more than the estimated h^2 truncation error
+    trunc_above = []
 
     def central(vi, k, hh, idx):
         Vp, Vm = V.copy(), V.copy()
@@ -292,9 +294,16 @@ def adjoint_leg(args, dev) -> dict:
         d2 = central(vi, k, h / 2, idx)
         den = max(abs(gv_h[vi, k]), scale)
         e1 = abs(d1 - gv_h[vi, k]) / den
-        fd_over += int(e1 > 1e-6)
+        # central differences err by c h^2 + O(h^4): d(h) - d(h/2) = (3/4) c h^2 estimates the truncation at h
+        trunc = (4.0 / 3.0) * abs(d1 - d2) / den
+        er = abs((4 * d2 - d1) / 3 - gv_h[vi, k]) / den
+        if e1 > 1e-6:
+            fd_over += 1
+            trunc_above.append(trunc)
+            fd_unexplained += int(e1 - trunc > 1e-6)
         fd_err = max(fd_err, e1)
-        fd_rich_err = max(fd_rich_err, abs((4 * d2 - d1) / 3 - gv_h[vi, k]) / den)
+        fd_rich_err = max(fd_rich_err, er)
+        fd_rich_over += int(er > 1e-6)
 
     # central differences on 64 rays, every perturbed ray re-traced on the GPU
     ridx = np.arange(0, n, n // 64)[:64]
@@ -334,6 +343,11 @@ def adjoint_leg(args, dev) -> dict:
             "forward_bit_exact_vs_oracle": fwd_exact, "forward_sample_rays": ns,
             "max_rel_err_vs_analytic_cpu": an_err, "analytic_sample_rays": na,
             "fd_vertices_max_rel_err": fd_err, "fd_vertices": 64, "fd_vertices_above_1e-6": fd_over,
+            "fd_bar": {"h": h, "bar": 1e-6, "relative_to": "max(|g|, 1e-3 * max|dL/dvertex|)",
+                       "central_met": fd_over == 0,
+                       "central_misses_truncation_estimates": [float(x) for x in trunc_above],
+                       "central_misses_beyond_truncation": fd_unexplained,
+                       "richardson_h_h2_max_rel_err": fd_rich_err, "richardson_met": fd_rich_over == 0},
             "fd_vertices_richardson_max_rel_err": fd_rich_err,
             "fd_rays_max_rel_err": ray_err, "fd_rays": 64, "fd_ray_perturbations_flipped": flipped}
 
@@ -396,6 +410,61 @@ def distance_leg(args, dev, v, f, scene, osc) -> dict:
     return {"closest_surface_mpoints_s": round(n / ms_c / 1e3, 3), "closest_ms": round(ms_c, 3), "points": n,
             "closest_bit_exact_vs_bruteforce_sample": exact, "sample": ns,
             "pairs_mpairs_s": round(npair / ms_p / 1e3, 2), "pairs": npair}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(osc, org, dirs, ns, threads) -> dict:
+    """The oracle (the reference-exact C restatement of intersect1, gcc -O3 for generic x86-64, no FMA contraction)
+    timed on this host: plain intersect1 (hit records only, no traversal trace, no normals), OpenMP
+    schedule(dynamic, 1024) over rays on every core of this process's affinity, and on one core over the first
+    quarter of the sample."""
+    t0 = time.perf_counter()
+    osc.intersect_batch(org[:ns], dirs[:ns], normals=False, trace=False, nthreads=threads)
+    all_s = time.perf_counter() - t0
+    n1 = max(1, ns // 4)
+    t0 = time.perf_counter()
+    osc.intersect_batch(org[:n1], dirs[:n1], normals=False, trace=False, nthreads=1)
+    one_s = time.perf_counter() - t0
+    log(f"cpu baseline: {ns / all_s / 1e6:.2f} Mrays/s on {threads} threads, {n1 / one_s / 1e6:.3f} on 1")
+    return {"value": round(ns / all_s / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "one_core_value": round(n1 / one_s / 1e6, 4), "cpu_model": cpu_model(),
+            "sample": f"{ns} of the incoherent rays on {threads} threads ({all_s:.2f} s wall) and the first {n1} "
+                      f"on 1 thread ({one_s:.2f} s); oracle intersect1 without trace or normals, gcc -O3 generic "
+                      "x86-64, OpenMP schedule(dynamic,1024)"}
+
+
+def kernel_sources_hash() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("kernels.hip", "engine.h", "layout.h"):
+        h.update(open(os.path.join(ROOT, "blazert_amd", "csrc", name), "rb").read())
+    h.update(open(os.path.join(ROOT, "include", "blazert_amd", "device", "exact.h"), "rb").read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP64 traversal kernel from profiles/traffic_fp64_icosphere.json (rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE of this workload, copy-calibrated). Reported only when the kernel sources hash to the
+    value recorded with the measurement; otherwise null, since the figure would describe other code."""
+    tfile = os.path.join(ROOT, "profiles", "traffic_fp64_icosphere.json")
+    if not os.path.exists(tfile):
+        return None, "no measurement (profiles/traffic_fp64_icosphere.json missing)"
+    rec = json.load(open(tfile))
+    cur = kernel_sources_hash()
+    if rec.get("kernel_sources_sha16") != cur:
+        return None, (f"profiles/traffic_fp64_icosphere.json was measured on kernel sources "
+                      f"{rec.get('kernel_sources_sha16')}, these are {cur}: not reported")
+    return rec.get("traffic_bytes"), ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this workload, "
+                                      f"copy-calibrated: profiles/traffic_fp64_icosphere.json (kernel sources {cur})")
 
 
 def float_scene_leg(args, dev, v, f, org, dirs) -> dict:
@@ -593,16 +662,14 @@ def main():
                                "fp32_trav_mrays_s": round(nc / cms32 / 1e3, 3)},
     }
 
-    # algorithmic bytes per ray from the reference traversal replay (oracle trace on the first rays)
+    # algorithmic bytes per ray from the reference traversal replay (oracle trace on the rays; not timed)
     from oracle import oracle as OR
     osc = OR.Scene()
     osc.add_mesh(v, f)
     osc.commit()
     ns = min(args.cpu_rays, n)
     threads = len(os.sched_getaffinity(0))
-    t0 = time.perf_counter()
     ref = osc.intersect_batch(org[:ns], dirs[:ns], normals=True, trace=True, nthreads=threads)
-    cpu_s = time.perf_counter() - t0
     tr = ref["trace"]
     visits = float(tr["node_visits"].mean())
     ptests = float(tr["prim_tests"].mean())
@@ -613,14 +680,18 @@ def main():
     exact = bool(np.array_equal(g_sub_t[ok].view(np.uint64), ref["t"][ok].view(np.uint64))
                  and np.array_equal(g_sub_p[ok], ref["prim_id"][ok]))
     achieved = bpr64 * n / (ms64 * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_fp64_icosphere.json")
-    if os.path.exists(tfile):  # rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel, copy-calibrated (see the file)
-        traffic = json.load(open(tfile)).get("traffic_bytes")
+    traffic, traffic_note = measured_traffic()
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
                        "bytes_per_ray": round(bpr64, 1), "node_visits_per_ray": round(visits, 2),
                        "tri_tests_per_ray": round(ptests, 2)}
+    # split precision: the same work numerator (the reference traversal's node visits and triangle tests) with the
+    # split path's node record size, 32 B of FP32 box + topology per node
+    bpr32 = S_RAY_HIT + visits * S_NODE32 + ptests * S_TRI
+    ach32 = bpr32 * n / (ms32 * 1e-3) / 1e9
+    out["split_precision"]["roofline"] = {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
+                                          "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
+                                          "bytes_per_ray": round(bpr32, 1)}
     out["parity"] = {"sample_rays": ns, "fp64_bit_exact_vs_oracle": exact,
                      "reference_stack_overflow_rays": int((~ok).sum())}
     out["split_precision"]["disagreements"] = classify_disagreements(osc, org, dirs, dis_idx, dis_other)
@@ -639,11 +710,7 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args, dev)
     if not args.no_cpu:
-        out["cpu_baseline"] = {"value": round(ns / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"{ns} of the {n} incoherent rays, oracle intersect1 with traversal "
-                                         f"trace, OpenMP schedule(dynamic,1024) on {threads} threads, "
-                                         f"{cpu_s:.2f} s wall"}
+        out["cpu_baseline"] = cpu_baseline(osc, org, dirs, ns, threads)
     print(json.dumps(out), flush=True)
 
 

@@ -64,6 +64,39 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     return out
 
 
+SANITIZED_LIB = os.path.join(HERE, "_asan", "libblazert_amd.so")
+
+
+def build_sanitized(verbose: bool = False, out: str = SANITIZED_LIB) -> str:
+    """The library with its host code (scene.cpp: builder, collections, C ABI; autotune.cpp) built by g++ with
+    AddressSanitizer and UndefinedBehaviorSanitizer; the device code is compiled as usual. For CPU test runs
+    (tests/run_sanitized.sh loads it through BLAZERT_AMD_LIB, with libasan / libubsan preloaded into Python)."""
+    cc = hipcc()
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = tempfile.mkdtemp(prefix="brt_asan_")
+    try:
+        objs = []
+        for src in ("kernels.hip", "mirror.hip", "build.hip", "distance.hip"):
+            obj = os.path.join(tmp, src.replace(".hip", ".o"))
+            _run([cc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, src), "-o", obj])
+            objs.append(obj)
+        san = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
+        for src in ("scene.cpp", "autotune.cpp"):
+            obj = os.path.join(tmp, src.replace(".cpp", ".o"))
+            _run(["g++", "-O1", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-D__HIP_PLATFORM_AMD__",
+                  f"-I{ROCM}/include", *san, "-c", os.path.join(CSRC, src), "-o", obj])
+            objs.append(obj)
+        out_tmp = out + ".tmp"
+        _run(["g++", "-shared", "-fPIC", "-pthread", *san, *objs, f"-L{ROCM}/lib", "-lamdhip64",
+              f"-Wl,-rpath,{ROCM}/lib", "-o", out_tmp])
+        os.replace(out_tmp, out)
+        if verbose:
+            print("built", out)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 USER_GEOMETRY_EXAMPLE = os.path.join(os.path.dirname(HERE), "examples", "user_geometry", "user_geometry.hip")
 USER_GEOMETRY_EXAMPLE_LIB = os.path.join(os.path.dirname(HERE), "examples", "user_geometry", "libuser_geometry.so")
@@ -83,5 +116,8 @@ def build_user_geometry(src: str, out: str, verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
-    build_user_geometry(USER_GEOMETRY_EXAMPLE, USER_GEOMETRY_EXAMPLE_LIB, verbose=True)
+    if sys.argv[1:] == ["--sanitize"]:
+        build_sanitized(verbose=True)
+    else:
+        build(verbose=True)
+        build_user_geometry(USER_GEOMETRY_EXAMPLE, USER_GEOMETRY_EXAMPLE_LIB, verbose=True)

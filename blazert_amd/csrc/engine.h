@@ -40,6 +40,14 @@ __host__ __device__ inline int collapse4(const brt_tree_node *T, uint32_t b, uin
   return m;
 }
 
+// Device counters of one traversal launch, in a group cleared by the launch itself: the persistent schedule's ray
+// pool; a flag a split-precision launch sets when it defers rays to the exact path; the ray pool of the FP64 launch
+// that traces them. Launches on one stream run in order, so one group per stream serves all of them.
+constexpr int COUNTER_POOL = 0;
+constexpr int COUNTER_DEFERRED = 1;
+constexpr int COUNTER_DEFERRED_POOL = 2;
+constexpr int COUNTER_GROUP = 4;  // slots per group (one spare)
+
 constexpr uint32_t SCHEDULE_PERSISTENT = 0;  // persistent waves with per-lane ray refill (default)
 constexpr uint32_t SCHEDULE_PER_THREAD = 1;  // one ray per thread, grid-stride
 
@@ -50,7 +58,7 @@ struct TraceLaunch {
   bool first;          // first geometry type of the scene: initialises every hit record
   uint32_t schedule;   // SCHEDULE_*
   uint32_t refill;     // persistent: refill a wave once this many lanes are idle (1..64)
-  unsigned long long *counter;  // persistent: device ray counter for this launch (zeroed on the stream)
+  unsigned long long *counter;  // COUNTER_GROUP device counters for this launch (cleared on the stream)
   const uint32_t *order;        // optional ray permutation (sorted launches); kernels trace rays[order[k]]
   uint32_t ring = 0;            // LDS stack slots per lane: 4, 8 or 16 (0 = default of the precision)
   bool overwrite = false;       // every hit replaces the record (traverse semantics), not only closer ones
@@ -84,6 +92,7 @@ struct SortWorkspace {
 
 // Grows ws to hold n rays (frees the previous buffers; not capturable in a graph).
 hipError_t sort_workspace_reserve(SortWorkspace &ws, uint64_t n);
+
 // Computes the sort keys (mode 1: direction octant + origin Morton code, mode 2: Morton only) and radix-sorts the
 // ray indices on `stream`; *order points into ws afterwards.
 hipError_t sort_rays(const RaysView &rays, uint64_t n, const double lo[3], const double hi[3], uint32_t mode,

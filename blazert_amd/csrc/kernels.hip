@@ -30,6 +30,10 @@ namespace {
 constexpr uint32_t PREC_FP64 = 0;
 constexpr uint32_t FLAG_INOUT = 1u;
 constexpr uint32_t FLAG_ANY_HIT = 2u;
+// Internal: an FP64 launch that traces only the rays a split-precision launch of the same geometry type deferred
+// (see deferred_to_exact), and none at all when that launch deferred nothing.
+constexpr uint32_t FLAG_DEFERRED_ONLY = 1u << 31;
+
 // Deepest stack a traversal can build: the binary FP64 DFS pushes at most one entry per level (trees are limited
 // to depth 63 at upload); the 4-wide FP32 DFS at most three per Node4 record, and a record spans at least one
 // binary level on every path, so at most 3 * 63 = 189.
@@ -40,6 +44,18 @@ constexpr int stack_cap() { return SPLIT ? 192 : 64; }
 // with user-collection traversal: include/blazert_amd/device/exact.h.
 using namespace exact;
 using Ray = RayT<double>;
+
+// Rays the split-precision path hands to the exact FP64 path: those whose reference slab test can produce a NaN
+// term or culls boxes its line crosses, i.e. every ray without RAY_NAN_FREE (a zero, non-finite or reciprocal-
+// overflowing direction component, a non-finite origin, a NaN tmin). For a raw component -0.0 the reference takes
+// the near plane from `direction < 0` (false) but multiplies by 1/-0.0 = -inf, so it culls every box the ray's line
+// lies in; an origin on a box plane of a zero component gives 0 * inf = NaN. The FP32 path's |1/d| > 1e30 axes
+// never cull, so on such rays it would reach primitives the reference cannot. Every other ray has NaN-free slab
+// terms in the reference, and the FP32 path's conservative boxes visit a superset of its nodes.
+template <class R>
+__device__ __forceinline__ bool deferred_to_exact(const RayT<R> &r) {
+  return !(r.sign & RAY_NAN_FREE);
+}
 
 // A pointer into device memory viewed in the global address space. Pointers loaded from memory (the batch kernel's
 // job table) are generic, and accesses through generic pointers are flat accesses; through this view they are global.
@@ -844,11 +860,13 @@ template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysViewT<typename Rec::real> rays,
                                                HitsViewT<typename Rec::real> hits, uint64_t n, uint32_t flags,
                                                uint32_t first, uint32_t is_tri, double S, const uint32_t *order,
-                                               uint32_t ring) {
+                                               uint32_t ring, unsigned long long *deferred) {
   using R = typename Rec::real;
   extern __shared__ unsigned char smem[];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
+  const bool deferred_only = !SPLIT && (flags & FLAG_DEFERRED_ONLY);
+  if (deferred_only && __atomic_load_n(deferred, __ATOMIC_RELAXED) == 0ull) return;
   SpillArea<SPLIT, R> spill;
   RingStack<SPLIT, R> st;
   ring_bind<SPLIT, R>(st, smem, &spill, ring);
@@ -858,6 +876,11 @@ __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysViewT<typename R
     Ray32 q;
     HitT<R> s;
     start_ray<SPLIT>(bvh, rays, i, S, r, q, s);
+    if (SPLIT && deferred_to_exact(r)) {
+      __atomic_store_n(deferred, 1ull, __ATOMIC_RELAXED);
+      continue;
+    }
+    if (deferred_only && !deferred_to_exact(r)) continue;
     Trav t;
     trav_begin<SPLIT>(bvh, r, q, s, t);
     st.reset();
@@ -871,10 +894,11 @@ __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysViewT<typename R
 // nodes) then no longer idles the wave until its slowest ray ends. The pool is a range of indices claimed from the
 // launch's counter with one atomic: every claim on that one address serialises across the GPU, and claiming per
 // refill made the counter a bottleneck at small refill thresholds. A claim takes POOL_BIG indices while the wave's
-// previous claim ended more than two big pools per wave before n, and POOL_CHUNK (or the number of idle lanes, if
-// larger) after that, so the final claims stay small. Every lane leaves the loop once the counter has passed n, the
-// pool is empty and its own ray is done, so the grid always drains; at the end a wave holds at most POOL_CHUNK
-// unstarted rays (one per lane).
+// own previous claim ended more than two big pools per wave before n, and POOL_CHUNK (or the number of idle lanes,
+// if larger) after that. The switch reads the wave's own last claim, not the counter, so a wave that lags behind
+// the others can still make a POOL_BIG claim near the end: the last claims are mostly, not always, small. Every
+// lane leaves the loop once the counter has passed n, the pool is empty and its own ray is done, so the grid
+// always drains.
 constexpr uint32_t POOL_CHUNK = 64;
 constexpr uint32_t POOL_BIG = 128;
 
@@ -923,10 +947,12 @@ template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
 k_trace_persistent(BvhView bvh, RaysViewT<typename Rec::real> rays, HitsViewT<typename Rec::real> hits, uint64_t n,
                    uint32_t flags, uint32_t first, uint32_t is_tri, double S, unsigned long long *counter,
-                   uint32_t refill, const uint32_t *order, uint32_t ring) {
+                   uint32_t refill, const uint32_t *order, uint32_t ring, unsigned long long *deferred) {
   using R = typename Rec::real;
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
+  const bool deferred_only = !SPLIT && (flags & FLAG_DEFERRED_ONLY);
+  if (deferred_only && __atomic_load_n(deferred, __ATOMIC_RELAXED) == 0ull) return;
   const uint32_t lane = __lane_id();
   SpillArea<SPLIT, R> spill;
   RingStack<SPLIT, R> st;
@@ -950,14 +976,20 @@ k_trace_persistent(BvhView bvh, RaysViewT<typename Rec::real> rays, HitsViewT<ty
         pool_refill(pool, counter, n, idle, nidle, lane, got, cand, exhausted);
         if (got) {
           idx = order ? (uint64_t)order[cand] : cand;
-          has_ray = true;
           start_ray<SPLIT>(bvh, rays, idx, S, r, q, s);
-          trav_begin<SPLIT>(bvh, r, q, s, t);
-          st.reset();
+          if (SPLIT && deferred_to_exact(r)) {
+            __atomic_store_n(deferred, 1ull, __ATOMIC_RELAXED);
+          } else if (!deferred_only || deferred_to_exact(r)) {
+            has_ray = true;
+            trav_begin<SPLIT>(bvh, r, q, s, t);
+            st.reset();
+          }
         }
       }
     }
-    if (!__ballot(has_ray)) break;
+    // lanes may drop the rays they were handed (deferred or skipped), so an idle wave leaves only once the pool is
+    // exhausted
+    if (!__ballot(has_ray) && exhausted) break;
     if (has_ray) {
       if (t.active) trav_step<Rec, SPLIT>(bvh, r, q, any_hit, s, t, st);
       if (!t.active) {
@@ -985,9 +1017,11 @@ __device__ __forceinline__ uint32_t find_job(const BatchJob *jobs, uint32_t njob
 template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
 k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total, uint32_t flags,
-              unsigned long long *counter, uint32_t refill, uint32_t ring) {
+              unsigned long long *counter, uint32_t refill, uint32_t ring, unsigned long long *deferred) {
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
+  const bool deferred_only = !SPLIT && (flags & FLAG_DEFERRED_ONLY);
+  if (deferred_only && __atomic_load_n(deferred, __ATOMIC_RELAXED) == 0ull) return;
   const uint32_t lane = __lane_id();
   SpillArea<SPLIT> spill;
   RingStack<SPLIT> st;
@@ -1016,15 +1050,19 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
           const BatchJob &J = jobs[j];
           const uint64_t local = g - J.offset;
           idx = J.order ? (uint64_t)J.order[local] : local;
-          has_ray = true;
           bv = J.bvh;
           start_ray<SPLIT>(bv, J.rays, idx, J.extent, r, q, s);
-          trav_begin<SPLIT>(bv, r, q, s, t);
-          st.reset();
+          if (SPLIT && deferred_to_exact(r)) {
+            __atomic_store_n(deferred, 1ull, __ATOMIC_RELAXED);
+          } else if (!deferred_only || deferred_to_exact(r)) {
+            has_ray = true;
+            trav_begin<SPLIT>(bv, r, q, s, t);
+            st.reset();
+          }
         }
       }
     }
-    if (!__ballot(has_ray)) break;
+    if (!__ballot(has_ray) && exhausted) break;
     if (has_ray) {
       if (t.active) trav_step<Rec, SPLIT>(bv, r, q, any_hit, s, t, st);
       if (!t.active) {
@@ -1036,7 +1074,11 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
   }
 }
 
-__global__ void k_zero_counter(unsigned long long *counter) { *counter = 0ull; }
+// Clears a launch's counter group (a fill kernel, not hipMemsetAsync, so that graph captures replay correctly).
+__global__ void k_zero_counters(unsigned long long *c) {
+#pragma unroll
+  for (int k = 0; k < COUNTER_GROUP; k++) c[k] = 0ull;
+}
 
 // 3 x 9-bit Morton interleave
 __device__ __forceinline__ uint32_t spread9(uint32_t x) {
@@ -1084,38 +1126,63 @@ __global__ void k_init_hits(HitsViewT<R> hits, uint64_t n) {
   }
 }
 
+// Blocks of a persistent launch: as many as fit on the device at once, but no more than the rays need.
+template <class K>
+hipError_t persistent_blocks(K kernel, uint32_t block, size_t lds, uint64_t n, uint64_t &blocks) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kernel, (int)block, lds);
+  if (e != hipSuccess) return e;
+  blocks = (uint64_t)cus * (uint64_t)(per_cu > 0 ? per_cu : 1);
+  const uint64_t need = (n + block - 1) / block;
+  if (blocks > need) blocks = need;
+  return hipSuccess;
+}
+
+// Enqueues one traversal of a geometry type: the counter group is cleared, then the trace kernel runs. A split-
+// precision launch is followed by the FP64 launch over the rays it deferred (deferred_to_exact), which returns at
+// once when there are none; both run in stream order before the next geometry type, so intersect1's merge order
+// holds for the deferred rays too.
 template <class Rec, bool SPLIT>
 hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real> &rays,
                          const HitsViewT<typename Rec::real> &hits, uint64_t n, const TraceLaunch &cfg,
                          const SplitParams &sp, uint32_t is_tri, hipStream_t stream) {
+  using R = typename Rec::real;
   const uint32_t block = cfg.block ? cfg.block : 256;
   const uint32_t first = cfg.first ? 1u : 0u;
   const uint32_t ring = cfg.ring ? cfg.ring : ring_default<SPLIT>();
-  const size_t lds = ring_lds_bytes<SPLIT, typename Rec::real>(block, ring);
+  const size_t lds = ring_lds_bytes<SPLIT, R>(block, ring);
+  unsigned long long *grp = cfg.counter;
+  if (!grp) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, stream, grp);
   if (cfg.schedule == SCHEDULE_PERSISTENT) {
-    if (!cfg.counter) return hipErrorInvalidValue;
-    int dev = 0, cus = 0, per_cu = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess)
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_trace_persistent<Rec, SPLIT>,
-                                                       (int)block, lds);
+    uint64_t blocks = 0;
+    hipError_t e = persistent_blocks(k_trace_persistent<Rec, SPLIT>, block, lds, n, blocks);
     if (e != hipSuccess) return e;
-    uint64_t blocks = (uint64_t)cus * (uint64_t)(per_cu > 0 ? per_cu : 1);
-    const uint64_t need = (n + block - 1) / block;
-    if (blocks > need) blocks = need;
-    if (blocks == 0) return hipSuccess;
     const uint32_t refill = cfg.refill ? cfg.refill : refill_default<Rec, SPLIT>();
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
     hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
-                       hits, n, cfg.flags, first, is_tri, sp.scene_extent, cfg.counter, refill, cfg.order, ring);
-    return hipGetLastError();
+                       hits, n, cfg.flags, first, is_tri, sp.scene_extent, grp + COUNTER_POOL, refill, cfg.order,
+                       ring, grp + COUNTER_DEFERRED);
+  } else {
+    uint64_t blocks = (n + block - 1) / block;
+    if (blocks > (1ull << 30)) blocks = 1ull << 30;
+    hipLaunchKernelGGL((k_trace<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays, hits, n,
+                       cfg.flags, first, is_tri, sp.scene_extent, cfg.order, ring, grp + COUNTER_DEFERRED);
   }
-  uint64_t blocks = (n + block - 1) / block;
-  if (blocks > (1ull << 30)) blocks = 1ull << 30;
-  if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_trace<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays, hits, n,
-                     cfg.flags, first, is_tri, sp.scene_extent, cfg.order, ring);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !SPLIT) return e;
+  // the FP64 launch over the deferred rays (default launch shape of the exact path)
+  const uint32_t ring64 = ring_default<false>();
+  const size_t lds64 = ring_lds_bytes<false, R>(256, ring64);
+  uint64_t blocks = 0;
+  e = persistent_blocks(k_trace_persistent<Rec, false>, 256, lds64, n, blocks);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_trace_persistent<Rec, false>), dim3((uint32_t)blocks), dim3(256), lds64, stream, bvh, rays,
+                     hits, n, cfg.flags | FLAG_DEFERRED_ONLY, first, is_tri, sp.scene_extent,
+                     grp + COUNTER_DEFERRED_POOL, refill_default<Rec, false>(), cfg.order, ring64,
+                     grp + COUNTER_DEFERRED);
   return hipGetLastError();
 }
 
@@ -1125,22 +1192,25 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
   const uint32_t block = cfg.block ? cfg.block : 256;
   const uint32_t ring = cfg.ring ? cfg.ring : ring_default<SPLIT>();
   const size_t lds = ring_lds_bytes<SPLIT>(block, ring);
-  if (!cfg.counter || njobs == 0) return hipErrorInvalidValue;
-  int dev = 0, cus = 0, per_cu = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_trace_batch<Rec, SPLIT>, (int)block,
-                                                     lds);
+  unsigned long long *grp = cfg.counter;
+  if (!grp || njobs == 0) return hipErrorInvalidValue;
+  if (total == 0) return hipSuccess;
+  uint64_t blocks = 0;
+  hipError_t e = persistent_blocks(k_trace_batch<Rec, SPLIT>, block, lds, total, blocks);
   if (e != hipSuccess) return e;
-  uint64_t blocks = (uint64_t)cus * (uint64_t)(per_cu > 0 ? per_cu : 1);
-  const uint64_t need = (total + block - 1) / block;
-  if (blocks > need) blocks = need;
-  if (blocks == 0) return hipSuccess;
   const uint32_t refill = cfg.refill ? cfg.refill : refill_default<Rec, SPLIT>();
-  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, cfg.counter);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, stream, grp);
   hipLaunchKernelGGL((k_trace_batch<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, jobs, njobs, total,
-                     cfg.flags, cfg.counter, refill, ring);
+                     cfg.flags, grp + COUNTER_POOL, refill, ring, grp + COUNTER_DEFERRED);
+  e = hipGetLastError();
+  if (e != hipSuccess || !SPLIT) return e;
+  const uint32_t ring64 = ring_default<false>();
+  const size_t lds64 = ring_lds_bytes<false>(256, ring64);
+  e = persistent_blocks(k_trace_batch<Rec, false>, 256, lds64, total, blocks);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_trace_batch<Rec, false>), dim3((uint32_t)blocks), dim3(256), lds64, stream, jobs, njobs, total,
+                     cfg.flags | FLAG_DEFERRED_ONLY, grp + COUNTER_DEFERRED_POOL, refill_default<Rec, false>(),
+                     ring64, grp + COUNTER_DEFERRED);
   return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -116,6 +117,27 @@ struct BoxesT {
 
 using Boxes = BoxesT<double>;
 
+// Primitive records are built in a byte buffer and accessed as TriRec / SphRec / PlaRec / CylRec (or their FP32
+// forms); the buffer is aligned like the most aligned of them (128 B), which std::vector<unsigned char> is not.
+template <class T, size_t Align>
+struct AlignedAllocator {
+  using value_type = T;
+  template <class U>
+  struct rebind {
+    using other = AlignedAllocator<U, Align>;
+  };
+  AlignedAllocator() = default;
+  template <class U>
+  AlignedAllocator(const AlignedAllocator<U, Align> &) {}
+  T *allocate(size_t n) { return static_cast<T *>(::operator new(n * sizeof(T), std::align_val_t(Align))); }
+  void deallocate(T *p, size_t) { ::operator delete(p, std::align_val_t(Align)); }
+  template <class U>
+  bool operator==(const AlignedAllocator<U, Align> &) const { return true; }
+  template <class U>
+  bool operator!=(const AlignedAllocator<U, Align> &) const { return false; }
+};
+using RecordBytes = std::vector<unsigned char, AlignedAllocator<unsigned char, 128>>;
+
 template <class R>
 struct GeometryT {
   bool present = false;
@@ -134,7 +156,7 @@ struct GeometryT {
   std::vector<Node64> n64;  // double scenes
   std::vector<Node4> n4;
   std::vector<Node32> n32;  // Scene<float>
-  std::vector<unsigned char> recs;
+  RecordBytes recs;
   size_t rec_size = 0;
   uint32_t num_n4 = 0;
   BvhView view{};
@@ -776,14 +798,14 @@ void build_geometry(GeometryT<R> &g, GeomType type, const brt_build_options &o) 
 }
 
 // ------------------------------------------------------------------------------------------------ device
-constexpr uint32_t NUM_COUNTERS = 256;
-
 struct DeviceScene {
   int device = -1;
   std::vector<void *> allocs;
   BvhView views[NUM_TYPES]{};
-  unsigned long long *counters = nullptr;  // ray counters of the persistent schedule, one per launch in flight
-  std::atomic<uint32_t> next_counter{0};
+  // One group of launch counters per stream (COUNTER_GROUP): launches on one stream run in order and each clears its
+  // group first, so they can share it; launches on different streams never share one.
+  std::mutex counter_mu;
+  std::vector<std::pair<hipStream_t, unsigned long long *>> stream_counters;
   std::mutex inv_mu;
   uint32_t *inv_order[NUM_TYPES]{};  // primitive id -> leaf position, made on the first distance query
   double *leaf_vertices = nullptr;    // triangles' vertices in leaf order, made on the first closest-surface query
@@ -792,7 +814,18 @@ struct DeviceScene {
   ~DeviceScene() {
     for (auto &w : sorts)
       if (w.second->base) (void)hipFree(w.second->base);
+    for (auto &c : stream_counters) (void)hipFree(c.second);
     for (void *p : allocs) (void)hipFree(p);
+  }
+  // The counter group of `st`, allocated on the stream's first launch (NULL if that allocation fails).
+  unsigned long long *counter_group(hipStream_t st) {
+    std::lock_guard<std::mutex> lk(counter_mu);
+    for (auto &c : stream_counters)
+      if (c.first == st) return c.second;
+    void *p = nullptr;
+    if (hipMalloc(&p, COUNTER_GROUP * sizeof(unsigned long long)) != hipSuccess) return nullptr;
+    stream_counters.emplace_back(st, static_cast<unsigned long long *>(p));
+    return stream_counters.back().second;
   }
   SortWorkspace *sort_workspace(hipStream_t st) {
     std::lock_guard<std::mutex> lk(sort_mu);
@@ -801,8 +834,8 @@ struct DeviceScene {
     sorts.emplace_back(st, std::make_unique<SortWorkspace>());
     return sorts.back().second.get();
   }
-  template <class T>
-  hipError_t put(const std::vector<T> &v, const T **dst) {
+  template <class T, class A>
+  hipError_t put(const std::vector<T, A> &v, const T **dst) {
     *dst = nullptr;
     if (v.empty()) return hipSuccess;
     void *p = nullptr;
@@ -829,6 +862,7 @@ int check_launch(const brt_launch &L) {
     return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
   if (L.stack_ring && L.stack_ring != 4 && L.stack_ring != 8 && L.stack_ring != 16)
     return fail(BRT_ERR_INVALID_ARGUMENT, "stack_ring must be 0, 4, 8 or 16");
+  if (L.flags & ~(BRT_FLAG_HITS_INOUT | BRT_FLAG_ANY_HIT)) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown launch flags");
   return BRT_OK;
 }
 
@@ -886,11 +920,6 @@ int double_only(const brt_scene *s, const char *what) {
 hipError_t new_device_scene(int dev, std::unique_ptr<DeviceScene> &out) {
   auto d = std::make_unique<DeviceScene>();
   d->device = dev;
-  void *p = nullptr;
-  hipError_t e = hipMalloc(&p, NUM_COUNTERS * sizeof(unsigned long long));
-  if (e != hipSuccess) return e;
-  d->allocs.push_back(p);
-  d->counters = static_cast<unsigned long long *>(p);
   out = std::move(d);
   return hipSuccess;
 }
@@ -1218,8 +1247,12 @@ int intersect1_t(const brt_scene *s, const R org[3], const R dir[3], R tmin, R t
     for (uint32_t t = 0; t < NUM_TYPES && e == hipSuccess; t++) {
       const GeometryT<R> &g = geo_of<R>(ms)[t];
       if (!g.present || g.boxes.n == 0) continue;
-      TraceLaunch cfg{BRT_PRECISION_FP64, flags | BRT_FLAG_HITS_INOUT, 64, false, SCHEDULE_PER_THREAD, 0, nullptr,
-                      nullptr};
+      TraceLaunch cfg{BRT_PRECISION_FP64, flags | BRT_FLAG_HITS_INOUT, 64, false, SCHEDULE_PER_THREAD, 0,
+                      ms->dev->counter_group(nullptr), nullptr};
+      if (!cfg.counter) {
+        e = hipErrorOutOfMemory;
+        break;
+      }
       if constexpr (std::is_same<R, float>::value)
         e = launch_trace_f32((GeomType)t, ms->dev->views[t], rv, hv, 1, cfg, nullptr);
       else
@@ -1492,6 +1525,7 @@ int brt_scene_export_tree(const brt_scene *s, uint32_t type, brt_tree_node *node
 int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hits, uint64_t n,
                   const brt_launch *launch, void *stream) {
   brt_scene *s = const_cast<brt_scene *>(cs);
+  hipStream_t st = static_cast<hipStream_t>(stream);
   if (!s || !rays || !hits) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
   if (int rc = double_only(s, "brt_intersect (use brt_intersect_f32)")) return rc;
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
@@ -1512,7 +1546,6 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
   if (rc) return rc;
   RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
   HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz, nullptr};
-  hipStream_t st = static_cast<hipStream_t>(stream);
   const uint32_t *order = nullptr;
   if (L.sort != BRT_SORT_NONE) {
     if (n > 0x7fffffffull) return fail(BRT_ERR_UNSUPPORTED, "sorted launches are limited to 2^31-1 rays");
@@ -1539,8 +1572,8 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
     TraceLaunch cfg{trace_precision(L.precision, s->dev->views[t]), L.flags, L.block_size, first, L.schedule,
                     L.refill, nullptr, order};
     cfg.ring = L.stack_ring;
-    if (L.schedule == BRT_SCHEDULE_PERSISTENT)
-      cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
+    cfg.counter = s->dev->counter_group(st);
+    if (!cfg.counter) return fail(BRT_ERR_OUT_OF_MEMORY, "launch counters");
     SplitParams sp{g.extent};
     hipError_t e = launch_trace((GeomType)t, s->dev->views[t], rv, hv, n, cfg, sp, st);
     if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
@@ -1584,8 +1617,8 @@ int brt_traverse(const brt_scene *cs, uint32_t geom_type, const brt_rays *rays, 
                   L.block_size, true, L.schedule, L.refill, nullptr, nullptr};
   cfg.ring = L.stack_ring;
   cfg.overwrite = true;
-  if (L.schedule == BRT_SCHEDULE_PERSISTENT)
-    cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
+  cfg.counter = s->dev->counter_group(static_cast<hipStream_t>(stream));
+  if (!cfg.counter) return fail(BRT_ERR_OUT_OF_MEMORY, "launch counters");
   SplitParams sp{g.extent};
   hipError_t e = launch_trace((GeomType)geom_type, s->dev->views[geom_type], rv, hv, n, cfg, sp,
                               static_cast<hipStream_t>(stream));
@@ -1635,8 +1668,8 @@ int brt_intersect_f32(const brt_scene *cs, const brt_rays_f32 *rays, const brt_h
     if (!g.present || g.boxes.n == 0) continue;
     TraceLaunch cfg{L.precision, L.flags, L.block_size, first, L.schedule, L.refill, nullptr, nullptr};
     cfg.ring = L.stack_ring;
-    if (L.schedule == BRT_SCHEDULE_PERSISTENT)
-      cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
+    cfg.counter = s->dev->counter_group(st);
+    if (!cfg.counter) return fail(BRT_ERR_OUT_OF_MEMORY, "launch counters");
     hipError_t e = launch_trace_f32((GeomType)t, s->dev->views[t], rv, hv, n, cfg, st);
     if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
     first = false;
@@ -1820,7 +1853,7 @@ int batch_enqueue(brt_batch *b, hipStream_t st) {
   for (uint32_t t = 0; t < NUM_TYPES; t++) {
     if (!b->njobs[t]) continue;
     TraceLaunch cfg{b->L.precision, b->L.flags, b->L.block_size, false, SCHEDULE_PERSISTENT, b->L.refill,
-                    b->counters + t, nullptr};
+                    b->counters + (size_t)t * COUNTER_GROUP, nullptr};
     cfg.ring = b->L.stack_ring;
     hipError_t e = launch_trace_batch((GeomType)t, static_cast<const BatchJob *>(b->d_jobs[t]), b->njobs[t],
                                       b->total[t], cfg, st);
@@ -1901,7 +1934,8 @@ int brt_batch_create(const brt_batch_item *items, uint32_t count, const brt_laun
       e = hipMemcpy(b->d_jobs[t], jobs[t].data(), jobs[t].size() * sizeof(BatchJob), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("batch table: ") + hipGetErrorString(e));
   }
-  if (hipMalloc(reinterpret_cast<void **>(&b->counters), NUM_TYPES * sizeof(unsigned long long)) != hipSuccess)
+  if (hipMalloc(reinterpret_cast<void **>(&b->counters), NUM_TYPES * COUNTER_GROUP * sizeof(unsigned long long)) !=
+      hipSuccess)
     return fail(BRT_ERR_OUT_OF_MEMORY, "batch counters");
   *out = b.release();
   return BRT_OK;
@@ -1986,8 +2020,8 @@ int brt_mirror_trace(const brt_scene *cs, const brt_rays *rays, uint64_t n, uint
                 nullptr, nullptr, nullptr, nullptr, nullptr};
     TraceLaunch cfg{L.precision, 0u, L.block_size, true, L.schedule, L.refill, nullptr, nullptr};
     cfg.ring = L.stack_ring;
-    if (L.schedule == BRT_SCHEDULE_PERSISTENT)
-      cfg.counter = s->dev->counters + (s->dev->next_counter.fetch_add(1) % NUM_COUNTERS);
+    cfg.counter = s->dev->counter_group(st);
+    if (!cfg.counter) return fail(BRT_ERR_OUT_OF_MEMORY, "launch counters");
     SplitParams sp{s->geo[TRIANGLES].extent};
     hipError_t e = launch_trace(TRIANGLES, bvh, rv, hv, n, cfg, sp, st);
     double *pos[3] = {out->px + off, out->py + off, out->pz + off};

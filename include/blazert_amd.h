@@ -211,8 +211,10 @@ int brt_closest_surface(const brt_scene *scene, const brt_points *points, uint64
 
 /* Batched intersect1 (blazert/scene.h:161-211) for n rays resident on the device, enqueued on `stream`
  * (a hipStream_t, NULL = default stream). launch may be NULL (FP64, no flags, persistent). Asynchronous.
- * The persistent schedule uses one of 256 per-scene device counters per launch, taken round robin, so more than
- * 256 launches of one scene in flight at once on different streams would share a counter. */
+ * The launches keep a few device counters per scene and stream, allocated on the first launch on that stream
+ * (do not graph-capture that first launch); launches on one stream run in order, so any number may be in flight.
+ * With BRT_PRECISION_FP32_TRAV_FP64_HIT, rays whose reference slab test can produce NaN terms (a zero, non-finite
+ * or reciprocal-overflowing direction component, a non-finite origin, a NaN tmin) are traced by the FP64 path. */
 int brt_intersect(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,
                   const brt_launch *launch, void *stream);
 

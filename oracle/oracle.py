@@ -14,7 +14,7 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_PATH = os.environ.get("BLAZERT_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 TRIANGLES, SPHERES, PLANES, CYLINDERS = 0, 1, 2, 3
 INVALID = 0xFFFFFFFF
@@ -487,7 +487,7 @@ class HitF(C.Structure):
 
 NODE_DTYPE_F32 = np.dtype([("min", np.float32, 3), ("max", np.float32, 3), ("leaf", np.uint32), ("axis", np.uint32),
                            ("children", np.uint32, 2), ("prim_begin", np.uint32), ("prim_count", np.uint32)])
-LIB_F32_PATH = os.path.join(HERE, "liboracle_f32.so")
+LIB_F32_PATH = os.environ.get("BLAZERT_ORACLE_F32_LIB") or os.path.join(HERE, "liboracle_f32.so")
 _lib_f32 = None
 
 

@@ -343,6 +343,26 @@ def test_degenerate_and_axis_parallel_rays(dev, oracle_mod, precision):
     assert cls["unexplained"] == 0, (cls, cls["unexplained_idx"][:10])
     assert_bit_exact(g, ref, same)
     assert same.mean() > 0.99
+    # rays with a zero direction component are deferred to the FP64 path: bit-exact to the reference
+    zero_dir = (dirs == 0.0).any(axis=1)
+    assert zero_dir.sum() > 1000
+    assert_bit_exact(g, ref, zero_dir)
+    # the same rays as a one-item split batch, captured and replayed (the deferred FP64 launch is part of the graph)
+    from blazert_amd import Batch
+    col = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    b = Batch([(p, col(org), col(dirs), col(tmin), col(tmax))], precision=1)
+    s = torch.cuda.Stream()
+    b.capture(s)
+    for _ in range(2):
+        b.hits[0]["t"].fill_(-1.0)
+        b.hits[0]["prim_id"].fill_(-7)
+        torch.cuda.synchronize()
+        b.replay(s)
+        s.synchronize()
+        h = b.hits[0]
+        assert np.array_equal(h["t"].cpu().numpy().view(np.uint64), g["t"].view(np.uint64))
+        assert np.array_equal(h["prim_id"].cpu().numpy().view(np.uint32), g["prim_id"])
+        assert np.array_equal(h["geom_id"].cpu().numpy().view(np.uint32), g["geom_id"])
 
 
 def test_hits_inout_semantics(dev, oracle_mod):
