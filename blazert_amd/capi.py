@@ -141,6 +141,8 @@ SIGNATURES = {
     "brt_scene_build_stats": (C.c_int, [_P, C.c_uint32, C.POINTER(BuildStats)]),
     "brt_scene_export_tree": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32, _U, _U]),
     "brt_intersect": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_uint64, C.POINTER(Launch), C.c_void_p]),
+    "brt_intersect_noalloc": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_uint64, C.POINTER(Launch),
+                                        C.c_void_p]),
     "brt_traverse": (C.c_int, [_P, C.c_uint32, C.POINTER(Rays), C.POINTER(Hits), C.c_uint64, C.POINTER(Launch),
                                C.c_void_p]),
     "brt_intersect1": (C.c_int, [_P, _D, _D, C.c_double, C.c_double, C.c_uint32, C.POINTER(Hit1),

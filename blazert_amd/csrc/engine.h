@@ -92,7 +92,17 @@ struct SortWorkspace {
 
 // Grows ws to hold n rays (frees the previous buffers; not capturable in a graph).
 hipError_t sort_workspace_reserve(SortWorkspace &ws, uint64_t n);
+// Lays out a workspace for n rays in caller memory [base, base + bytes) without allocating; hipErrorInvalidValue
+// when it does not fit.
+hipError_t sort_workspace_in(SortWorkspace &ws, void *base, size_t bytes, uint64_t n);
 
+// Device scratch of the allocation-free intersect (brt_intersect_noalloc): a file-scope __device__ array, one copy
+// per GPU, holding a launch counter group in its first 256 bytes and then the workspace of a ray sort of up to
+// NOALLOC_MAX_SORTED_RAYS rays.
+constexpr uint64_t NOALLOC_MAX_SORTED_RAYS = 1ull << 20;
+constexpr size_t NOALLOC_SCRATCH_BYTES = 24ull << 20;
+// This device's address of the scratch array (looked up once per device with hipGetSymbolAddress).
+hipError_t noalloc_scratch(unsigned char **ptr);
 // Computes the sort keys (mode 1: direction octant + origin Morton code, mode 2: Morton only) and radix-sorts the
 // ray indices on `stream`; *order points into ws afterwards.
 hipError_t sort_rays(const RaysView &rays, uint64_t n, const double lo[3], const double hi[3], uint32_t mode,

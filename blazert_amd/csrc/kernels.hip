@@ -15,6 +15,7 @@
 #include <float.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include <hipcub/hipcub.hpp>
@@ -1262,6 +1263,26 @@ hipError_t sort_workspace_reserve(SortWorkspace &ws, uint64_t n) {
   return hipSuccess;
 }
 
+hipError_t sort_workspace_in(SortWorkspace &ws, void *base, size_t bytes, uint64_t n) {
+  if (n > 0x7fffffffull) return hipErrorInvalidValue;
+  size_t temp = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                    (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, 30);
+  if (e != hipSuccess) return e;
+  const size_t arr = ((size_t)n * 4 + 255) & ~(size_t)255;
+  if (4 * arr + temp > bytes) return hipErrorInvalidValue;
+  unsigned char *b = static_cast<unsigned char *>(base);
+  ws.base = nullptr;  // not owned
+  ws.keys_in = reinterpret_cast<uint32_t *>(b);
+  ws.keys_out = reinterpret_cast<uint32_t *>(b + arr);
+  ws.vals_in = reinterpret_cast<uint32_t *>(b + 2 * arr);
+  ws.vals_out = reinterpret_cast<uint32_t *>(b + 3 * arr);
+  ws.temp = b + 4 * arr;
+  ws.temp_bytes = temp;
+  ws.capacity = n;
+  return hipSuccess;
+}
+
 hipError_t sort_rays(const RaysView &rays, uint64_t n, const double lo[3], const double hi[3], uint32_t mode,
                      SortWorkspace &ws, hipStream_t stream, const uint32_t **order) {
   *order = nullptr;
@@ -1316,6 +1337,30 @@ hipError_t init_hits(const HitsViewT<R> &hits, uint64_t n, hipStream_t stream) {
 hipError_t launch_init_hits(const HitsView &hits, uint64_t n, hipStream_t stream) { return init_hits(hits, n, stream); }
 hipError_t launch_init_hits_f32(const HitsViewF &hits, uint64_t n, hipStream_t stream) {
   return init_hits(hits, n, stream);
+}
+
+// Scratch memory of brt_intersect_noalloc, for callers that cannot allocate (graph capture, serving loops): each
+// GPU that loads this library holds one copy until the process ends. Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol); it uses the device address from noalloc_scratch.
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch(unsigned char **ptr) {
+  constexpr int MAX_DEVICES = 64;
+  static std::mutex mu;
+  static unsigned char *addr[MAX_DEVICES] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = static_cast<unsigned char *>(p);
+  }
+  *ptr = addr[dev];
+  return hipSuccess;
 }
 
 // Scene<float>: the exact path in FP32 (no split precision: the scene type is already FP32).

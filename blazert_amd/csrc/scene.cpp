@@ -1522,10 +1522,11 @@ int brt_scene_export_tree(const brt_scene *s, uint32_t type, brt_tree_node *node
   return BRT_OK;
 }
 
-int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hits, uint64_t n,
-                  const brt_launch *launch, void *stream) {
-  brt_scene *s = const_cast<brt_scene *>(cs);
-  hipStream_t st = static_cast<hipStream_t>(stream);
+// brt_intersect, and with `noalloc` brt_intersect_noalloc: the launch counters and the ray-sort workspace then come
+// from the library's static device scratch (noalloc_scratch) instead of per-stream and per-scene allocations, and a
+// scene not yet on the device is an error rather than uploaded.
+static int intersect_impl(brt_scene *s, const brt_rays *rays, const brt_hits *hits, uint64_t n,
+                          const brt_launch *launch, hipStream_t st, bool noalloc) {
   if (!s || !rays || !hits) return fail(BRT_ERR_INVALID_ARGUMENT, "NULL argument");
   if (int rc = double_only(s, "brt_intersect (use brt_intersect_f32)")) return rc;
   if (!s->committed) return fail(BRT_ERR_STATE, "scene not committed");
@@ -1542,8 +1543,22 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
   if (launch) L = *launch;
   int rc = check_launch(L);
   if (rc) return rc;
-  rc = upload(s);
-  if (rc) return rc;
+  unsigned char *scratch = nullptr;
+  if (noalloc) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(BRT_ERR_NO_DEVICE, "no HIP device");
+    if (!s->dev || s->dev->device != dev)
+      return fail(BRT_ERR_STATE, "brt_intersect_noalloc needs the scene on the current device already "
+                                 "(brt_scene_upload, brt_scene_commit_device or an earlier brt_intersect allocate)");
+    if (L.sort != BRT_SORT_NONE && n > NOALLOC_MAX_SORTED_RAYS)
+      return fail(BRT_ERR_UNSUPPORTED, "brt_intersect_noalloc sorts at most " +
+                                           std::to_string(NOALLOC_MAX_SORTED_RAYS) +
+                                           " rays (its sort workspace is fixed device scratch); got " + std::to_string(n));
+    if (hipError_t e = noalloc_scratch(&scratch)) return device_error(e, "scratch lookup");
+  } else {
+    rc = upload(s);
+    if (rc) return rc;
+  }
   RaysView rv{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
   HitsView hv{hits->t, hits->u, hits->v, hits->prim_id, hits->geom_id, hits->nx, hits->ny, hits->nz, nullptr};
   const uint32_t *order = nullptr;
@@ -1558,8 +1573,16 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
         hi[k] = std::max(hi[k], g.view.root_max[k]);
       }
     }
-    SortWorkspace *ws = s->dev->sort_workspace(st);
-    hipError_t e = sort_workspace_reserve(*ws, n);
+    SortWorkspace fixed;
+    SortWorkspace *ws = &fixed;
+    hipError_t e = hipSuccess;
+    if (noalloc) {
+      e = sort_workspace_in(fixed, scratch + 256, NOALLOC_SCRATCH_BYTES - 256, n);
+      if (e == hipErrorInvalidValue) return fail(BRT_ERR_UNSUPPORTED, "sort workspace does not fit the scratch");
+    } else {
+      ws = s->dev->sort_workspace(st);
+      e = sort_workspace_reserve(*ws, n);
+    }
     if (e == hipSuccess) e = sort_rays(rv, n, lo, hi, L.sort, *ws, st, &order);
     if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("ray sort failed: ") + hipGetErrorString(e));
   }
@@ -1572,7 +1595,7 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
     TraceLaunch cfg{trace_precision(L.precision, s->dev->views[t]), L.flags, L.block_size, first, L.schedule,
                     L.refill, nullptr, order};
     cfg.ring = L.stack_ring;
-    cfg.counter = s->dev->counter_group(st);
+    cfg.counter = noalloc ? reinterpret_cast<unsigned long long *>(scratch) : s->dev->counter_group(st);
     if (!cfg.counter) return fail(BRT_ERR_OUT_OF_MEMORY, "launch counters");
     SplitParams sp{g.extent};
     hipError_t e = launch_trace((GeomType)t, s->dev->views[t], rv, hv, n, cfg, sp, st);
@@ -1584,6 +1607,16 @@ int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hit
     if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(e));
   }
   return BRT_OK;
+}
+
+int brt_intersect(const brt_scene *cs, const brt_rays *rays, const brt_hits *hits, uint64_t n,
+                  const brt_launch *launch, void *stream) {
+  return intersect_impl(const_cast<brt_scene *>(cs), rays, hits, n, launch, static_cast<hipStream_t>(stream), false);
+}
+
+int brt_intersect_noalloc(const brt_scene *cs, const brt_rays *rays, const brt_hits *hits, uint64_t n,
+                          const brt_launch *launch, void *stream) {
+  return intersect_impl(const_cast<brt_scene *>(cs), rays, hits, n, launch, static_cast<hipStream_t>(stream), true);
 }
 
 int brt_traverse(const brt_scene *cs, uint32_t geom_type, const brt_rays *rays, const brt_hits *hits, uint64_t n,

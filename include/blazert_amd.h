@@ -218,6 +218,18 @@ int brt_closest_surface(const brt_scene *scene, const brt_points *points, uint64
 int brt_intersect(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,
                   const brt_launch *launch, void *stream);
 
+/* brt_intersect that allocates nothing. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc). Same arguments and
+ * hits as brt_intersect, bit for bit. Its launch counters and its ray-sort workspace live in one static device
+ * array of the library (24 MiB per GPU that loads the library, held until the process ends) instead of per-stream
+ * and per-scene allocations. Limits, each refused with an error before any device work: the scene must already be
+ * on the current device (BRT_ERR_STATE; brt_scene_upload, brt_scene_commit_device or a brt_intersect put it
+ * there), and a sorted launch takes at most 2^20 rays (BRT_ERR_UNSUPPORTED); unsorted launches take any n. All
+ * calls on one GPU share that array: calls that overlap in time on one GPU (on different streams, or from
+ * different threads) race and must be serialised by the caller. */
+int brt_intersect_noalloc(const brt_scene *scene, const brt_rays *rays, const brt_hits *hits, uint64_t n,
+                          const brt_launch *launch, void *stream);
+
 /* Low-level traverse of one geometry type's BVH (blazert/bvh/accel.h:60-120): like the reference, a ray that hits
  * gets its record overwritten by post_traversal (t, u, v, prim_id, normal; geom_id is set to that geometry's id,
  * which the reference leaves to intersect1), and a ray that misses keeps the record the caller passed in. launch may be
