@@ -89,6 +89,7 @@ struct Ray32 {
   float offn[3];  // rounded-down (-o*inv - E)
   float offf[3];  // rounded-up   (-o*inv + E)
   float tmin;     // rounded down
+  float om;       // max |o_i| rounded up; +inf when above FILTER_MAX_EXTENT (no FP32 triangle pre-test)
   uint32_t sign;
 };
 
@@ -97,6 +98,8 @@ struct Ray32 {
 __device__ __forceinline__ void make_ray32(const Ray &r, double S, Ray32 &q) {
   q.sign = r.sign;
   q.tmin = f32_down(r.tmin);
+  const double om = fmax(fabs(r.o[0]), fmax(fabs(r.o[1]), fabs(r.o[2])));
+  q.om = om <= FILTER_MAX_EXTENT ? f32_up(om) : __builtin_inff();
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const double inv = r.inv[i];
@@ -128,6 +131,52 @@ __device__ __forceinline__ bool slab32(const float *b, const Ray32 &q, float &ke
   }
   key = A;
   return A <= M;
+}
+
+// Conservative FP32 pre-test of one triangle for the split path: returns false only when the FP64 test isect_tri
+// (blazert/primitives/trimesh.h:153-181) rejects the triangle for the ray at the current hit distance h <= h32.
+//
+// With u = 2^-24, Pm = vmax + om >= |v0_i| + |o_i|, sa = sum |e1_i|, sb = sum |e2_i|, the values computed here from
+// v0, e1, e2, n and o rounded to nearest and the unit direction d rounded to nearest differ from the exact ones (and
+// from the FP64 test's, whose own rounding is ~2^-29 smaller) by at most
+//   u, v:  13.1 u Pm sb and 13.1 u Pm sa     (c = v0 - o: 2.01 u Pm per component; r = d x c: 5.1 u sqrt2 Pm)
+//   det:    5.1 u sa sb                      (n rounded once from the FP64 test's own n)
+//   t:      6.1 u sa sb Pm
+//   w = |det| - u - v: the sum of those plus 2 u (|det| + |u| + |v|) <= 0.2 Ed + 0.18 (Eu + Ev)
+// The bounds used (Eu = kub Pm, Ev = kua Pm, Ed = ed, Et = 1.2 ed Pm, Ew = 1.3 ed + 1.2 (kua + kub) Pm) keep a
+// margin for the rounding of the bounds themselves; ed and Et hold an absolute 1e-29, Eu, Ev and Ew the reference's
+// tolerance, for results in the FP32 denormal range. The reference accepts u, v, w >= -4 DBL_EPSILON (tol), t >= |det| tmin and |det| h > t with the signs
+// taken from det; where the sign of det is not certain (|det| <= Ed) the triangle passes. Scenes and ray origins
+// above FILTER_MAX_EXTENT are not pre-tested, so no value here overflows; every rejection is a comparison that is
+// false for NaN operands.
+__device__ __forceinline__ bool tri_filter(const TriFilt &T, const float D[3], const float O[3], float om,
+                                           float tmin32, float h32) {
+  const float u = 5.9604644775390625e-08f;
+  const float tol = 9e-16f;  // 4 DBL_EPSILON rounded up
+  const float det = T.n[0] * D[0] + T.n[1] * D[1] + T.n[2] * D[2];
+  const float ad = fabsf(det);
+  if (!(ad > T.ed)) return true;  // the sign of det is not certain
+  const float c0 = T.v0[0] - O[0], c1 = T.v0[1] - O[1], c2 = T.v0[2] - O[2];
+  const float rx = D[1] * c2 - D[2] * c1;
+  const float ry = D[2] * c0 - D[0] * c2;
+  const float rz = D[0] * c1 - D[1] * c0;
+  const float Pm = T.vmax + om;
+  const float *a = T.e1, *b = T.e2;
+  const float us = product_sign(rx * b[0] + ry * b[1] + rz * b[2], det);
+  const float vs = product_sign(rx * a[0] + ry * a[1] + rz * a[2], det);
+  if (us < -__builtin_fmaf(T.kub, Pm, tol) || vs < -__builtin_fmaf(T.kua, Pm, tol)) return false;
+  const float Ew = __builtin_fmaf(1.2f * (T.kua + T.kub), Pm, __builtin_fmaf(1.3f, T.ed, tol));
+  if ((ad - us) - vs < -Ew) return false;
+  const float ts = product_sign(T.n[0] * c0 + T.n[1] * c1 + T.n[2] * c2, det);
+  const float Et = __builtin_fmaf(1.2f * T.ed, Pm, 1e-29f);
+  // t >= |det| tmin: a lower bound of the right side (tmin32 <= tmin; no bound for negative tmin)
+  if (tmin32 >= 0.0f) {
+    const float L = tmin32 > 0.0f ? ((ad - T.ed) * tmin32) * (1.0f - 4.0f * u) : 0.0f;
+    if (ts + Et < L) return false;
+  }
+  // |det| h > t: an upper bound of the left side (h32 >= h)
+  if (ts - Et >= ((ad + T.ed) * h32) * (1.0f + 4.0f * u)) return false;
+  return true;
 }
 
 // ------------------------------------------------------------------------------------------------ intersectors
@@ -509,7 +558,8 @@ __device__ void post_normal(const BvhView &bvh, const RayT<typename Rec::real> &
 // whole leaf), so a persistent wave can retire finished rays and pull new ones between steps.
 struct Trav {
   uint32_t cur, cnt;  // current node reference and leaf count
-  uint32_t pos;       // next primitive of the current leaf
+  uint32_t pos;       // next primitive of the current leaf (with the triangle pre-test: first of the current group)
+  uint32_t cand;      // split-precision triangles: group members that passed the FP32 pre-test, still to test
   bool active;        // traversal still running
   float h32;          // split precision: hit distance rounded up to FP32
 };
@@ -642,6 +692,7 @@ __device__ __forceinline__ void trav_begin(const BvhView &bvh, const RayT<R> &r,
   t.cur = SPLIT ? bvh.root_ref4 : bvh.root_ref;
   t.cnt = bvh.root_cnt;
   t.pos = 0;
+  t.cand = 0;
   if constexpr (SPLIT) {
     float k32;
     const float rb[6] = {bvh.root_min32[0], bvh.root_min32[1], bvh.root_min32[2],
@@ -660,6 +711,8 @@ __device__ __forceinline__ void trav_begin(const BvhView &bvh, const RayT<R> &r,
 
 // One unit of the DFS: an internal node (test both children, push far, descend near) or a leaf (test its
 // primitives in order, blazert/bvh/intersect.h:66-78), followed by pops until a node passes or the stack is empty.
+// On the split path with triangle pre-test records, a leaf step is either a pre-test of the next FILTER_GROUP
+// triangles in FP32 (tri_filter) or the FP64 test of the next one that passed it, in leaf order.
 template <class Rec, bool SPLIT>
 __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typename Rec::real> &r, const Ray32 &q,
                                           bool any_hit, HitT<typename Rec::real> &s, Trav &t,
@@ -667,19 +720,24 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
   using R = typename Rec::real;
   using Node = typename std::conditional<SPLIT, Node4,
                                          typename std::conditional<sizeof(R) == 8, Node64, Node32>::type>::type;
-  // One fetch per lane, issued together: the internal record for lanes at an inner node, the primitive record for
-  // lanes at a leaf. Written as separate branches, each would wait for its own loads, and a wave holding both kinds
-  // of lanes would pay two memory round trips per step. The chunks both records have are loaded from a per-lane
-  // address in one set of instructions; the rest of the longer one follows in a short branch (from the same lines,
-  // already on their way), and the compute branches below run without waiting in between.
+  // One fetch per lane, issued together: the internal record for lanes at an inner node, the primitive record (or
+  // the group of pre-test records) for lanes at a leaf. Written as separate branches, each would wait for its own
+  // loads, and a wave holding several kinds of lanes would pay several memory round trips per step. The chunks the
+  // records have in common are loaded from a per-lane address in one set of instructions; the rest of the longer
+  // ones follow in short branches (from the same lines, already on their way), and the compute branches below run
+  // without waiting in between.
+  constexpr bool FILTER = SPLIT && Rec::kind == TRIANGLES;
   constexpr int NODE_Q = SPLIT ? 8 : (sizeof(R) == 8 ? 7 : 4);  // 16-byte chunks of the record used (Node64: 112 B)
   constexpr int REC_Q = (int)(sizeof(Rec) / 16);
-  constexpr int COMMON = NODE_Q < REC_Q ? NODE_Q : REC_Q;
-  constexpr int QMAX = NODE_Q < REC_Q ? REC_Q : NODE_Q;
+  constexpr int FILT_Q = FILTER ? (int)(FILTER_GROUP * sizeof(TriFilt) / 16) : 0;
+  constexpr int QMAX0 = NODE_Q < REC_Q ? REC_Q : NODE_Q;
+  constexpr int QMAX = QMAX0 < FILT_Q ? FILT_Q : QMAX0;
   static_assert(sizeof(Rec) % 16 == 0 && sizeof(Node) >= 16 * NODE_Q, "record chunking");
   const bool at_leaf = (t.cur & LEAF_BIT) != 0;
-  const bool fetch_rec = at_leaf && t.pos < t.cnt;
-  const uint32_t p = (t.cur & ~LEAF_BIT) + t.pos;
+  const bool filt = FILTER && bvh.tri_filter != nullptr;  // the same for every lane of a launch
+  const bool filter_step = filt && at_leaf && t.cand == 0u && t.pos < t.cnt;
+  const bool fetch_rec = at_leaf && t.pos < t.cnt && !filter_step;
+  const uint32_t p = (t.cur & ~LEAF_BIT) + t.pos + (filt ? (uint32_t)__builtin_ctz(t.cand | 0x80000000u) : 0u);
   const Node *nodes;
   if constexpr (SPLIT) nodes = reinterpret_cast<const Node *>(bvh.nodes4);
   else if constexpr (sizeof(R) == 8) nodes = reinterpret_cast<const Node *>(bvh.nodes64);
@@ -688,19 +746,29 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
   // table in memory, and as generic pointers every access through them would be a flat access.
   using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
   using GlobalChunk = const __attribute__((address_space(1))) u32x4;
-  GlobalChunk *src = at_leaf ? (GlobalChunk *)(reinterpret_cast<const Rec *>(bvh.prims) + p)
-                             : (GlobalChunk *)(nodes + t.cur);
+  GlobalChunk *src = !at_leaf ? (GlobalChunk *)(nodes + t.cur)
+                     : filter_step ? (GlobalChunk *)(bvh.tri_filter + ((t.cur & ~LEAF_BIT) + t.pos))
+                                   : (GlobalChunk *)(reinterpret_cast<const Rec *>(bvh.prims) + p);
   u32x4 chunk[QMAX];
-  if (!at_leaf || fetch_rec) {
+  if constexpr (FILTER) {
+    // three record kinds: each chunk is loaded by the lanes whose record has it
+    const int nq = !at_leaf ? NODE_Q : (filter_step ? FILT_Q : (fetch_rec ? REC_Q : 0));
 #pragma unroll
-    for (int k = 0; k < COMMON; k++) chunk[k] = src[k];
-  }
-  if (!at_leaf) {
+    for (int k = 0; k < QMAX; k++)
+      if (k < nq) chunk[k] = src[k];
+  } else {
+    constexpr int COMMON = NODE_Q < REC_Q ? NODE_Q : REC_Q;
+    if (!at_leaf || fetch_rec) {
 #pragma unroll
-    for (int k = COMMON; k < NODE_Q; k++) chunk[k] = src[k];
-  } else if (fetch_rec) {
+      for (int k = 0; k < COMMON; k++) chunk[k] = src[k];
+    }
+    if (!at_leaf) {
 #pragma unroll
-    for (int k = COMMON; k < REC_Q; k++) chunk[k] = src[k];
+      for (int k = COMMON; k < NODE_Q; k++) chunk[k] = src[k];
+    } else if (fetch_rec) {
+#pragma unroll
+      for (int k = COMMON; k < REC_Q; k++) chunk[k] = src[k];
+    }
   }
   // an empty asm taking every chunk as an operand: the loads cannot be sunk into the compute branches below, so all
   // of them are issued first and waited for once
@@ -767,6 +835,23 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
       t.pos = 0;
       return;
     }
+  } else if (filter_step) {
+    if constexpr (FILTER) {
+      // pre-test the group; members past the leaf's end are padding or the next leaf's triangles
+      TriFilt g[FILTER_GROUP];
+      __builtin_memcpy(g, chunk, sizeof g);
+      const float D[3] = {(float)r.d[0], (float)r.d[1], (float)r.d[2]};
+      const float O[3] = {(float)r.o[0], (float)r.o[1], (float)r.o[2]};
+      const uint32_t left = t.cnt - t.pos;
+      uint32_t mask = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < FILTER_GROUP; j++)
+        if (j < left && tri_filter(g[j], D, O, q.om, q.tmin, t.h32)) mask |= 1u << j;
+      t.cand = mask;
+      if (mask) return;
+      t.pos += FILTER_GROUP;
+      if (t.pos < t.cnt) return;
+    }
   } else {
     // one primitive per step: a leaf's tests (blazert/bvh/intersect.h:66-78, in order, each against the hit
     // distance the previous one left) are spread over consecutive steps, so lanes at leaves and lanes at inner
@@ -781,7 +866,14 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
           return;
         }
       }
-      if (++t.pos < t.cnt) return;
+      if (filt) {
+        t.cand &= t.cand - 1u;
+        if (t.cand) return;
+        t.pos += FILTER_GROUP;
+        if (t.pos < t.cnt) return;
+      } else if (++t.pos < t.cnt) {
+        return;
+      }
     }
   }
   // pop the next far child whose entry distance does not exceed the current hit distance
@@ -1076,6 +1168,37 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
 }
 
 // Clears a launch's counter group (a fill kernel, not hipMemsetAsync, so that graph captures replay correctly).
+// Split-precision pre-test records of the triangles in leaf order (TriFilt), from their FP64 records; the FILTER_PAD
+// records after the last one are zero.
+__global__ void k_tri_filter_records(const TriRec *__restrict__ recs, uint32_t n, TriFilt *out, uint32_t total) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    TriFilt f{};
+    if (i < n) {
+      const TriRec &T = recs[i];
+      double nn[3];
+      tri_normal(T.e1, T.e2, nn);  // the FP64 test's n, operation for operation
+      double vm = 0.0, sa = 0.0, sb = 0.0;
+      for (int k = 0; k < 3; k++) {
+        f.v0[k] = (float)T.v0[k];
+        f.e1[k] = (float)T.e1[k];
+        f.e2[k] = (float)T.e2[k];
+        f.n[k] = (float)nn[k];
+        vm = fmax(vm, fabs(T.v0[k]));
+        sa += fabs(T.e1[k]);
+        sb += fabs(T.e2[k]);
+      }
+      // upper bounds (the FP64 sums and products of non-negative terms are within a few ulps of exact)
+      const double up = 1.0 + 0x1p-48, u = 0x1p-24;
+      f.vmax = f32_up(vm * up);
+      f.kua = f32_up(16.0 * u * sa * up);
+      f.kub = f32_up(16.0 * u * sb * up);
+      f.ed = f32_up((10.0 * u * sa * sb + 1e-29) * up);
+    }
+    out[i] = f;
+  }
+}
+
 __global__ void k_zero_counters(unsigned long long *c) {
 #pragma unroll
   for (int k = 0; k < COUNTER_GROUP; k++) c[k] = 0ull;
@@ -1232,6 +1355,15 @@ hipError_t launch_trace(GeomType type, const BvhView &bvh, const RaysView &rays,
   case PLANES: return launch_typed<PlaRec>(bvh, rays, hits, n, cfg, split, cfg.overwrite ? 1u : 0u, stream);
   default: return launch_typed<CylRec>(bvh, rays, hits, n, cfg, split, cfg.overwrite ? 1u : 0u, stream);
   }
+}
+
+hipError_t launch_tri_filter_records(const void *recs, uint32_t n, void *out, hipStream_t stream) {
+  const uint32_t total = n + FILTER_PAD;
+  uint32_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_tri_filter_records, dim3(blocks), dim3(256), 0, stream, static_cast<const TriRec *>(recs), n,
+                     static_cast<TriFilt *>(out), total);
+  return hipGetLastError();
 }
 
 hipError_t sort_workspace_reserve(SortWorkspace &ws, uint64_t n) {

@@ -52,6 +52,28 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 80, "TriRec size");
 
+// Split precision: FP32 pre-test record of a triangle, in leaf order beside the TriRec array: v0, e1, e2 and the
+// FP64 test's n = cross(e1, e2), each rounded to nearest, and the constants of the pre-test's error bound, rounded
+// up from FP64: vmax >= max |v0_i|, kua = 16 u sum |e1_i|, kub = 16 u sum |e2_i|, ed = 10 u sum |e1_i| sum |e2_i|
+// + 1e-29 (u = 2^-24). Built on the device from the TriRec array; FILTER_PAD records of padding follow the last one,
+// so a group of FILTER_GROUP records read from any leaf start stays in bounds.
+struct alignas(16) TriFilt {
+  float v0[3];
+  float e1[3];
+  float e2[3];
+  float n[3];
+  float vmax, kua, kub, ed;
+};
+static_assert(sizeof(TriFilt) == 64, "TriFilt size");
+#ifndef BRT_FILTER_GROUP
+#define BRT_FILTER_GROUP 2
+#endif
+constexpr uint32_t FILTER_GROUP = BRT_FILTER_GROUP;  // triangles pre-tested per traversal step
+constexpr uint32_t FILTER_PAD = FILTER_GROUP - 1;
+// Largest scene extent (and ray origin magnitude) for which the FP32 pre-test is used: below it no product of the
+// pre-test or of its error bound can overflow FP32.
+constexpr double FILTER_MAX_EXTENT = 1e12;
+
 struct alignas(32) SphRec {
   using real = double;
   static constexpr uint32_t kind = SPHERES;
@@ -135,6 +157,7 @@ struct BvhView {
   const double *vnormals;     // triangles only: vertex normals (3 per vertex)
   const double *vertices;     // triangles only: user vertices (3 per vertex), for mirror tracing and its adjoint
   const Node32 *nodes32;      // Scene<float>: FP32 nodes (nodes64 / nodes4 unused)
+  const TriFilt *tri_filter;  // split precision, triangles: FP32 pre-test records (NULL: every test in FP64)
   const float *vnormals32;    // Scene<float> triangles: vertex normals
   double root_min[3], root_max[3];
   float root_min32[3], root_max32[3];

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -943,6 +944,26 @@ int device_error(hipError_t e, const char *what) {
               std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Split precision: the FP32 pre-test records of a triangle geometry (TriFilt), made on the device from its uploaded
+// FP64 records and waited for; none when the scene's extent is above FILTER_MAX_EXTENT.
+// BLAZERT_AMD_SPLIT_PRETEST=0 in the environment at upload time leaves them out (every split-precision triangle test
+// then runs in FP64; the hits are the same, which the tests check).
+hipError_t make_tri_filter(DeviceScene &d, BvhView &v, double extent, hipStream_t st, uint64_t *device_bytes) {
+  v.tri_filter = nullptr;
+  if (!(extent <= FILTER_MAX_EXTENT) || v.n_prims == 0 || !v.prims) return hipSuccess;
+  const char *env = std::getenv("BLAZERT_AMD_SPLIT_PRETEST");
+  if (env && env[0] == '0') return hipSuccess;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, (size_t)(v.n_prims + FILTER_PAD) * sizeof(TriFilt));
+  if (e != hipSuccess) return e;
+  d.allocs.push_back(p);
+  e = launch_tri_filter_records(v.prims, v.n_prims, p, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  v.tri_filter = static_cast<const TriFilt *>(p);
+  *device_bytes += (uint64_t)(v.n_prims + FILTER_PAD) * sizeof(TriFilt);
+  return e;
+}
+
 // Copies one host-built geometry to the device.
 int put_geometry(DeviceScene &d, Geometry &g, uint32_t t) {
   if (g.stats.max_tree_depth >= 64) return fail(BRT_ERR_UNSUPPORTED, "tree deeper than the 64-entry device stack");
@@ -957,6 +978,7 @@ int put_geometry(DeviceScene &d, Geometry &g, uint32_t t) {
     if (e == hipSuccess) e = d.put(g.faces, &v.faces);
     if (e == hipSuccess) e = d.put(g.vnormals, &v.vnormals);
     if (e == hipSuccess) e = d.put(g.vertices, &v.vertices);
+    if (e == hipSuccess) e = make_tri_filter(d, v, g.extent, nullptr, &g.stats.device_bytes);
   }
   if (e != hipSuccess) return device_error(e, "upload failed");
   d.views[t] = v;
@@ -1110,11 +1132,14 @@ int commit_on_device(brt_scene *s, hipStream_t st) {
     v.nodes4 = r.n4;
     v.prims = r.recs;
     v.prim_ids = r.order;
-    d->views[t] = v;
-    bs.build_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     bs.device_bytes = (uint64_t)r.num_n64 * sizeof(Node64) + (uint64_t)r.num_n4 * sizeof(Node4) +
                       (uint64_t)g.boxes.n * (r.rec_size + 4) + g.faces.size() * 4 + g.vnormals.size() * 8 +
                       g.vertices.size() * 8;
+    if (t == TRIANGLES)
+      if (hipError_t fe = make_tri_filter(*d, v, g.extent, st, &bs.device_bytes))
+        return device_error(fe, "triangle pre-test records");
+    d->views[t] = v;
+    bs.build_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   s->dev = std::move(d);
   return BRT_OK;
