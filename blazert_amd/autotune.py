@@ -17,7 +17,7 @@ from . import capi
 from .capi import check, lib
 from .scene import Scene, _alloc_hits, _columns, _hits_struct, _OnStream
 
-LAUNCH_KEYS = ("block_size", "refill", "stack_ring", "sort")
+LAUNCH_KEYS = ("block_size", "refill", "stack_ring", "sort", "pool_size")
 
 
 def cache_path() -> str:
@@ -67,8 +67,8 @@ def _bucket(n: int) -> int:
 def _result(r: capi.TuneResult) -> dict:
     L = r.launch
     return {"block_size": L.block_size, "refill": L.refill, "stack_ring": L.stack_ring, "sort": L.sort,
-            "min_leaf_primitives": r.min_leaf_primitives, "candidates": r.candidates, "best_ms": r.best_ms,
-            "default_ms": r.default_ms}
+            "pool_size": L.pool_size, "min_leaf_primitives": r.min_leaf_primitives, "candidates": r.candidates,
+            "best_ms": r.best_ms, "default_ms": r.default_ms}
 
 
 def _rays(org, dirs):
@@ -89,7 +89,7 @@ def tune_launch(scene: Scene, org, dirs, *, precision: int = capi.PRECISION_FP64
     key = f"launch|{_device()}|p{precision}|f{flags}|{scene_signature(scene)}|n{_bucket(n)}"
     if use_cache:
         hit = _load().get(key)
-        if hit is not None:
+        if hit is not None and all(k in hit for k in LAUNCH_KEYS):  # entries from older versions are re-tuned
             return hit
     hits = _alloc_hits(n, co[0].device, False)
     on = _OnStream(stream, co[0].device)
@@ -104,7 +104,8 @@ def tune_launch(scene: Scene, org, dirs, *, precision: int = capi.PRECISION_FP64
 
 
 def tune_batch(batch, *, repeats: int = 3, stream=None) -> dict:
-    """Tunes a Batch in place (it keeps the fastest block size / refill / stack ring); call before capture."""
+    """Tunes a Batch in place (it keeps the fastest block size / refill / stack ring / pool size); call before
+    capture."""
     import torch
 
     st = stream if stream is not None else torch.cuda.current_stream()

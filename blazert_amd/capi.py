@@ -69,7 +69,7 @@ ARRAY_PRIM_IDS = 3
 class Launch(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("flags", C.c_uint32), ("block_size", C.c_uint32),
                 ("schedule", C.c_uint32), ("refill", C.c_uint32), ("sort", C.c_uint32), ("stack_ring", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("pool_size", C.c_uint32)]
 
 
 class BatchItem(C.Structure):

@@ -1,5 +1,5 @@
 // Autotuner: chooses the launch parameters (threads per block, persistent refill threshold, LDS stack depth, ray
-// sort key) and the BVH leaf size by timing candidates on the caller's own device rays with HIP events. Every
+// sort key, ray-pool claim size) and the BVH leaf size by timing candidates on the caller's own device rays with HIP events. Every
 // candidate computes the same hits (bit for bit on the FP64 path; on every non-tie ray with split precision), so
 // tuning changes speed only. Results are returned to the caller, who may keep them (the Python layer caches them
 // per device and workload).
@@ -21,6 +21,7 @@ constexpr uint32_t BLOCKS[] = {64, 128, 192, 256};
 constexpr uint32_t REFILLS[] = {4, 8, 16, 32};
 constexpr uint32_t RINGS[] = {4, 8, 16};
 constexpr uint32_t SORTS[] = {BRT_SORT_NONE, BRT_SORT_OCTANT_MORTON};
+constexpr uint32_t POOLS[] = {64};  // tried with the best shape found (the default is 128)
 
 // Mean time of `repeats` enqueues of `run` on `st` after one untimed warm-up; < 0 with *rc set on failure.
 double time_ms(const std::function<int()> &run, hipStream_t st, uint32_t repeats, int *rc) {
@@ -96,6 +97,18 @@ int brt_autotune_launch(const brt_scene *scene, const brt_rays *rays, const brt_
             r.launch = c;
           }
         }
+  // then the ray-pool claim size with the best shape (image-order rays balance better in 64-ray claims)
+  for (uint32_t pool : POOLS) {
+    brt_launch c = r.launch;
+    c.pool_size = pool;
+    const double ms = timed(c);
+    if (rc) return rc;
+    r.candidates++;
+    if (ms < r.best_ms) {
+      r.best_ms = ms;
+      r.launch = c;
+    }
+  }
   *out = r;
   return BRT_OK;
 }
@@ -132,6 +145,18 @@ int brt_batch_autotune(brt_batch *batch, uint32_t repeats, brt_tune_result *out,
           r.launch = c;
         }
       }
+  for (uint32_t pool : POOLS) {
+    if (rc) break;
+    brt_launch c = r.launch;
+    c.pool_size = pool;
+    const double ms = timed(c);
+    if (rc) break;
+    r.candidates++;
+    if (ms < r.best_ms) {
+      r.best_ms = ms;
+      r.launch = c;
+    }
+  }
   if (rc) {
     (void)brt::batch_set_launch_config(batch, orig);
     return rc;

@@ -61,6 +61,7 @@ struct TraceLaunch {
   unsigned long long *counter;  // COUNTER_GROUP device counters for this launch (cleared on the stream)
   const uint32_t *order;        // optional ray permutation (sorted launches); kernels trace rays[order[k]]
   uint32_t ring = 0;            // LDS stack slots per lane: 4, 8 or 16 (0 = default of the precision)
+  uint32_t pool = 0;            // persistent: indices per claim away from the end, 64 or 128 (0 = 128)
   bool overwrite = false;       // every hit replaces the record (traverse semantics), not only closer ones
 };
 

@@ -986,10 +986,11 @@ __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysViewT<typename R
 // the next indices of the wave's pool. Skewed per-ray work (many rays miss the root box, a few visit hundreds of
 // nodes) then no longer idles the wave until its slowest ray ends. The pool is a range of indices claimed from the
 // launch's counter with one atomic: every claim on that one address serialises across the GPU, and claiming per
-// refill made the counter a bottleneck at small refill thresholds. A claim takes POOL_BIG indices while the wave's
-// own previous claim ended more than two big pools per wave before n, and POOL_CHUNK (or the number of idle lanes,
-// if larger) after that. The switch reads the wave's own last claim, not the counter, so a wave that lags behind
-// the others can still make a POOL_BIG claim near the end: the last claims are mostly, not always, small. Every
+// refill made the counter a bottleneck at small refill thresholds. A claim takes `big` indices (POOL_BIG, or 64 when
+// the launch asks for it: image-order rays balance better in smaller claims) while the wave's own previous claim
+// ended more than two big pools per wave before n, and POOL_CHUNK (or the number of idle lanes, if larger) after
+// that. The switch reads the wave's own last claim, not the counter, so a wave that lags behind the others can still
+// make a big claim near the end: the last claims are mostly, not always, small. Every
 // lane leaves the loop once the counter has passed n, the pool is empty and its own ray is done, so the grid
 // always drains.
 constexpr uint32_t POOL_CHUNK = 64;
@@ -1003,13 +1004,13 @@ struct WavePool {
 
 // Called by every lane of a wave with the same pool state when the wave refills. An idle lane handed an index gets
 // it in `idx` and `got`; `exhausted` is set once nothing is left to hand out.
-__device__ __forceinline__ void pool_refill(WavePool &P, unsigned long long *counter, uint64_t n,
+__device__ __forceinline__ void pool_refill(WavePool &P, unsigned long long *counter, uint64_t n, uint32_t big,
                                             unsigned long long idle, uint32_t nidle, uint32_t lane, bool &got,
                                             uint64_t &idx, bool &exhausted) {
   got = false;
   if (P.next == P.end && !P.last) {
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64u);
-    const uint32_t chunk = (P.end + 2ull * POOL_BIG * waves < n) ? POOL_BIG : POOL_CHUNK;
+    const uint32_t chunk = (P.end + 2ull * big * waves < n) ? big : POOL_CHUNK;
     const uint32_t want = nidle > chunk ? nidle : chunk;
     unsigned long long base = 0;
     const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
@@ -1040,7 +1041,8 @@ template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
 k_trace_persistent(BvhView bvh, RaysViewT<typename Rec::real> rays, HitsViewT<typename Rec::real> hits, uint64_t n,
                    uint32_t flags, uint32_t first, uint32_t is_tri, double S, unsigned long long *counter,
-                   uint32_t refill, const uint32_t *order, uint32_t ring, unsigned long long *deferred) {
+                   uint32_t refill, const uint32_t *order, uint32_t ring, unsigned long long *deferred,
+                   uint32_t pool_big) {
   using R = typename Rec::real;
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
@@ -1066,7 +1068,7 @@ k_trace_persistent(BvhView bvh, RaysViewT<typename Rec::real> rays, HitsViewT<ty
       if (nidle >= refill || nidle == 64u) {
         bool got;
         uint64_t cand = 0;
-        pool_refill(pool, counter, n, idle, nidle, lane, got, cand, exhausted);
+        pool_refill(pool, counter, n, pool_big, idle, nidle, lane, got, cand, exhausted);
         if (got) {
           idx = order ? (uint64_t)order[cand] : cand;
           start_ray<SPLIT>(bvh, rays, idx, S, r, q, s);
@@ -1110,7 +1112,8 @@ __device__ __forceinline__ uint32_t find_job(const BatchJob *jobs, uint32_t njob
 template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
 k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total, uint32_t flags,
-              unsigned long long *counter, uint32_t refill, uint32_t ring, unsigned long long *deferred) {
+              unsigned long long *counter, uint32_t refill, uint32_t ring, unsigned long long *deferred,
+              uint32_t pool_big) {
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
   const bool deferred_only = !SPLIT && (flags & FLAG_DEFERRED_ONLY);
@@ -1137,7 +1140,7 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
       if (nidle >= refill || nidle == 64u) {
         bool got;
         uint64_t g = 0;
-        pool_refill(pool, counter, total, idle, nidle, lane, got, g, exhausted);
+        pool_refill(pool, counter, total, pool_big, idle, nidle, lane, got, g, exhausted);
         if (got) {
           j = find_job(jobs, njobs, g);
           const BatchJob &J = jobs[j];
@@ -1288,7 +1291,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real>
     const uint32_t refill = cfg.refill ? cfg.refill : refill_default<Rec, SPLIT>();
     hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
                        hits, n, cfg.flags, first, is_tri, sp.scene_extent, grp + COUNTER_POOL, refill, cfg.order,
-                       ring, grp + COUNTER_DEFERRED);
+                       ring, grp + COUNTER_DEFERRED, cfg.pool ? cfg.pool : POOL_BIG);
   } else {
     uint64_t blocks = (n + block - 1) / block;
     if (blocks > (1ull << 30)) blocks = 1ull << 30;
@@ -1306,7 +1309,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real>
   hipLaunchKernelGGL((k_trace_persistent<Rec, false>), dim3((uint32_t)blocks), dim3(256), lds64, stream, bvh, rays,
                      hits, n, cfg.flags | FLAG_DEFERRED_ONLY, first, is_tri, sp.scene_extent,
                      grp + COUNTER_DEFERRED_POOL, refill_default<Rec, false>(), cfg.order, ring64,
-                     grp + COUNTER_DEFERRED);
+                     grp + COUNTER_DEFERRED, POOL_BIG);
   return hipGetLastError();
 }
 
@@ -1325,7 +1328,7 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
   const uint32_t refill = cfg.refill ? cfg.refill : refill_default<Rec, SPLIT>();
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, stream, grp);
   hipLaunchKernelGGL((k_trace_batch<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, jobs, njobs, total,
-                     cfg.flags, grp + COUNTER_POOL, refill, ring, grp + COUNTER_DEFERRED);
+                     cfg.flags, grp + COUNTER_POOL, refill, ring, grp + COUNTER_DEFERRED, cfg.pool ? cfg.pool : POOL_BIG);
   e = hipGetLastError();
   if (e != hipSuccess || !SPLIT) return e;
   const uint32_t ring64 = ring_default<false>();
@@ -1334,7 +1337,7 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_trace_batch<Rec, false>), dim3((uint32_t)blocks), dim3(256), lds64, stream, jobs, njobs, total,
                      cfg.flags | FLAG_DEFERRED_ONLY, grp + COUNTER_DEFERRED_POOL, refill_default<Rec, false>(),
-                     ring64, grp + COUNTER_DEFERRED);
+                     ring64, grp + COUNTER_DEFERRED, POOL_BIG);
   return hipGetLastError();
 }
 

@@ -863,6 +863,8 @@ int check_launch(const brt_launch &L) {
     return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
   if (L.stack_ring && L.stack_ring != 4 && L.stack_ring != 8 && L.stack_ring != 16)
     return fail(BRT_ERR_INVALID_ARGUMENT, "stack_ring must be 0, 4, 8 or 16");
+  if (L.pool_size && L.pool_size != 64 && L.pool_size != 128)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "pool_size must be 0, 64 or 128");
   if (L.flags & ~(BRT_FLAG_HITS_INOUT | BRT_FLAG_ANY_HIT)) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown launch flags");
   return BRT_OK;
 }
@@ -1620,6 +1622,7 @@ static int intersect_impl(brt_scene *s, const brt_rays *rays, const brt_hits *hi
     TraceLaunch cfg{trace_precision(L.precision, s->dev->views[t]), L.flags, L.block_size, first, L.schedule,
                     L.refill, nullptr, order};
     cfg.ring = L.stack_ring;
+    cfg.pool = L.pool_size;
     cfg.counter = noalloc ? reinterpret_cast<unsigned long long *>(scratch) : s->dev->counter_group(st);
     if (!cfg.counter) return fail(BRT_ERR_OUT_OF_MEMORY, "launch counters");
     SplitParams sp{g.extent};
@@ -1674,6 +1677,7 @@ int brt_traverse(const brt_scene *cs, uint32_t geom_type, const brt_rays *rays, 
   TraceLaunch cfg{trace_precision(L.precision, s->dev->views[geom_type]), L.flags | BRT_FLAG_HITS_INOUT,
                   L.block_size, true, L.schedule, L.refill, nullptr, nullptr};
   cfg.ring = L.stack_ring;
+  cfg.pool = L.pool_size;
   cfg.overwrite = true;
   cfg.counter = s->dev->counter_group(static_cast<hipStream_t>(stream));
   if (!cfg.counter) return fail(BRT_ERR_OUT_OF_MEMORY, "launch counters");
@@ -1726,6 +1730,7 @@ int brt_intersect_f32(const brt_scene *cs, const brt_rays_f32 *rays, const brt_h
     if (!g.present || g.boxes.n == 0) continue;
     TraceLaunch cfg{L.precision, L.flags, L.block_size, first, L.schedule, L.refill, nullptr, nullptr};
     cfg.ring = L.stack_ring;
+    cfg.pool = L.pool_size;
     cfg.counter = s->dev->counter_group(st);
     if (!cfg.counter) return fail(BRT_ERR_OUT_OF_MEMORY, "launch counters");
     hipError_t e = launch_trace_f32((GeomType)t, s->dev->views[t], rv, hv, n, cfg, st);
@@ -1913,6 +1918,7 @@ int batch_enqueue(brt_batch *b, hipStream_t st) {
     TraceLaunch cfg{b->L.precision, b->L.flags, b->L.block_size, false, SCHEDULE_PERSISTENT, b->L.refill,
                     b->counters + (size_t)t * COUNTER_GROUP, nullptr};
     cfg.ring = b->L.stack_ring;
+    cfg.pool = b->L.pool_size;
     hipError_t e = launch_trace_batch((GeomType)t, static_cast<const BatchJob *>(b->d_jobs[t]), b->njobs[t],
                                       b->total[t], cfg, st);
     if (e != hipSuccess) return fail(BRT_ERR_DEVICE, std::string("batch launch failed: ") + hipGetErrorString(e));
@@ -2078,6 +2084,7 @@ int brt_mirror_trace(const brt_scene *cs, const brt_rays *rays, uint64_t n, uint
                 nullptr, nullptr, nullptr, nullptr, nullptr};
     TraceLaunch cfg{L.precision, 0u, L.block_size, true, L.schedule, L.refill, nullptr, nullptr};
     cfg.ring = L.stack_ring;
+    cfg.pool = L.pool_size;
     cfg.counter = s->dev->counter_group(st);
     if (!cfg.counter) return fail(BRT_ERR_OUT_OF_MEMORY, "launch counters");
     SplitParams sp{s->geo[TRIANGLES].extent};

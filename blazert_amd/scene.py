@@ -159,7 +159,7 @@ class Scene:
     def intersect(self, org, dirs, tmin=None, tmax=None, *, precision: int = capi.PRECISION_FP64,
                   any_hit: bool = False, normals: bool = False, hits: dict | None = None, block_size: int = 0,
                   schedule: int = capi.SCHEDULE_PERSISTENT, refill: int = 0, sort: int = capi.SORT_NONE,
-                  stack_ring: int = 0, stream=None) -> dict:
+                  stack_ring: int = 0, pool_size: int = 0, stream=None) -> dict:
         """Batched intersect1 over n rays. org / dirs: (n,3) CUDA tensors or tuples of three columns, in the
         scene type (float64, or float32 for a Scene<float>).
 
@@ -194,7 +194,7 @@ class Scene:
         H = capi.Hits(hits["t"].data_ptr(), hits["u"].data_ptr(), hits["v"].data_ptr(), hits["prim_id"].data_ptr(),
                       hits["geom_id"].data_ptr(), *([nrm[k].data_ptr() for k in range(3)] if nrm is not None
                                                     else [None, None, None]))
-        L = capi.Launch(precision, flags, block_size, schedule, refill, sort, stack_ring)
+        L = capi.Launch(precision, flags, block_size, schedule, refill, sort, stack_ring, pool_size)
         on = _OnStream(stream, dev)
         check(self._fn("brt_intersect")(self._h, C.byref(rays), C.byref(H), n, C.byref(L), on.handle))
         on.hold(cols_o, cols_d, tmn, tmx, hits)
@@ -292,7 +292,8 @@ class Batch:
     directions as for ``Scene.intersect``. ``hits[i]`` holds item i's results after a launch or replay."""
 
     def __init__(self, items, *, precision: int = capi.PRECISION_FP64, any_hit: bool = False,
-                 normals: bool = False, block_size: int = 0, refill: int = 0, stack_ring: int = 0):
+                 normals: bool = False, block_size: int = 0, refill: int = 0, stack_ring: int = 0,
+                 pool_size: int = 0):
         self._keep = []
         self.scenes = []
         self.hits = []
@@ -330,7 +331,7 @@ class Batch:
         for d in devices:
             torch.cuda.current_stream(d).synchronize()
         L = capi.Launch(precision, capi.FLAG_ANY_HIT if any_hit else 0, block_size, capi.SCHEDULE_PERSISTENT, refill,
-                        capi.SORT_NONE, stack_ring)
+                        capi.SORT_NONE, stack_ring, pool_size)
         h = C.c_void_p()
         check(lib().brt_batch_create(arr, len(items), C.byref(L), C.byref(h)))
         self._h = h

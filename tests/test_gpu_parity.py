@@ -202,6 +202,9 @@ def test_icosphere8_full_persistent_equals_per_thread(dev, ico8):
         a = gpu_intersect(p, org, dirs, dev, precision=precision)
         b = gpu_intersect(p, org, dirs, dev, precision=precision, schedule=1)
         assert_bit_exact(a, b)
+        # 64-index claims (the pool_size launch option) hand out the same rays
+        c = gpu_intersect(p, org, dirs, dev, precision=precision, pool_size=64)
+        assert_bit_exact(a, c)
 
 
 def test_icosphere8_full_incoherent_properties(dev, ico8):

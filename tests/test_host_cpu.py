@@ -174,6 +174,7 @@ def test_launch_options_validated_on_host():
     rays = capi.Rays(1, 1, 1, 1, 1, 1, None, None)  # never dereferenced: validation fails first
     hits = capi.Hits(1, 1, 1, 1, 1, None, None, None)
     for L in (capi.Launch(0, 0, 512, 0, 0, 0), capi.Launch(0, 0, 96, 0, 0, 0), capi.Launch(7, 0, 0, 0, 0, 0),
-              capi.Launch(0, 0, 0, 5, 0, 0), capi.Launch(0, 0, 0, 0, 65, 0), capi.Launch(0, 0, 0, 0, 0, 9)):
+              capi.Launch(0, 0, 0, 5, 0, 0), capi.Launch(0, 0, 0, 0, 65, 0), capi.Launch(0, 0, 0, 0, 0, 9),
+              capi.Launch(0, 0, 0, 0, 0, 0, 0, 100), capi.Launch(0, 4, 0, 0, 0, 0)):
         rc = capi.lib().brt_intersect(s.handle, C.byref(rays), C.byref(hits), 10, C.byref(L), None)
         assert rc == capi.ERR_INVALID_ARGUMENT, (L.block_size, rc)
