@@ -224,8 +224,12 @@ def adjoint_leg(args, dev) -> dict:
     O, D = cols(org), cols(dirs)
     Wt = torch.from_numpy(w).to(dev)
     fwd_ms, _ = time_launch(lambda: mirror_trace(scene, O, D, B), args.steps, args.warmup)
+    fwd32_ms, _ = time_launch(lambda: mirror_trace(scene, O, D, B, precision=1), args.steps, args.warmup)
+    out32 = mirror_trace(scene, O, D, B, precision=1)
     out = mirror_trace(scene, O, D, B)
     torch.cuda.synchronize()
+    fwd32_same = float((out32["prim_id"] == out["prim_id"]).all(dim=0).double().mean().item())
+    del out32
     prim = out["prim_id"]
     gv = torch.zeros((len(V), 3), dtype=torch.float64, device=dev)
 
@@ -236,7 +240,8 @@ def adjoint_leg(args, dev) -> dict:
     adj_ms, _ = time_launch(step, args.steps, args.warmup)
     _, go, gd = step()
     torch.cuda.synchronize()
-    log(f"mirror: forward {fwd_ms:.3f} ms ({B} bounces), adjoint {adj_ms:.3f} ms -> {n / adj_ms / 1e3:.1f} Mrays/s")
+    log(f"mirror: forward {fwd_ms:.3f} ms ({B} bounces; split precision {fwd32_ms:.3f} ms, same face ids on "
+        f"{fwd32_same:.6f} of the rays), adjoint {adj_ms:.3f} ms -> {n / adj_ms / 1e3:.1f} Mrays/s")
     pid = prim.cpu().numpy().view(np.uint32)
     gv_h, go_h, gd_h = gv.cpu().numpy(), go.cpu().numpy(), gd.cpu().numpy()
     threads = len(os.sched_getaffinity(0))
@@ -333,8 +338,29 @@ def adjoint_leg(args, dev) -> dict:
             ray_err = max(ray_err, abs(fd[r] - g[r]) / max(abs(g[r]), 1e-3))
 
     hits = int(valid.sum())
+    # the vertex scatter is the bound: every bounce hit adds three FP64 corner gradients (9 doubles) at random
+    # addresses of the (nv, 3) gradient; the hot box vertices are aggregated in LDS first, the sphere's are not.
+    # Ceiling: random FP64 atomic adds into a buffer of the same size, timed with torch's index_add_ (atomics)
+    sph = valid & (pid < len(F) - 12)
+    n_atomics = int(sph.sum()) * 9
+    idx = torch.from_numpy((W.Rng(77).uniform(n_atomics) * (3 * len(V))).astype(np.int64)).to(dev)
+    ones = torch.ones(n_atomics, dtype=torch.float64, device=dev)
+    flat = torch.zeros(3 * len(V), dtype=torch.float64, device=dev)
+    at_ms, _ = time_launch(lambda: flat.index_add_(0, idx, ones), max(3, args.steps // 4), 1)
+    del idx, ones, flat
+    at_peak = n_atomics / at_ms / 1e6
+    at_achieved = n_atomics / adj_ms / 1e6
+    log(f"adjoint: {n_atomics} sphere-vertex FP64 atomics, {at_achieved:.1f} G/s in the adjoint; random-atomic "
+        f"ceiling {at_peak:.1f} G/s")
     return {"mrays_s": round(n / adj_ms / 1e3, 3), "unit": "Mrays/s", "ms_per_step": round(adj_ms, 4),
+            "roofline": {"bound": "fp64 global atomics (vertex scatter)", "achieved": round(at_achieved, 2),
+                         "peak": round(at_peak, 2), "unit": "G atomics/s", "frac": round(at_achieved / at_peak, 3),
+                         "atomics_per_launch": n_atomics,
+                         "peak_source": "torch index_add_ of the same count of FP64 adds at random addresses of "
+                                        "a buffer the size of dL/dvertex, timed here"},
             "forward_ms": round(fwd_ms, 4), "forward_bounce_mrays_s": round(B * n / fwd_ms / 1e3, 3),
+            "forward_split_precision_ms": round(fwd32_ms, 4),
+            "forward_split_precision_same_face_ids": fwd32_same,
             "fwd_plus_adjoint_mrays_s": round(n / (fwd_ms + adj_ms) / 1e3, 3),
             "workload": f"displaced icosphere (1,310,720 tris) + 12-tri box, {n} rays x {B} mirror bounces, "
                         "loss = sum w*t, w ~ U(0,1)",

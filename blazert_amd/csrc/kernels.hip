@@ -153,21 +153,25 @@ __device__ __forceinline__ bool tri_filter(const TriFilt &T, const float D[3], c
                                            float tmin32, float h32) {
   const float u = 5.9604644775390625e-08f;
   const float tol = 9e-16f;  // 4 DBL_EPSILON rounded up
-  const float det = T.n[0] * D[0] + T.n[1] * D[1] + T.n[2] * D[2];
+  // fused multiply-adds round once where a product and a sum round twice, so the bounds below hold for them too
+  auto dot3 = [](float a0, float a1, float a2, float b0, float b1, float b2) {
+    return __builtin_fmaf(a0, b0, __builtin_fmaf(a1, b1, a2 * b2));
+  };
+  const float det = dot3(T.n[0], T.n[1], T.n[2], D[0], D[1], D[2]);
   const float ad = fabsf(det);
   if (!(ad > T.ed)) return true;  // the sign of det is not certain
   const float c0 = T.v0[0] - O[0], c1 = T.v0[1] - O[1], c2 = T.v0[2] - O[2];
-  const float rx = D[1] * c2 - D[2] * c1;
-  const float ry = D[2] * c0 - D[0] * c2;
-  const float rz = D[0] * c1 - D[1] * c0;
+  const float rx = __builtin_fmaf(D[1], c2, -(D[2] * c1));
+  const float ry = __builtin_fmaf(D[2], c0, -(D[0] * c2));
+  const float rz = __builtin_fmaf(D[0], c1, -(D[1] * c0));
   const float Pm = T.vmax + om;
   const float *a = T.e1, *b = T.e2;
-  const float us = product_sign(rx * b[0] + ry * b[1] + rz * b[2], det);
-  const float vs = product_sign(rx * a[0] + ry * a[1] + rz * a[2], det);
+  const float us = product_sign(dot3(rx, ry, rz, b[0], b[1], b[2]), det);
+  const float vs = product_sign(dot3(rx, ry, rz, a[0], a[1], a[2]), det);
   if (us < -__builtin_fmaf(T.kub, Pm, tol) || vs < -__builtin_fmaf(T.kua, Pm, tol)) return false;
   const float Ew = __builtin_fmaf(1.2f * (T.kua + T.kub), Pm, __builtin_fmaf(1.3f, T.ed, tol));
   if ((ad - us) - vs < -Ew) return false;
-  const float ts = product_sign(T.n[0] * c0 + T.n[1] * c1 + T.n[2] * c2, det);
+  const float ts = product_sign(dot3(T.n[0], T.n[1], T.n[2], c0, c1, c2), det);
   const float Et = __builtin_fmaf(1.2f * T.ed, Pm, 1e-29f);
   // t >= |det| tmin: a lower bound of the right side (tmin32 <= tmin; no bound for negative tmin)
   if (tmin32 >= 0.0f) {

@@ -2063,12 +2063,13 @@ int brt_mirror_trace(const brt_scene *cs, const brt_rays *rays, uint64_t n, uint
   brt_launch L{};
   if (launch) L = *launch;
   if (int rc = check_launch(L)) return rc;
-  if (L.precision != BRT_PRECISION_FP64) return fail(BRT_ERR_UNSUPPORTED, "mirror tracing runs the FP64 path");
   if (L.flags) return fail(BRT_ERR_INVALID_ARGUMENT, "mirror tracing takes no launch flags");
   if (L.sort != BRT_SORT_NONE) return fail(BRT_ERR_UNSUPPORTED, "mirror tracing does not sort rays");
   int rc = mirror_check(s, n, bounces);
   if (rc) return rc;
   if (n == 0) return BRT_OK;
+  if (L.precision == BRT_PRECISION_FP32_TRAV_FP64_HIT)
+    if (int rc2 = split_supported(s->geo[TRIANGLES])) return rc2;
   const BvhView &bvh = s->dev->views[TRIANGLES];
   hipStream_t st = static_cast<hipStream_t>(stream);
   for (uint32_t b = 0; b < bounces; b++) {
@@ -2082,7 +2083,8 @@ int brt_mirror_trace(const brt_scene *cs, const brt_rays *rays, uint64_t n, uint
     }
     HitsView hv{out->t + off, out->u ? out->u + off : nullptr, out->v ? out->v + off : nullptr, out->prim_id + off,
                 nullptr, nullptr, nullptr, nullptr, nullptr};
-    TraceLaunch cfg{L.precision, 0u, L.block_size, true, L.schedule, L.refill, nullptr, nullptr};
+    TraceLaunch cfg{trace_precision(L.precision, bvh), 0u, L.block_size, true, L.schedule, L.refill, nullptr,
+                    nullptr};
     cfg.ring = L.stack_ring;
     cfg.pool = L.pool_size;
     cfg.counter = s->dev->counter_group(st);

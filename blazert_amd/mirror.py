@@ -35,12 +35,13 @@ def _cuda_on(x, dev, what):
 
 def mirror_trace(scene: Scene, org, dirs, bounces: int = 3, bounce_tmin: float = DEFAULT_BOUNCE_TMIN, tmin=None,
                  tmax=None, *, uv: bool = False, block_size: int = 0, schedule: int = capi.SCHEDULE_PERSISTENT,
-                 refill: int = 0, stream=None) -> dict:
+                 refill: int = 0, precision: int = capi.PRECISION_FP64, stream=None) -> dict:
     """Traces `bounces` mirror bounces of n rays. org / dirs: (n,3) float64 CUDA tensors or three columns.
 
     Returns CUDA tensors t (bounces, n), prim_id (bounces, n; int32 holding uint32 bits, -1 = miss), position and
     direction (3, bounces, n) (x, y, z planes: the next bounce's origin and direction; NaN after a miss) and, with
-    uv=True, u and v (bounces, n). Asynchronous on `stream`.
+    uv=True, u and v (bounces, n). precision=PRECISION_FP32_TRAV_FP64_HIT traces each bounce with FP32 traversal
+    and FP64 hits. Asynchronous on `stream`.
     """
     import torch
 
@@ -63,7 +64,7 @@ def mirror_trace(scene: Scene, org, dirs, bounces: int = 3, bounce_tmin: float =
                         out["v"].data_ptr() if uv else None, out["prim_id"].data_ptr(),
                         P[0].data_ptr(), P[1].data_ptr(), P[2].data_ptr(), D[0].data_ptr(), D[1].data_ptr(),
                         D[2].data_ptr())
-    L = capi.Launch(capi.PRECISION_FP64, 0, block_size, schedule, refill, capi.SORT_NONE)
+    L = capi.Launch(precision, 0, block_size, schedule, refill, capi.SORT_NONE)
     on = _OnStream(stream, dev)
     check(lib().brt_mirror_trace(scene.handle, C.byref(rays), n, B, float(bounce_tmin), C.byref(L), C.byref(mo),
                                  on.handle))

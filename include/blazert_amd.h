@@ -277,7 +277,9 @@ typedef struct brt_mirror_out {
   double *rx, *ry, *rz;  /* mirror direction: the next bounce's direction */
 } brt_mirror_out;
 
-/* launch may be NULL; precision must be FP64, flags 0, sort NONE. Asynchronous on `stream`. */
+/* launch may be NULL; flags must be 0 and sort NONE. With BRT_PRECISION_FP32_TRAV_FP64_HIT each bounce is traced
+ * by the split-precision path (hit ids as FP64 except near-ties; a tie resolved the other way reflects off the other
+ * face, so that ray's later bounces differ). Asynchronous on `stream`. */
 int brt_mirror_trace(const brt_scene *scene, const brt_rays *rays, uint64_t n, uint32_t bounces, double bounce_tmin,
                      const brt_launch *launch, const brt_mirror_out *out, void *stream);
 

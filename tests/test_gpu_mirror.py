@@ -222,3 +222,24 @@ def test_autograd_function(dev, adjoint_case):
     assert np.abs(Vt.grad.cpu().numpy() - c["gv"]).max() <= 1e-10 * scale
     assert np.abs(O.grad.cpu().numpy() - c["go"]).max() <= 1e-10 * np.abs(c["go"]).max()
     assert np.abs(D.grad.cpu().numpy() - c["gd"]).max() <= 1e-10 * np.abs(c["gd"]).max()
+
+
+def test_split_precision_forward(dev, adjoint_case):
+    """Bounces traced with FP32 traversal and FP64 hits give the FP64 forward's face ids, distances and positions on
+    every ray whose bounces have no near-tie (a box face is two coplanar triangles: a ray through their shared
+    diagonal may take the other triangle, which reflects the same way)."""
+    from blazert_amd.mirror import mirror_trace
+    c = adjoint_case
+    O, D = torch.from_numpy(c["o"]).to(dev), torch.from_numpy(c["d"]).to(dev)
+    a = mirror_trace(c["scene"], O, D, 3)
+    b = mirror_trace(c["scene"], O, D, 3, precision=1)
+    torch.cuda.synchronize()
+    pa = a["prim_id"].cpu().numpy().view(np.uint32)
+    pb = b["prim_id"].cpu().numpy().view(np.uint32)
+    same = (pa == pb).all(axis=0)
+    assert same.mean() > 0.999, same.mean()
+    ta, tb = a["t"].cpu().numpy()[:, same], b["t"].cpu().numpy()[:, same]
+    assert np.array_equal(ta.view(np.uint64), tb.view(np.uint64))
+    xa = a["position"].cpu().numpy()[:, :, same]
+    xb = b["position"].cpu().numpy()[:, :, same]
+    assert np.array_equal(np.nan_to_num(xa).view(np.uint64), np.nan_to_num(xb).view(np.uint64))

@@ -61,7 +61,7 @@ def test_mirror_validation():
         capi.ERR_INVALID_ARGUMENT
     assert lib.brt_mirror_trace(tri.handle, C.byref(rays), 4, 3, -1.0, None, C.byref(out), None) == \
         capi.ERR_INVALID_ARGUMENT
-    L = capi.Launch(capi.PRECISION_FP32_TRAV_FP64_HIT, 0, 0, 0, 0, 0)
+    L = capi.Launch(capi.PRECISION_FP32_TRAV_FP64_HIT, 0, 0, 0, 0, capi.SORT_MORTON)  # mirror bounces keep ray order
     assert lib.brt_mirror_trace(tri.handle, C.byref(rays), 4, 3, 1e-9, C.byref(L), C.byref(out), None) == \
         capi.ERR_UNSUPPORTED
     g = capi.MirrorGrads(None, 1, None, None, None, None, None, None, None, None, None)  # gp x without y, z
