@@ -433,9 +433,15 @@ def distance_leg(args, dev, v, f, scene, osc) -> dict:
     ms_p, _ = time_launch(lambda: scene.distance_to_surface(0, qp, ids), args.steps, 2)
     log(f"closest surface {n / ms_c / 1e3:.2f} Mpoints/s (exact on {ns}: {exact}), "
         f"pairs {npair / ms_p / 1e3:.1f} Mpairs/s")
+    # per pair: the point (24 B) and primitive id (4 B) in, the leaf position of the id (4 B), the triangle's three
+    # vertices in leaf order (72 B), the distance out (8 B)
+    pair_bytes = 24 + 4 + 4 + 72 + 8
+    ach = pair_bytes * npair / (ms_p * 1e-3) / 1e9
     return {"closest_surface_mpoints_s": round(n / ms_c / 1e3, 3), "closest_ms": round(ms_c, 3), "points": n,
             "closest_bit_exact_vs_bruteforce_sample": exact, "sample": ns,
-            "pairs_mpairs_s": round(npair / ms_p / 1e3, 2), "pairs": npair}
+            "pairs_mpairs_s": round(npair / ms_p / 1e3, 2), "pairs": npair,
+            "pairs_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_pair": pair_bytes}}
 
 
 def cpu_model() -> str:
@@ -478,19 +484,21 @@ def kernel_sources_hash() -> str:
 
 
 def measured_traffic():
-    """HBM bytes per launch of the FP64 traversal kernel from profiles/traffic_fp64_icosphere.json (rocprofv3 --pmc
-    FETCH_SIZE and WRITE_SIZE of this workload, copy-calibrated). Reported only when the kernel sources hash to the
-    value recorded with the measurement; otherwise null, since the figure would describe other code."""
+    """HBM bytes per launch of the FP64 and split traversal kernels from profiles/traffic_fp64_icosphere.json
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this workload, copy-calibrated). Reported only when the kernel
+    sources hash to the value recorded with the measurement; otherwise null, since the figure would describe other
+    code."""
     tfile = os.path.join(ROOT, "profiles", "traffic_fp64_icosphere.json")
     if not os.path.exists(tfile):
-        return None, "no measurement (profiles/traffic_fp64_icosphere.json missing)"
+        return None, None, "no measurement (profiles/traffic_fp64_icosphere.json missing)"
     rec = json.load(open(tfile))
     cur = kernel_sources_hash()
     if rec.get("kernel_sources_sha16") != cur:
-        return None, (f"profiles/traffic_fp64_icosphere.json was measured on kernel sources "
-                      f"{rec.get('kernel_sources_sha16')}, these are {cur}: not reported")
-    return rec.get("traffic_bytes"), ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this workload, "
-                                      f"copy-calibrated: profiles/traffic_fp64_icosphere.json (kernel sources {cur})")
+        return None, None, (f"profiles/traffic_fp64_icosphere.json was measured on kernel sources "
+                            f"{rec.get('kernel_sources_sha16')}, these are {cur}: not reported")
+    return rec.get("traffic_bytes"), rec.get("split", {}).get("traffic_bytes"), (
+        "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this workload, copy-calibrated: "
+        f"profiles/traffic_fp64_icosphere.json (kernel sources {cur})")
 
 
 def float_scene_leg(args, dev, v, f, org, dirs) -> dict:
@@ -580,11 +588,25 @@ def user_geometry_leg(args, dev) -> dict:
     hits = int((hb["prim_id"] != -1).sum())
     s.close()
     bvh.close()
+    # algorithmic bytes per ray (both paths read the same tree and sphere records): the reference traversal's node
+    # visits x 56 B + sphere tests x 36 B + 96 B of ray and hit, from the oracle's replay on the first 2^18 rays
+    from oracle import oracle as OR
+    osph = OR.Scene()
+    osph.add_spheres(c, r)
+    osph.commit()
+    nsmp = 1 << 18
+    tr = osph.intersect_batch(org[:nsmp], dirs[:nsmp], normals=False, trace=True,
+                              nthreads=len(os.sched_getaffinity(0)))["trace"]
+    bpr = S_RAY_HIT + float(tr["node_visits"].mean()) * S_NODE64 + float(tr["prim_tests"].mean()) * 36
+    roof = {k: {"bound": "hbm", "achieved": round(bpr * n / (t * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(bpr * n / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            for k, t in (("user", ms_user), ("builtin", ms_builtin))}
     log(f"user collection {n / ms_user / 1e3:.1f} Mrays/s ({ms_user:.3f} ms) vs built-in spheres "
         f"{n / ms_builtin / 1e3:.1f} Mrays/s, identical: {exact}")
     return {"value": round(n / ms_user / 1e3, 3), "unit": "Mrays/s", "ms_per_step": round(ms_user, 4),
             "builtin_spheres_mrays_s": round(n / ms_builtin / 1e3, 3), "builtin_ms": round(ms_builtin, 4),
             "identical_to_builtin_all_rays": exact, "hits": hits, "tree_build_s": round(build_s, 3),
+            "bytes_per_ray": round(bpr, 1), "roofline_user": roof["user"], "roofline_builtin": roof["builtin"],
             "workload": "1,048,576 spheres (r in [0.1, 0.5]) in [-40, 40]^3, 4,194,304 incoherent rays from inside, "
                         "example sphere intersector through include/blazert_amd/user_geometry.h"}
 
@@ -714,7 +736,7 @@ def main():
     exact = bool(np.array_equal(g_sub_t[ok].view(np.uint64), ref["t"][ok].view(np.uint64))
                  and np.array_equal(g_sub_p[ok], ref["prim_id"][ok]))
     achieved = bpr64 * n / (ms64 * 1e-3) / 1e9
-    traffic, traffic_note = measured_traffic()
+    traffic, traffic32, traffic_note = measured_traffic()
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
                        "bytes_per_ray": round(bpr64, 1), "node_visits_per_ray": round(visits, 2),
@@ -725,7 +747,8 @@ def main():
     ach32 = bpr32 * n / (ms32 * 1e-3) / 1e9
     out["split_precision"]["roofline"] = {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
                                           "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
-                                          "bytes_per_ray": round(bpr32, 1)}
+                                          "bytes_per_ray": round(bpr32, 1), "traffic": traffic32,
+                                          "traffic_source": traffic_note}
     out["parity"] = {"sample_rays": ns, "fp64_bit_exact_vs_oracle": exact,
                      "reference_stack_overflow_rays": int((~ok).sum())}
     out["split_precision"]["disagreements"] = classify_disagreements(osc, org, dirs, dis_idx, dis_other)
