@@ -338,26 +338,20 @@ def adjoint_leg(args, dev) -> dict:
             ray_err = max(ray_err, abs(fd[r] - g[r]) / max(abs(g[r]), 1e-3))
 
     hits = int(valid.sum())
-    # the vertex scatter is the bound: every bounce hit adds three FP64 corner gradients (9 doubles) at random
-    # addresses of the (nv, 3) gradient; the hot box vertices are aggregated in LDS first, the sphere's are not.
-    # Ceiling: random FP64 atomic adds into a buffer of the same size, timed with torch's index_add_ (atomics)
-    sph = valid & (pid < len(F) - 12)
-    n_atomics = int(sph.sum()) * 9
-    idx = torch.from_numpy((W.Rng(77).uniform(n_atomics) * (3 * len(V))).astype(np.int64)).to(dev)
-    ones = torch.ones(n_atomics, dtype=torch.float64, device=dev)
-    flat = torch.zeros(3 * len(V), dtype=torch.float64, device=dev)
-    at_ms, _ = time_launch(lambda: flat.index_add_(0, idx, ones), max(3, args.steps // 4), 1)
-    del idx, ones, flat
-    at_peak = n_atomics / at_ms / 1e6
-    at_achieved = n_atomics / adj_ms / 1e6
-    log(f"adjoint: {n_atomics} sphere-vertex FP64 atomics, {at_achieved:.1f} G/s in the adjoint; random-atomic "
-        f"ceiling {at_peak:.1f} G/s")
+    # algorithmic bytes: per ray 48 B of rays, 12 B of face ids, 24 B of dL/dt and 48 B of ray gradients; per bounce
+    # hit a 12 B face record and 72 B of vertices, read twice (forward recompute and reverse sweep)
+    adj_bytes = 132 * n + 168 * hits
+    adj_ach = adj_bytes / (adj_ms * 1e-3) / 1e9
+    sph_corners = int((valid & (pid < len(F) - 12)).sum()) * 3
+    log(f"adjoint: {adj_ach:.0f} GB/s algorithmic ({adj_ach / HBM_PEAK_GBS:.3f} of HBM); {sph_corners} sphere-vertex "
+        f"corner gradients scattered with FP64 atomics")
     return {"mrays_s": round(n / adj_ms / 1e3, 3), "unit": "Mrays/s", "ms_per_step": round(adj_ms, 4),
-            "roofline": {"bound": "fp64 global atomics (vertex scatter)", "achieved": round(at_achieved, 2),
-                         "peak": round(at_peak, 2), "unit": "G atomics/s", "frac": round(at_achieved / at_peak, 3),
-                         "atomics_per_launch": n_atomics,
-                         "peak_source": "torch index_add_ of the same count of FP64 adds at random addresses of "
-                                        "a buffer the size of dL/dvertex, timed here"},
+            "roofline": {"bound": "hbm", "achieved": round(adj_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(adj_ach / HBM_PEAK_GBS, 4), "bytes_per_launch": adj_bytes,
+                         "bytes": "132 B per ray + 168 B per bounce hit"},
+            "sphere_corner_atomics": {"corners": sph_corners, "fp64_adds": 3 * sph_corners,
+                                      "note": "corners whose vertex finds no slot in the block's LDS table: three "
+                                              "lanes add x, y, z of one vertex in one request"},
             "forward_ms": round(fwd_ms, 4), "forward_bounce_mrays_s": round(B * n / fwd_ms / 1e3, 3),
             "forward_split_precision_ms": round(fwd32_ms, 4),
             "forward_split_precision_same_face_ids": fwd32_same,

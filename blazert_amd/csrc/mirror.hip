@@ -10,7 +10,8 @@
 // of the stored face, no intermediates kept between passes), then runs the chain backwards in FP64: dL/d(origin,
 // raw direction) per ray, and dL/dvertex scatter-added per face corner. The scatter goes through a per-block LDS
 // table (open addressing keyed by vertex) so that vertices hit by many rays of a block (large walls) cost one global
-// atomic per block instead of one per ray; vertices that find no slot add to global memory directly.
+// atomic per block instead of one per ray; corners whose vertex finds no slot are added to global memory by the
+// wave together, three lanes per vertex (x, y, z in one request).
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <stdint.h>
@@ -125,55 +126,110 @@ __device__ __forceinline__ void bounce_backward(const Bounce &B, D3 p0, D3 p1, D
   gd = (1.0 / B.s) * (gdh - dot(gdh, B.dh) * B.dh);
 }
 
-constexpr uint32_t VTAB = 1024;  // LDS vertex slots per block (power of two)
+constexpr uint32_t VTAB = 1024;  // LDS vertex slots per block: 128 buckets of 8
 constexpr uint32_t VTAB_EMPTY = 0xFFFFFFFFu;
-constexpr int VTAB_PROBES = 8;   // linear probing: hot vertices find a slot even when their home slot is taken
+constexpr uint32_t ADJ_WAVES = 4;          // waves of a 256-thread block
+constexpr uint32_t ADJ_STAGE = 64u * 3u;   // staged corners per wave and bounce: three per lane
 
-__device__ __forceinline__ uint32_t vtab_slot(uint32_t v) { return (v * 2654435761u) >> 22; }  // top 10 bits
+__device__ __forceinline__ uint32_t vtab_bucket(uint32_t v) { return (v * 2654435761u) >> 25; }  // top 7 bits
 
+// Orders the LDS accesses of one wave: what its lanes wrote before is visible to its lanes after.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Every wave takes 64 consecutive rays per iteration, so each wave runs the loop body with all its lanes (rays past
+// n are idle lanes): the corner scatter below is a wave-wide operation.
 template <int NB>
 __global__ void __launch_bounds__(256) k_mirror_adjoint(RaysView rays, const uint32_t *__restrict__ prims,
                                                         const double *__restrict__ V, const uint32_t *__restrict__ F,
                                                         uint32_t nf, uint64_t n, MirrorGrads g) {
-  __shared__ uint32_t keys[VTAB];
+  __shared__ __align__(16) uint32_t keys[VTAB];
   __shared__ double acc[3 * VTAB];
+  __shared__ uint32_t stage_v[ADJ_WAVES][ADJ_STAGE];
+  __shared__ double stage_g[ADJ_WAVES][ADJ_STAGE][3];
   for (uint32_t s = threadIdx.x; s < VTAB; s += blockDim.x) {
     keys[s] = VTAB_EMPTY;
     acc[3 * s] = acc[3 * s + 1] = acc[3 * s + 2] = 0.0;
   }
   __syncthreads();
-  // finds or claims the vertex's LDS slot (linear probing from its hash) and adds there; a vertex that finds no
-  // slot within VTAB_PROBES (the table is full of other vertices) adds to global memory directly
-  auto scatter = [&](uint32_t v, D3 gvec) {
-    uint32_t slot = vtab_slot(v);
-    for (int p = 0; p < VTAB_PROBES; p++) {
-      uint32_t k = keys[slot];
-      if (k == VTAB_EMPTY) {
-        k = atomicCAS(&keys[slot], VTAB_EMPTY, v);
-        if (k == VTAB_EMPTY) k = v;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  // Adds the corner gradient to the vertex's LDS slot in its bucket of 8 (read as two 16-byte loads), claiming the
+  // bucket's first free slot for a vertex not yet there; false when the bucket is full of other vertices. Slots are
+  // only ever claimed, so a failed claim (another lane took the slot) is retried on the fuller bucket and the loop
+  // ends within 8 claims.
+  auto to_table = [&](uint32_t v, D3 gvec) -> bool {
+    const uint32_t b0 = vtab_bucket(v) * 8u;
+    for (int attempt = 0; attempt < 8; attempt++) {
+      const uint4 ka = *reinterpret_cast<const uint4 *>(&keys[b0]);
+      const uint4 kb = *reinterpret_cast<const uint4 *>(&keys[b0 + 4]);
+      const uint32_t ks[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
+      int found = -1, free_slot = -1;
+#pragma unroll
+      for (int j = 7; j >= 0; j--) {
+        if (ks[j] == v) found = j;
+        if (ks[j] == VTAB_EMPTY) free_slot = j;
       }
-      if (k == v) {
-        unsafeAtomicAdd(&acc[3 * slot], gvec.x);
-        unsafeAtomicAdd(&acc[3 * slot + 1], gvec.y);
-        unsafeAtomicAdd(&acc[3 * slot + 2], gvec.z);
-        return;
+      if (found < 0) {
+        if (free_slot < 0) return false;
+        const uint32_t k = atomicCAS(&keys[b0 + free_slot], VTAB_EMPTY, v);
+        if (k != VTAB_EMPTY && k != v) continue;
+        found = free_slot;
       }
-      slot = (slot + 1) & (VTAB - 1);
+      const uint32_t slot = b0 + (uint32_t)found;
+      unsafeAtomicAdd(&acc[3 * slot], gvec.x);
+      unsafeAtomicAdd(&acc[3 * slot + 1], gvec.y);
+      unsafeAtomicAdd(&acc[3 * slot + 2], gvec.z);
+      return true;
     }
-    unsafeAtomicAdd(&g.gv[3ull * v], gvec.x);
-    unsafeAtomicAdd(&g.gv[3ull * v + 1], gvec.y);
-    unsafeAtomicAdd(&g.gv[3ull * v + 2], gvec.z);
+    return false;
+  };
+  // One bounce's three corner gradients of every active lane. Corners without an LDS slot are staged in the wave's
+  // LDS area and added to global memory with three adjacent lanes carrying x, y and z of one vertex: one atomic
+  // instruction then covers 21 vertices, each in one request, where a lane per vertex needs three instructions and
+  // three requests per vertex (random FP64 atomics are bound by requests, not by adds).
+  auto scatter3 = [&](bool act, const uint32_t *vs, const D3 *gs) {
+    bool need[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) need[c] = act && !to_table(vs[c], gs[c]);
+    uint32_t base = 0;
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const unsigned long long m = __ballot(need[c]);
+      if (need[c]) {
+        const uint32_t slot = base + (uint32_t)__popcll(m & below);
+        stage_v[wave][slot] = vs[c];
+        stage_g[wave][slot][0] = gs[c].x;
+        stage_g[wave][slot][1] = gs[c].y;
+        stage_g[wave][slot][2] = gs[c].z;
+      }
+      base += (uint32_t)__popcll(m);
+    }
+    wave_sync();
+    const uint32_t comp = lane % 3u, rec = lane / 3u;
+    for (uint32_t r0 = 0; r0 < base; r0 += 21u) {
+      const uint32_t r = r0 + rec;
+      if (lane < 63u && r < base) unsafeAtomicAdd(&g.gv[3ull * stage_v[wave][r] + comp], stage_g[wave][r][comp]);
+    }
+    wave_sync();  // the area is refilled by the next bounce
   };
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  for (uint64_t w0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); w0 < n; w0 += stride) {
+    const uint64_t i = w0 + lane;
+    const bool live = i < n;
     D3 o[NB], d[NB];
     uint32_t f[NB];
     int nb = 0;
-    o[0] = d3(rays.ox[i], rays.oy[i], rays.oz[i]);
-    d[0] = d3(rays.dx[i], rays.dy[i], rays.dz[i]);
+    if (live) {
+      o[0] = d3(rays.ox[i], rays.oy[i], rays.oz[i]);
+      d[0] = d3(rays.dx[i], rays.dy[i], rays.dz[i]);
+    }
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-      f[b] = prims[(uint64_t)b * n + i];
+      f[b] = live ? prims[(uint64_t)b * n + i] : INVALID_ID;
       if (nb == b && f[b] < nf) {  // the chain ends at the first miss (or any id that is not a face)
         nb = b + 1;
         if (b + 1 < NB) {
@@ -186,46 +242,49 @@ __global__ void __launch_bounds__(256) k_mirror_adjoint(RaysView rays, const uin
     D3 gx_next = d3(0, 0, 0), gd_next = d3(0, 0, 0);
 #pragma unroll
     for (int b = NB - 1; b >= 0; b--) {
-      if (b < nb) {
+      const bool act = b < nb;
+      uint32_t vs[3] = {0u, 0u, 0u};
+      D3 gs[3];
+      if (act) {
         const uint32_t *fc = F + 3ull * f[b];
-        const uint32_t v0 = fc[0], v1 = fc[1], v2 = fc[2];
-        const D3 p0 = vertex(V, v0), p1 = vertex(V, v1), p2 = vertex(V, v2);
+        vs[0] = fc[0];
+        vs[1] = fc[1];
+        vs[2] = fc[2];
+        const D3 p0 = vertex(V, vs[0]), p1 = vertex(V, vs[1]), p2 = vertex(V, vs[2]);
         Bounce B;
         D3 x, dn;
         bounce_forward(o[b], d[b], p0, p1, p2, B, x, dn);
         const uint64_t k = (uint64_t)b * n + i;
         const double gt = g.gt ? g.gt[k] : 0.0;
         const D3 gp = g.gpx ? d3(g.gpx[k], g.gpy[k], g.gpz[k]) : d3(0, 0, 0);
-        D3 go, gd, g0, g1, g2;
-        bounce_backward(B, p0, p1, p2, gt, gp + gx_next, gd_next, go, gd, g0, g1, g2);
-        if (g.gv) {
-          scatter(v0, g0);
-          scatter(v1, g1);
-          scatter(v2, g2);
-        }
+        D3 go, gd;
+        bounce_backward(B, p0, p1, p2, gt, gp + gx_next, gd_next, go, gd, gs[0], gs[1], gs[2]);
         gx_next = go;
         gd_next = gd;
       }
+      if (g.gv) scatter3(act, vs, gs);
     }
-    if (g.gox) {
+    if (live && g.gox) {
       g.gox[i] = gx_next.x;
       g.goy[i] = gx_next.y;
       g.goz[i] = gx_next.z;
     }
-    if (g.gdx) {
+    if (live && g.gdx) {
       g.gdx[i] = gd_next.x;
       g.gdy[i] = gd_next.y;
       g.gdz[i] = gd_next.z;
     }
   }
   __syncthreads();
+  // flush: three adjacent lanes add x, y and z of one slot's vertex (21 slots per wave and pass)
   if (g.gv) {
-    for (uint32_t s = threadIdx.x; s < VTAB; s += blockDim.x) {
-      const uint32_t v = keys[s];
-      if (v == VTAB_EMPTY) continue;
-      unsafeAtomicAdd(&g.gv[3ull * v], acc[3 * s]);
-      unsafeAtomicAdd(&g.gv[3ull * v + 1], acc[3 * s + 1]);
-      unsafeAtomicAdd(&g.gv[3ull * v + 2], acc[3 * s + 2]);
+    const uint32_t comp = lane % 3u;
+    for (uint32_t s0 = wave * 21u; s0 < VTAB; s0 += ADJ_WAVES * 21u) {
+      const uint32_t s = s0 + lane / 3u;
+      if (lane < 63u && s < VTAB) {
+        const uint32_t v = keys[s];
+        if (v != VTAB_EMPTY) unsafeAtomicAdd(&g.gv[3ull * v + comp], acc[3 * s + comp]);
+      }
     }
   }
 }
