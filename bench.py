@@ -605,6 +605,40 @@ def user_geometry_leg(args, dev) -> dict:
                         "example sphere intersector through include/blazert_amd/user_geometry.h"}
 
 
+def ceilings(scene_bytes: int, lanes: int, adjoint_rows: int, adjoint_adds: int) -> dict:
+    """The memory ceilings the kernels are compared with, measured on this GPU by blazert_amd/libbrt_microbench.so
+    (csrc/microbench.hip): `lanes` dependent chains of random 128-byte lines in a buffer of the scene's size (the
+    traversal's pattern: each record's address comes from the previous record), independent random 128-byte
+    gathers (plain and non-temporal loads), and random FP64 atomic adds into the adjoint's (vertices, 3) gradient,
+    one component per lane and as lane triples (x, y, z of one vertex in three adjacent lanes)."""
+    import ctypes as C
+
+    from blazert_amd import build as B
+    lib = C.CDLL(B.MICROBENCH_LIB)
+    U64, U32, DP = C.c_uint64, C.c_uint32, C.POINTER(C.c_double)
+    lib.brt_mb_line_chase.argtypes = [U64, U32, U32, DP]
+    lib.brt_mb_line_gather.argtypes = [U64, U64, C.c_int, DP]
+    lib.brt_mb_fp64_atomics.argtypes = [U64, U64, C.c_int, DP]
+    r = {}
+
+    def run(name, fn, *a):
+        x = C.c_double(0.0)
+        rc = fn(*a, C.byref(x))
+        if rc != 0:
+            raise RuntimeError(f"microbenchmark {name} failed: {rc}")
+        r[name] = round(x.value, 1)
+
+    run("dependent_line_chains_gbs", lib.brt_mb_line_chase, scene_bytes, lanes, 200)
+    run("random_line_gather_gbs", lib.brt_mb_line_gather, scene_bytes, 1 << 26, 0)
+    run("random_line_gather_nontemporal_gbs", lib.brt_mb_line_gather, scene_bytes, 1 << 26, 1)
+    adds = adjoint_adds - adjoint_adds % 3
+    run("fp64_atomics_per_lane_gadds", lib.brt_mb_fp64_atomics, adjoint_rows, adds, 0)
+    run("fp64_atomics_lane_triples_gadds", lib.brt_mb_fp64_atomics, adjoint_rows, adds, 1)
+    r["config"] = {"buffer_bytes": scene_bytes, "chains": lanes, "chain_steps": 200, "gathers": 1 << 26,
+                   "atomic_rows": adjoint_rows, "atomic_adds": adds}
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -618,6 +652,8 @@ def main():
     ap.add_argument("--no-distance", action="store_true", help="skip the distance-query leg")
     ap.add_argument("--no-float", action="store_true", help="skip the Scene<float> leg")
     ap.add_argument("--no-user", action="store_true", help="skip the user-defined collection leg")
+    ap.add_argument("--microbench", action="store_true",
+                    help="only run the ceiling microbenchmarks (blazert_amd/libbrt_microbench.so) and print them")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale factor for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -636,6 +672,13 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": a.numel() * 8, "copies": 3}))
+        return
+
+    # the scene's device footprint (nodes, records, ids: about 190 MB), the FP64 kernel's grid, and the mirror
+    # scene's vertices with the bench adjoint's count of sphere-vertex FP64 adds
+    CEIL = dict(scene_bytes=192 << 20, lanes=262144, adjoint_rows=655370, adjoint_adds=24221979)
+    if args.microbench:
+        print(json.dumps(ceilings(**CEIL)))
         return
 
     from blazert_amd import PRECISION_FP32_TRAV_FP64_HIT, PRECISION_FP64, Scene, workloads as W
@@ -743,6 +786,14 @@ def main():
                                           "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
                                           "bytes_per_ray": round(bpr32, 1), "traffic": traffic32,
                                           "traffic_source": traffic_note}
+    ceil = ceilings(**CEIL)
+    out["ceilings"] = ceil
+    chase = ceil["dependent_line_chains_gbs"]
+    for rf, tb, ms in ((out["roofline"], traffic, ms64), (out["split_precision"]["roofline"], traffic32, ms32)):
+        rf["dependent_line_ceiling_gbs"] = chase
+        if tb:
+            rf["traffic_gbs"] = round(tb / (ms * 1e-3) / 1e9, 1)
+            rf["traffic_frac_of_dependent_line_ceiling"] = round(tb / (ms * 1e-3) / 1e9 / chase, 4)
     out["parity"] = {"sample_rays": ns, "fp64_bit_exact_vs_oracle": exact,
                      "reference_stack_overflow_rays": int((~ok).sum())}
     out["split_precision"]["disagreements"] = classify_disagreements(osc, org, dirs, dis_idx, dis_other)
@@ -760,6 +811,11 @@ def main():
         out["batch_uneven"] = batch_leg(args, dev)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args, dev)
+        # the sphere-vertex scatter against the measured random FP64 atomic rates
+        sc = out["adjoint"]["sphere_corner_atomics"]
+        sc["per_lane_ceiling_gadds"] = ceil["fp64_atomics_per_lane_gadds"]
+        sc["lane_triples_ceiling_gadds"] = ceil["fp64_atomics_lane_triples_gadds"]
+        sc["lane_triples_floor_ms"] = round(sc["fp64_adds"] / ceil["fp64_atomics_lane_triples_gadds"] / 1e6, 4)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(osc, org, dirs, ns, threads)
     print(json.dumps(out), flush=True)

@@ -115,9 +115,26 @@ def build_user_geometry(src: str, out: str, verbose: bool = False) -> str:
     return out
 
 
+MICROBENCH_SRC = os.path.join(CSRC, "microbench.hip")
+MICROBENCH_LIB = os.path.join(HERE, "libbrt_microbench.so")
+
+
+def build_microbench(verbose: bool = False, out: str = MICROBENCH_LIB) -> str:
+    """Compiles the ceiling microbenchmarks (csrc/microbench.hip: dependent random line chains, random line gathers,
+    random FP64 atomics) into their own library, run by `bench.py` beside the kernels it measures."""
+    cc = hipcc()
+    out_tmp = out + ".tmp"
+    _run([cc, *COMMON, *DEVICE, "-shared", MICROBENCH_SRC, "-o", out_tmp])
+    os.replace(out_tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["--sanitize"]:
         build_sanitized(verbose=True)
     else:
         build(verbose=True)
         build_user_geometry(USER_GEOMETRY_EXAMPLE, USER_GEOMETRY_EXAMPLE_LIB, verbose=True)
+        build_microbench(verbose=True)

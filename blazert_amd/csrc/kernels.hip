@@ -34,6 +34,28 @@ constexpr uint32_t FLAG_ANY_HIT = 2u;
 // Internal: an FP64 launch that traces only the rays a split-precision launch of the same geometry type deferred
 // (see deferred_to_exact), and none at all when that launch deferred nothing.
 constexpr uint32_t FLAG_DEFERRED_ONLY = 1u << 31;
+// Exact paths: primitives of a leaf tested per traversal step, per primitive type (triangles, spheres, planes,
+// cylinders). They are tested in leaf order, each against the hit distance the previous one left, so the count
+// changes speed only.
+#ifndef BRT_EXACT_GROUP_TRI
+#define BRT_EXACT_GROUP_TRI 2  // 1.31M-triangle incoherent set: 2-3% faster than 1, bit-identical hits
+#endif
+#ifndef BRT_EXACT_GROUP_SPH
+#define BRT_EXACT_GROUP_SPH 1  // mixed scene: 2 and 4 per step 4% and 15% slower (the FP64 sphere test is long)
+#endif
+#ifndef BRT_EXACT_GROUP_PLA
+#define BRT_EXACT_GROUP_PLA 1  // mixed scene: 2 per step 13% slower
+#endif
+#ifndef BRT_EXACT_GROUP_CYL
+#define BRT_EXACT_GROUP_CYL 1  // mixed scene: 2 per step 18% slower
+#endif
+template <uint32_t KIND>
+constexpr int exact_group() {
+  return KIND == TRIANGLES ? BRT_EXACT_GROUP_TRI
+         : KIND == SPHERES ? BRT_EXACT_GROUP_SPH
+         : KIND == PLANES  ? BRT_EXACT_GROUP_PLA
+                           : BRT_EXACT_GROUP_CYL;
+}
 
 // Deepest stack a traversal can build: the binary FP64 DFS pushes at most one entry per level (trees are limited
 // to depth 63 at upload); the 4-wide FP32 DFS at most three per Node4 record, and a record spans at least one
@@ -713,10 +735,10 @@ __device__ __forceinline__ void trav_begin(const BvhView &bvh, const RayT<R> &r,
   }
 }
 
-// One unit of the DFS: an internal node (test both children, push far, descend near) or a leaf (test its
-// primitives in order, blazert/bvh/intersect.h:66-78), followed by pops until a node passes or the stack is empty.
-// On the split path with triangle pre-test records, a leaf step is either a pre-test of the next FILTER_GROUP
-// triangles in FP32 (tri_filter) or the FP64 test of the next one that passed it, in leaf order.
+// One unit of the DFS: an internal node (test both children, push far, descend near) or a leaf's next primitives
+// (exact_group per type, in order, blazert/bvh/intersect.h:66-78), followed by pops until a node passes or the stack
+// is empty. On the split path with triangle pre-test records, a leaf step is either a pre-test of the next
+// FILTER_GROUP triangles in FP32 (tri_filter) or the FP64 test of the next one that passed it, in leaf order.
 template <class Rec, bool SPLIT>
 __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typename Rec::real> &r, const Ray32 &q,
                                           bool any_hit, HitT<typename Rec::real> &s, Trav &t,
@@ -734,7 +756,9 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
   constexpr int NODE_Q = SPLIT ? 8 : (sizeof(R) == 8 ? 7 : 4);  // 16-byte chunks of the record used (Node64: 112 B)
   constexpr int REC_Q = (int)(sizeof(Rec) / 16);
   constexpr int FILT_Q = FILTER ? (int)(FILTER_GROUP * sizeof(TriFilt) / 16) : 0;
-  constexpr int QMAX0 = NODE_Q < REC_Q ? REC_Q : NODE_Q;
+  // exact paths: primitives tested per leaf step
+  constexpr int PG = SPLIT ? 1 : exact_group<Rec::kind>();
+  constexpr int QMAX0 = NODE_Q < PG * REC_Q ? PG * REC_Q : NODE_Q;
   constexpr int QMAX = QMAX0 < FILT_Q ? FILT_Q : QMAX0;
   static_assert(sizeof(Rec) % 16 == 0 && sizeof(Node) >= 16 * NODE_Q, "record chunking");
   const bool at_leaf = (t.cur & LEAF_BIT) != 0;
@@ -772,6 +796,12 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
     } else if (fetch_rec) {
 #pragma unroll
       for (int k = COMMON; k < REC_Q; k++) chunk[k] = src[k];
+#pragma unroll
+      for (int g = 1; g < PG; g++)
+        if (t.pos + g < t.cnt) {
+#pragma unroll
+          for (int k = g * REC_Q; k < (g + 1) * REC_Q; k++) chunk[k] = src[k];
+        }
     }
   }
   // an empty asm taking every chunk as an operand: the loads cannot be sunk into the compute branches below, so all
@@ -857,7 +887,7 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
       if (t.pos < t.cnt) return;
     }
   } else {
-    // one primitive per step: a leaf's tests (blazert/bvh/intersect.h:66-78, in order, each against the hit
+    // a few primitives per step: a leaf's tests (blazert/bvh/intersect.h:66-78, in order, each against the hit
     // distance the previous one left) are spread over consecutive steps, so lanes at leaves and lanes at inner
     // nodes run branches of similar length instead of inner-node lanes idling through a whole leaf
     if (fetch_rec) {
@@ -866,6 +896,16 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
       if (isect(rec, r, s, p)) {
         if (SPLIT) t.h32 = f32_up(s.h);
         if (any_hit) {
+          t.active = false;
+          return;
+        }
+      }
+#pragma unroll
+      for (int g = 1; g < PG; g++) {  // the leaf's next primitives, each against the hit distance just left
+        if (t.pos + 1 >= t.cnt) break;
+        __builtin_memcpy(&rec, chunk + g * REC_Q, sizeof(Rec));
+        t.pos++;
+        if (isect(rec, r, s, p + g) && any_hit) {
           t.active = false;
           return;
         }

@@ -178,3 +178,16 @@ def test_launch_options_validated_on_host():
               capi.Launch(0, 0, 0, 0, 0, 0, 0, 100), capi.Launch(0, 4, 0, 0, 0, 0)):
         rc = capi.lib().brt_intersect(s.handle, C.byref(rays), C.byref(hits), 10, C.byref(L), None)
         assert rc == capi.ERR_INVALID_ARGUMENT, (L.block_size, rc)
+
+
+def test_microbench_library_exports():
+    """The bench's ceiling microbenchmarks (blazert_amd/csrc/microbench.hip) build into their own library, apart from
+    the ray-query library, and export their three entry points."""
+    import ctypes as C
+    from blazert_amd import build as B
+    if not os.path.exists(B.MICROBENCH_LIB):
+        B.build_microbench()
+    lib = C.CDLL(B.MICROBENCH_LIB)
+    for name in ("brt_mb_line_chase", "brt_mb_line_gather", "brt_mb_fp64_atomics"):
+        assert hasattr(lib, name), name
+    assert not hasattr(capi.lib(), "brt_mb_line_chase")
