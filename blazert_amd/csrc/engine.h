@@ -116,6 +116,8 @@ struct SplitParams {
 
 // Split precision: writes the FP32 pre-test records (TriFilt, n + FILTER_PAD of them) of n leaf-order TriRecs.
 hipError_t launch_tri_filter_records(const void *recs, uint32_t n, void *out, hipStream_t stream);
+// Split precision: writes the FP32 pre-test records (SphFilt, n + SPH_FILTER_PAD of them) of n leaf-order SphRecs.
+hipError_t launch_sph_filter_records(const void *recs, uint32_t n, void *out, hipStream_t stream);
 
 // Enqueues the traversal of one geometry type over n rays on `stream` (asynchronous).
 hipError_t launch_trace(GeomType type, const BvhView &bvh, const RaysView &rays, const HitsView &hits, uint64_t n,

@@ -205,6 +205,44 @@ __device__ __forceinline__ bool tri_filter(const TriFilt &T, const float D[3], c
   return true;
 }
 
+// Conservative FP32 pre-test of one sphere for the split path: returns false only when the FP64 test isect_sph
+// (blazert/primitives/sphere.h:364-406) reports no hit for the ray at the current hit distance h <= h32.
+//
+// The FP64 test computes, from v = o - c and the unit direction d, A = d.d, B = 2 v.d, C = v.v - r r and
+// D = B B - 4 A C, and accepts a root t = (-B -+ sqrt(D)) / 2A with tmin < t < h when D >= 0. With exact arithmetic
+// on its inputs D = 4 (A r^2 - |v x d|^2); its roundings move D by at most 1e-13 (P^2 + r^2), where
+// P >= |c_i| + |o_i| (P = (max |c32_i| + om) (1 + 4u), u = 2^-24). So
+//   (a) D < 0 whenever |v x d| > r + 1.61e-7 (P + r), and
+//   (b) every root lies within r + 1.61e-7 (P + r) of -v.d.
+// Here v, v x d (one FMA per component) and v.d are computed in FP32 from the rounded center, origin and direction:
+// each component of v is within 2.01 u P of the exact one, v x d within 12 u P (as a vector) and v.d within 10 u P.
+// The record's radius is |r| rounded up. The test rejects when |v x d| in FP32 exceeds the radius plus both error
+// terms (compared as squares, each rounding covered by a factor), or when the interval of (b), widened by the error
+// of -v.d and the rounding of its ends, lies below tmin32 <= tmin or at or above h32 >= h (s.h carries the FP64 unit
+// direction's norm, within 4 DBL_EPSILON of 1: covered by the 1 + 4u factor). A radius term below 1e-18, whose
+// square would fall in the FP32 denormal range, skips test (a). Scenes and origins above FILTER_MAX_EXTENT are not
+// pre-tested, and every rejection is a comparison that is false for NaN operands.
+__device__ __forceinline__ bool sph_filter(const SphFilt &S, const float D[3], const float O[3], float om,
+                                           float tmin32, float h32) {
+  const float u = 5.9604644775390625e-08f;
+  const float cm = fmaxf(fabsf(S.c[0]), fmaxf(fabsf(S.c[1]), fabsf(S.c[2])));
+  const float P = (cm + om) * (1.0f + 4.0f * u);
+  const float v0 = O[0] - S.c[0], v1 = O[1] - S.c[1], v2 = O[2] - S.c[2];
+  const float wx = __builtin_fmaf(v1, D[2], -(v2 * D[1]));
+  const float wy = __builtin_fmaf(v2, D[0], -(v0 * D[2]));
+  const float wz = __builtin_fmaf(v0, D[1], -(v1 * D[0]));
+  const float dist2 = __builtin_fmaf(wx, wx, __builtin_fmaf(wy, wy, wz * wz));
+  const float e64 = __builtin_fmaf(1.61e-7f, P + S.r, 1e-30f);  // the FP64 test's own rounding, as a distance
+  const float R = (S.r + __builtin_fmaf(12.0f * u, P, e64)) * (1.0f + 4.0f * u);
+  if (R >= 1e-18f && dist2 * (1.0f - 8.0f * u) > (R * R) * (1.0f + 4.0f * u)) return false;
+  const float tc = -__builtin_fmaf(v0, D[0], __builtin_fmaf(v1, D[1], v2 * D[2]));
+  float rho = S.r + __builtin_fmaf(10.0f * u, P, e64);
+  rho = __builtin_fmaf(4.0f * u, fabsf(tc) + rho, rho);
+  if (tc + rho < tmin32) return false;
+  if (tc - rho >= h32 * (1.0f + 4.0f * u)) return false;
+  return true;
+}
+
 // ------------------------------------------------------------------------------------------------ intersectors
 // Closest-hit state of one traversal (prepare_traversal: hit = tmax, uv = 0, prim invalid).
 template <class R>
@@ -737,8 +775,9 @@ __device__ __forceinline__ void trav_begin(const BvhView &bvh, const RayT<R> &r,
 
 // One unit of the DFS: an internal node (test both children, push far, descend near) or a leaf's next primitives
 // (exact_group per type, in order, blazert/bvh/intersect.h:66-78), followed by pops until a node passes or the stack
-// is empty. On the split path with triangle pre-test records, a leaf step is either a pre-test of the next
-// FILTER_GROUP triangles in FP32 (tri_filter) or the FP64 test of the next one that passed it, in leaf order.
+// is empty. On the split path with triangle or sphere pre-test records, a leaf step is either a pre-test of the
+// next FILTER_GROUP triangles or SPH_FILTER_GROUP spheres in FP32 (tri_filter, sph_filter) or the FP64 test of the
+// next one that passed it, in leaf order.
 template <class Rec, bool SPLIT>
 __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typename Rec::real> &r, const Ray32 &q,
                                           bool any_hit, HitT<typename Rec::real> &s, Trav &t,
@@ -752,17 +791,25 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
   // records have in common are loaded from a per-lane address in one set of instructions; the rest of the longer
   // ones follow in short branches (from the same lines, already on their way), and the compute branches below run
   // without waiting in between.
-  constexpr bool FILTER = SPLIT && Rec::kind == TRIANGLES;
+  // split precision, triangles and spheres: FP32 pre-test records, FG of them per step
+  constexpr bool FILTER = SPLIT && (Rec::kind == TRIANGLES || Rec::kind == SPHERES);
+  using Filt = typename std::conditional<Rec::kind == TRIANGLES, TriFilt, SphFilt>::type;
+  constexpr uint32_t FG = Rec::kind == TRIANGLES ? FILTER_GROUP : SPH_FILTER_GROUP;
   constexpr int NODE_Q = SPLIT ? 8 : (sizeof(R) == 8 ? 7 : 4);  // 16-byte chunks of the record used (Node64: 112 B)
   constexpr int REC_Q = (int)(sizeof(Rec) / 16);
-  constexpr int FILT_Q = FILTER ? (int)(FILTER_GROUP * sizeof(TriFilt) / 16) : 0;
+  constexpr int FILT_Q = FILTER ? (int)(FG * sizeof(Filt) / 16) : 0;
   // exact paths: primitives tested per leaf step
   constexpr int PG = SPLIT ? 1 : exact_group<Rec::kind>();
   constexpr int QMAX0 = NODE_Q < PG * REC_Q ? PG * REC_Q : NODE_Q;
   constexpr int QMAX = QMAX0 < FILT_Q ? FILT_Q : QMAX0;
   static_assert(sizeof(Rec) % 16 == 0 && sizeof(Node) >= 16 * NODE_Q, "record chunking");
   const bool at_leaf = (t.cur & LEAF_BIT) != 0;
-  const bool filt = FILTER && bvh.tri_filter != nullptr;  // the same for every lane of a launch
+  const Filt *frecs = nullptr;
+  if constexpr (FILTER) {
+    if constexpr (Rec::kind == TRIANGLES) frecs = bvh.tri_filter;
+    else frecs = bvh.sph_filter;
+  }
+  const bool filt = FILTER && frecs != nullptr;  // the same for every lane of a launch
   const bool filter_step = filt && at_leaf && t.cand == 0u && t.pos < t.cnt;
   const bool fetch_rec = at_leaf && t.pos < t.cnt && !filter_step;
   const uint32_t p = (t.cur & ~LEAF_BIT) + t.pos + (filt ? (uint32_t)__builtin_ctz(t.cand | 0x80000000u) : 0u);
@@ -775,7 +822,7 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
   using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
   using GlobalChunk = const __attribute__((address_space(1))) u32x4;
   GlobalChunk *src = !at_leaf ? (GlobalChunk *)(nodes + t.cur)
-                     : filter_step ? (GlobalChunk *)(bvh.tri_filter + ((t.cur & ~LEAF_BIT) + t.pos))
+                     : filter_step ? (GlobalChunk *)(frecs + ((t.cur & ~LEAF_BIT) + t.pos))
                                    : (GlobalChunk *)(reinterpret_cast<const Rec *>(bvh.prims) + p);
   u32x4 chunk[QMAX];
   if constexpr (FILTER) {
@@ -871,19 +918,23 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
     }
   } else if (filter_step) {
     if constexpr (FILTER) {
-      // pre-test the group; members past the leaf's end are padding or the next leaf's triangles
-      TriFilt g[FILTER_GROUP];
+      // pre-test the group; members past the leaf's end are padding or the next leaf's primitives
+      Filt g[FG];
       __builtin_memcpy(g, chunk, sizeof g);
       const float D[3] = {(float)r.d[0], (float)r.d[1], (float)r.d[2]};
       const float O[3] = {(float)r.o[0], (float)r.o[1], (float)r.o[2]};
       const uint32_t left = t.cnt - t.pos;
       uint32_t mask = 0;
 #pragma unroll
-      for (uint32_t j = 0; j < FILTER_GROUP; j++)
-        if (j < left && tri_filter(g[j], D, O, q.om, q.tmin, t.h32)) mask |= 1u << j;
+      for (uint32_t j = 0; j < FG; j++) {
+        bool pass;
+        if constexpr (Rec::kind == TRIANGLES) pass = tri_filter(g[j], D, O, q.om, q.tmin, t.h32);
+        else pass = sph_filter(g[j], D, O, q.om, q.tmin, t.h32);
+        if (j < left && pass) mask |= 1u << j;
+      }
       t.cand = mask;
       if (mask) return;
-      t.pos += FILTER_GROUP;
+      t.pos += FG;
       if (t.pos < t.cnt) return;
     }
   } else {
@@ -913,7 +964,7 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
       if (filt) {
         t.cand &= t.cand - 1u;
         if (t.cand) return;
-        t.pos += FILTER_GROUP;
+        t.pos += FG;
         if (t.pos < t.cnt) return;
       } else if (++t.pos < t.cnt) {
         return;
@@ -1214,7 +1265,6 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
   }
 }
 
-// Clears a launch's counter group (a fill kernel, not hipMemsetAsync, so that graph captures replay correctly).
 // Split-precision pre-test records of the triangles in leaf order (TriFilt), from their FP64 records; the FILTER_PAD
 // records after the last one are zero.
 __global__ void k_tri_filter_records(const TriRec *__restrict__ recs, uint32_t n, TriFilt *out, uint32_t total) {
@@ -1246,6 +1296,22 @@ __global__ void k_tri_filter_records(const TriRec *__restrict__ recs, uint32_t n
   }
 }
 
+// Split-precision pre-test records of the spheres in leaf order (SphFilt): the center rounded to nearest and |r|
+// rounded up; the SPH_FILTER_PAD records after the last one are zero.
+__global__ void k_sph_filter_records(const SphRec *__restrict__ recs, uint32_t n, SphFilt *out, uint32_t total) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    SphFilt f{};
+    if (i < n) {
+      const SphRec &S = recs[i];
+      for (int k = 0; k < 3; k++) f.c[k] = (float)S.c[k];
+      f.r = f32_up(fabs(S.r));
+    }
+    out[i] = f;
+  }
+}
+
+// Clears a launch's counter group (a fill kernel, not hipMemsetAsync, so that graph captures replay correctly).
 __global__ void k_zero_counters(unsigned long long *c) {
 #pragma unroll
   for (int k = 0; k < COUNTER_GROUP; k++) c[k] = 0ull;
@@ -1402,6 +1468,15 @@ hipError_t launch_trace(GeomType type, const BvhView &bvh, const RaysView &rays,
   case PLANES: return launch_typed<PlaRec>(bvh, rays, hits, n, cfg, split, cfg.overwrite ? 1u : 0u, stream);
   default: return launch_typed<CylRec>(bvh, rays, hits, n, cfg, split, cfg.overwrite ? 1u : 0u, stream);
   }
+}
+
+hipError_t launch_sph_filter_records(const void *recs, uint32_t n, void *out, hipStream_t stream) {
+  const uint32_t total = n + SPH_FILTER_PAD;
+  uint32_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_sph_filter_records, dim3(blocks), dim3(256), 0, stream, static_cast<const SphRec *>(recs), n,
+                     static_cast<SphFilt *>(out), total);
+  return hipGetLastError();
 }
 
 hipError_t launch_tri_filter_records(const void *recs, uint32_t n, void *out, hipStream_t stream) {

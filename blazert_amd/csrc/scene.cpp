@@ -946,23 +946,30 @@ int device_error(hipError_t e, const char *what) {
               std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Split precision: the FP32 pre-test records of a triangle geometry (TriFilt), made on the device from its uploaded
-// FP64 records and waited for; none when the scene's extent is above FILTER_MAX_EXTENT.
-// BLAZERT_AMD_SPLIT_PRETEST=0 in the environment at upload time leaves them out (every split-precision triangle test
-// then runs in FP64; the hits are the same, which the tests check).
-hipError_t make_tri_filter(DeviceScene &d, BvhView &v, double extent, hipStream_t st, uint64_t *device_bytes) {
+// Split precision: the FP32 pre-test records of a triangle (TriFilt) or sphere (SphFilt) geometry, made on the
+// device from its uploaded FP64 records and waited for; none for other types, or when the scene's extent is above
+// FILTER_MAX_EXTENT. BLAZERT_AMD_SPLIT_PRETEST=0 in the environment at upload time leaves them out (every
+// split-precision primitive test then runs in FP64; the hits are the same, which the tests check).
+hipError_t make_prim_filter(DeviceScene &d, BvhView &v, uint32_t type, double extent, hipStream_t st,
+                            uint64_t *device_bytes) {
   v.tri_filter = nullptr;
+  v.sph_filter = nullptr;
+  if (type != TRIANGLES && type != SPHERES) return hipSuccess;
   if (!(extent <= FILTER_MAX_EXTENT) || v.n_prims == 0 || !v.prims) return hipSuccess;
   const char *env = std::getenv("BLAZERT_AMD_SPLIT_PRETEST");
   if (env && env[0] == '0') return hipSuccess;
+  const size_t bytes = type == TRIANGLES ? (size_t)(v.n_prims + FILTER_PAD) * sizeof(TriFilt)
+                                         : (size_t)(v.n_prims + SPH_FILTER_PAD) * sizeof(SphFilt);
   void *p = nullptr;
-  hipError_t e = hipMalloc(&p, (size_t)(v.n_prims + FILTER_PAD) * sizeof(TriFilt));
+  hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) return e;
   d.allocs.push_back(p);
-  e = launch_tri_filter_records(v.prims, v.n_prims, p, st);
+  e = type == TRIANGLES ? launch_tri_filter_records(v.prims, v.n_prims, p, st)
+                        : launch_sph_filter_records(v.prims, v.n_prims, p, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  v.tri_filter = static_cast<const TriFilt *>(p);
-  *device_bytes += (uint64_t)(v.n_prims + FILTER_PAD) * sizeof(TriFilt);
+  if (type == TRIANGLES) v.tri_filter = static_cast<const TriFilt *>(p);
+  else v.sph_filter = static_cast<const SphFilt *>(p);
+  *device_bytes += bytes;
   return e;
 }
 
@@ -980,8 +987,8 @@ int put_geometry(DeviceScene &d, Geometry &g, uint32_t t) {
     if (e == hipSuccess) e = d.put(g.faces, &v.faces);
     if (e == hipSuccess) e = d.put(g.vnormals, &v.vnormals);
     if (e == hipSuccess) e = d.put(g.vertices, &v.vertices);
-    if (e == hipSuccess) e = make_tri_filter(d, v, g.extent, nullptr, &g.stats.device_bytes);
   }
+  if (e == hipSuccess) e = make_prim_filter(d, v, t, g.extent, nullptr, &g.stats.device_bytes);
   if (e != hipSuccess) return device_error(e, "upload failed");
   d.views[t] = v;
   return BRT_OK;
@@ -1137,9 +1144,8 @@ int commit_on_device(brt_scene *s, hipStream_t st) {
     bs.device_bytes = (uint64_t)r.num_n64 * sizeof(Node64) + (uint64_t)r.num_n4 * sizeof(Node4) +
                       (uint64_t)g.boxes.n * (r.rec_size + 4) + g.faces.size() * 4 + g.vnormals.size() * 8 +
                       g.vertices.size() * 8;
-    if (t == TRIANGLES)
-      if (hipError_t fe = make_tri_filter(*d, v, g.extent, st, &bs.device_bytes))
-        return device_error(fe, "triangle pre-test records");
+    if (hipError_t fe = make_prim_filter(*d, v, t, g.extent, st, &bs.device_bytes))
+      return device_error(fe, "pre-test records");
     d->views[t] = v;
     bs.build_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }

@@ -1,8 +1,9 @@
-"""The split path's FP32 triangle pre-test (tri_filter in blazert_amd/csrc/kernels.hip) only skips FP64 triangle
-tests that would reject: with and without the pre-test records (BLAZERT_AMD_SPLIT_PRETEST=0 at upload) the
-split-precision hits are identical bit for bit, on every ray. The rays are made to sit on the pre-test's error
-bound: aimed at vertices and edge points (u, v or w within rounding of 0), grazing the triangles' planes (det near
-0), with tmin / tmax windows ending at the hit, at scene scales from 1e-6 to 1e11."""
+"""The split path's FP32 triangle and sphere pre-tests (tri_filter and sph_filter in blazert_amd/csrc/kernels.hip)
+only skip FP64 tests that would reject: with and without the pre-test records (BLAZERT_AMD_SPLIT_PRETEST=0 at upload)
+the split-precision hits are identical bit for bit, on every ray. The rays are made to sit on the pre-tests' error
+bounds: aimed at triangle vertices and edge points (u, v or w within rounding of 0), grazing the triangles' planes
+(det near 0), grazing spheres within rounding of their radius, with tmin / tmax windows ending at the hit, at scene
+scales from 1e-6 to 1e11 and far from the origin."""
 import os
 
 import numpy as np
@@ -98,6 +99,76 @@ def test_pretest_never_changes_a_hit(dev, scale):
     tmin = np.where(np.arange(len(m)) % 2 == 0, 0.0, th * (1 - 1e-14))
     a2 = trace(with_pt, o2, d2, dev, tmin=tmin, tmax=tmax)
     b2 = trace(without, o2, d2, dev, tmin=tmin, tmax=tmax)
+    for k in ("t", "prim_id"):
+        assert np.array_equal(a2[k].view(np.uint64 if k == "t" else np.uint32),
+                              b2[k].view(np.uint64 if k == "t" else np.uint32)), k
+    with_pt.close()
+    without.close()
+
+
+def sphere_scene(c, r, pretest):
+    old = os.environ.get("BLAZERT_AMD_SPLIT_PRETEST")
+    os.environ["BLAZERT_AMD_SPLIT_PRETEST"] = "1" if pretest else "0"
+    try:
+        s = Scene()
+        s.add_spheres(c, r)
+        s.commit()
+        s.upload()
+    finally:
+        if old is None:
+            del os.environ["BLAZERT_AMD_SPLIT_PRETEST"]
+        else:
+            os.environ["BLAZERT_AMD_SPLIT_PRETEST"] = old
+    return s
+
+
+def tangent_rays(c, r, n, seed, scale):
+    """Rays whose line passes the center of a random sphere at a distance within rounding of its radius (grazing
+    the sphere from either side), rays from inside a sphere, and rays aimed at a sphere's center."""
+    g = np.random.default_rng(seed)
+    k = g.integers(len(c), size=n)
+    d = g.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    side = g.normal(size=(n, 3))
+    side -= (side * d).sum(1, keepdims=True) * d
+    side /= np.linalg.norm(side, axis=1, keepdims=True)
+    kind = g.integers(3, size=n)
+    rel = 1.0 + np.where(g.random(n) < 0.5, -1.0, 1.0) * 10.0 ** g.uniform(-16, -5, n)
+    off = np.select([kind == 0, kind == 1], [r[k] * rel, r[k] * g.random(n) * 0.9], np.zeros(n))
+    through = c[k] + side * off[:, None]
+    back = np.where(kind == 1, 0.0, g.uniform(0.5, 20.0, n) * scale)  # kind 1 starts inside the sphere
+    return through - d * back[:, None], d
+
+
+@pytest.mark.parametrize("scale,shift", [(1.0, 0.0), (1e-5, 0.0), (1.0, 3e3), (1e3, 1e8)])
+def test_sphere_pretest_never_changes_a_hit(dev, scale, shift):
+    """The split path's FP32 sphere pre-test (sph_filter) only skips FP64 sphere tests that report no hit: split
+    hits with and without the pre-test records are identical bit for bit, on rays that graze spheres within rounding
+    of their radius, start inside them or pass their centers, at several scales and far from the origin."""
+    g = np.random.default_rng(int(np.log10(scale) + 30) + int(shift > 0))
+    m = 20000
+    c = (g.uniform(-10, 10, size=(m, 3)) * scale + shift).astype(np.float64)
+    r = (10.0 ** g.uniform(-3, 0, m) * scale).astype(np.float64)
+    with_pt, without = sphere_scene(c, r, True), sphere_scene(c, r, False)
+    n = 1 << 17
+    org, dirs = tangent_rays(c, r, n, 7, scale)
+    inc_o, inc_d = W.incoherent_rays(n, 9, 12.0)
+    org = np.concatenate([org, inc_o * scale + shift])
+    dirs = np.concatenate([dirs, inc_d])
+    a, b = trace(with_pt, org, dirs, dev), trace(without, org, dirs, dev)
+    for k in ("t", "u", "v", "normal"):
+        assert np.array_equal(a[k].view(np.uint64), b[k].view(np.uint64)), k
+    for k in ("prim_id", "geom_id"):
+        assert np.array_equal(a[k], b[k]), k
+    assert (a["prim_id"] != -1).sum() > n // 2
+    # windows that end at the hit
+    t = b["t"]
+    hit = np.nonzero(b["prim_id"] != -1)[0][:n // 2]
+    th = t[hit]
+    tmax = th * np.array([1 + 1e-15, 1.0, 1 - 1e-15])[np.arange(len(hit)) % 3]
+    tmin = np.where(np.arange(len(hit)) % 2 == 0, 0.0, th * (1 - 1e-14))
+    a2 = trace(with_pt, org[hit], dirs[hit], dev, tmin=tmin, tmax=tmax)
+    b2 = trace(without, org[hit], dirs[hit], dev, tmin=tmin, tmax=tmax)
     for k in ("t", "prim_id"):
         assert np.array_equal(a2[k].view(np.uint64 if k == "t" else np.uint32),
                               b2[k].view(np.uint64 if k == "t" else np.uint32)), k
