@@ -21,7 +21,7 @@ constexpr uint32_t BLOCKS[] = {64, 128, 192, 256};
 constexpr uint32_t REFILLS[] = {4, 8, 16, 32};
 constexpr uint32_t RINGS[] = {4, 8, 16};
 constexpr uint32_t SORTS[] = {BRT_SORT_NONE, BRT_SORT_OCTANT_MORTON};
-constexpr uint32_t POOLS[] = {64};  // tried with the best shape found (the default is 128)
+constexpr uint32_t POOLS[] = {32, 128};  // tried with the best shape found (the default is 64)
 
 // Mean time of `repeats` enqueues of `run` on `st` after one untimed warm-up; < 0 with *rc set on failure.
 double time_ms(const std::function<int()> &run, hipStream_t st, uint32_t repeats, int *rc) {
@@ -97,7 +97,7 @@ int brt_autotune_launch(const brt_scene *scene, const brt_rays *rays, const brt_
             r.launch = c;
           }
         }
-  // then the ray-pool claim size with the best shape (image-order rays balance better in 64-ray claims)
+  // then the ray-pool claim size with the best shape
   for (uint32_t pool : POOLS) {
     brt_launch c = r.launch;
     c.pool_size = pool;

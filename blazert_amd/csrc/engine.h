@@ -43,10 +43,14 @@ __host__ __device__ inline int collapse4(const brt_tree_node *T, uint32_t b, uin
 // Device counters of one traversal launch, in a group cleared by the launch itself: the persistent schedule's ray
 // pool; a flag a split-precision launch sets when it defers rays to the exact path; the ray pool of the FP64 launch
 // that traces them. Launches on one stream run in order, so one group per stream serves all of them.
+// The ray pools take claims from CLAIM_COUNTERS counters, each on its own 128-byte line (COUNTER_STRIDE slots apart):
+// slot k * COUNTER_STRIDE + COUNTER_POOL for the launch, + COUNTER_DEFERRED_POOL for its deferred-ray launch.
 constexpr int COUNTER_POOL = 0;
 constexpr int COUNTER_DEFERRED = 1;
 constexpr int COUNTER_DEFERRED_POOL = 2;
-constexpr int COUNTER_GROUP = 4;  // slots per group (one spare)
+constexpr int CLAIM_COUNTERS = 8;
+constexpr int COUNTER_STRIDE = 16;
+constexpr int COUNTER_GROUP = CLAIM_COUNTERS * COUNTER_STRIDE;  // slots per group (1 KiB)
 
 constexpr uint32_t SCHEDULE_PERSISTENT = 0;  // persistent waves with per-lane ray refill (default)
 constexpr uint32_t SCHEDULE_PER_THREAD = 1;  // one ray per thread, grid-stride
@@ -61,7 +65,7 @@ struct TraceLaunch {
   unsigned long long *counter;  // COUNTER_GROUP device counters for this launch (cleared on the stream)
   const uint32_t *order;        // optional ray permutation (sorted launches); kernels trace rays[order[k]]
   uint32_t ring = 0;            // LDS stack slots per lane: 4, 8 or 16 (0 = default of the precision)
-  uint32_t pool = 0;            // persistent: indices per claim away from the end, 64 or 128 (0 = 128)
+  uint32_t pool = 0;            // persistent: ray indices per claim, 32, 64 or 128 (0 = 64)
   bool overwrite = false;       // every hit replaces the record (traverse semantics), not only closer ones
 };
 

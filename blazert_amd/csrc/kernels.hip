@@ -1079,43 +1079,54 @@ __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysViewT<typename R
 
 // Persistent waves: every lane owns one ray at a time; when at least `refill` lanes of a wave are idle, they start
 // the next indices of the wave's pool. Skewed per-ray work (many rays miss the root box, a few visit hundreds of
-// nodes) then no longer idles the wave until its slowest ray ends. The pool is a range of indices claimed from the
-// launch's counter with one atomic: every claim on that one address serialises across the GPU, and claiming per
-// refill made the counter a bottleneck at small refill thresholds. A claim takes `big` indices (POOL_BIG, or 64 when
-// the launch asks for it: image-order rays balance better in smaller claims) while the wave's own previous claim
-// ended more than two big pools per wave before n, and POOL_CHUNK (or the number of idle lanes, if larger) after
-// that. The switch reads the wave's own last claim, not the counter, so a wave that lags behind the others can still
-// make a big claim near the end: the last claims are mostly, not always, small. Every
-// lane leaves the loop once the counter has passed n, the pool is empty and its own ray is done, so the grid
+// nodes) then no longer idles the wave until its slowest ray ends. The pool is a chunk of `chunk` consecutive indices
+// (POOL_CHUNK unless the launch asks for 32 or 128) claimed with one atomic add. Claims on one address serialise
+// across the GPU (about 14 ns each), which bounded short launches and image-order rays, so they are spread over
+// CLAIM_COUNTERS counters on separate 128-byte lines: counter k hands out chunks k, k + CLAIM_COUNTERS,
+// k + 2 CLAIM_COUNTERS, ... A wave starts at the counter of its block (block index mod CLAIM_COUNTERS, which spreads
+// the blocks of every XCD over all counters) and moves on to the next counter once one has passed n. When all have,
+// the pool is exhausted; every lane leaves the loop once its pool is exhausted and its own ray is done, so the grid
 // always drains.
 constexpr uint32_t POOL_CHUNK = 64;
-constexpr uint32_t POOL_BIG = 128;
 
 // Wave-uniform pool of claimed, not yet started ray indices [next, end).
 struct WavePool {
   uint64_t next = 0, end = 0;
-  bool last = false;  // the claim that reached n has been made
+  bool last = false;  // every counter has passed n
 };
 
 // Called by every lane of a wave with the same pool state when the wave refills. An idle lane handed an index gets
-// it in `idx` and `got`; `exhausted` is set once nothing is left to hand out.
-__device__ __forceinline__ void pool_refill(WavePool &P, unsigned long long *counter, uint64_t n, uint32_t big,
+// it in `idx` and `got`; `exhausted` is set once nothing is left to hand out. `counter` is the first of the
+// launch's CLAIM_COUNTERS counters (COUNTER_STRIDE slots apart).
+__device__ __forceinline__ void pool_refill(WavePool &P, unsigned long long *counter, uint64_t n, uint32_t chunk,
                                             unsigned long long idle, uint32_t nidle, uint32_t lane, bool &got,
                                             uint64_t &idx, bool &exhausted) {
   got = false;
   if (P.next == P.end && !P.last) {
-    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64u);
-    const uint32_t chunk = (P.end + 2ull * big * waves < n) ? big : POOL_CHUNK;
-    const uint32_t want = nidle > chunk ? nidle : chunk;
-    unsigned long long base = 0;
+    unsigned long long base = n;
     const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
-    if (lane == leader) base = atomicAdd(counter, (unsigned long long)want);
+    if (lane == leader) {
+      const uint32_t home = blockIdx.x % (uint32_t)CLAIM_COUNTERS;
+      for (uint32_t a = 0; a < (uint32_t)CLAIM_COUNTERS; a++) {
+        const uint32_t k = (home + a) % (uint32_t)CLAIM_COUNTERS;
+        const unsigned long long c = atomicAdd(counter + k * COUNTER_STRIDE, 1ull);
+        const unsigned long long start = (c * CLAIM_COUNTERS + k) * chunk;
+        if (start < n) {
+          base = start;
+          break;
+        }
+      }
+    }
     const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)base, (int)leader);
     const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(base >> 32), (int)leader);
     base = ((unsigned long long)hi << 32) | lo;
-    P.next = base < n ? base : n;
-    P.end = base + want < n ? base + want : n;
-    if (base + want >= n) P.last = true;
+    if (base >= n) {
+      P.last = true;
+      P.next = P.end = n;
+    } else {
+      P.next = base;
+      P.end = base + chunk < n ? base + chunk : n;
+    }
   }
   const uint64_t avail = P.end - P.next;
   const uint32_t take = avail < nidle ? (uint32_t)avail : nidle;
@@ -1137,7 +1148,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WA
 k_trace_persistent(BvhView bvh, RaysViewT<typename Rec::real> rays, HitsViewT<typename Rec::real> hits, uint64_t n,
                    uint32_t flags, uint32_t first, uint32_t is_tri, double S, unsigned long long *counter,
                    uint32_t refill, const uint32_t *order, uint32_t ring, unsigned long long *deferred,
-                   uint32_t pool_big) {
+                   uint32_t chunk) {
   using R = typename Rec::real;
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
@@ -1163,7 +1174,7 @@ k_trace_persistent(BvhView bvh, RaysViewT<typename Rec::real> rays, HitsViewT<ty
       if (nidle >= refill || nidle == 64u) {
         bool got;
         uint64_t cand = 0;
-        pool_refill(pool, counter, n, pool_big, idle, nidle, lane, got, cand, exhausted);
+        pool_refill(pool, counter, n, chunk, idle, nidle, lane, got, cand, exhausted);
         if (got) {
           idx = order ? (uint64_t)order[cand] : cand;
           start_ray<SPLIT>(bvh, rays, idx, S, r, q, s);
@@ -1208,7 +1219,7 @@ template <class Rec, bool SPLIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BRT_WAVES_PER_EU)))
 k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total, uint32_t flags,
               unsigned long long *counter, uint32_t refill, uint32_t ring, unsigned long long *deferred,
-              uint32_t pool_big) {
+              uint32_t chunk) {
   extern __shared__ unsigned char smem[];
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
   const bool deferred_only = !SPLIT && (flags & FLAG_DEFERRED_ONLY);
@@ -1235,7 +1246,7 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
       if (nidle >= refill || nidle == 64u) {
         bool got;
         uint64_t g = 0;
-        pool_refill(pool, counter, total, pool_big, idle, nidle, lane, got, g, exhausted);
+        pool_refill(pool, counter, total, chunk, idle, nidle, lane, got, g, exhausted);
         if (got) {
           j = find_job(jobs, njobs, g);
           const BatchJob &J = jobs[j];
@@ -1313,8 +1324,7 @@ __global__ void k_sph_filter_records(const SphRec *__restrict__ recs, uint32_t n
 
 // Clears a launch's counter group (a fill kernel, not hipMemsetAsync, so that graph captures replay correctly).
 __global__ void k_zero_counters(unsigned long long *c) {
-#pragma unroll
-  for (int k = 0; k < COUNTER_GROUP; k++) c[k] = 0ull;
+  for (uint32_t k = threadIdx.x; k < (uint32_t)COUNTER_GROUP; k += blockDim.x) c[k] = 0ull;
 }
 
 // 3 x 9-bit Morton interleave
@@ -1393,7 +1403,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real>
   unsigned long long *grp = cfg.counter;
   if (!grp) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, stream, grp);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, grp);
   if (cfg.schedule == SCHEDULE_PERSISTENT) {
     uint64_t blocks = 0;
     hipError_t e = persistent_blocks(k_trace_persistent<Rec, SPLIT>, block, lds, n, blocks);
@@ -1401,7 +1411,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real>
     const uint32_t refill = cfg.refill ? cfg.refill : refill_default<Rec, SPLIT>();
     hipLaunchKernelGGL((k_trace_persistent<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, bvh, rays,
                        hits, n, cfg.flags, first, is_tri, sp.scene_extent, grp + COUNTER_POOL, refill, cfg.order,
-                       ring, grp + COUNTER_DEFERRED, cfg.pool ? cfg.pool : POOL_BIG);
+                       ring, grp + COUNTER_DEFERRED, cfg.pool ? cfg.pool : POOL_CHUNK);
   } else {
     uint64_t blocks = (n + block - 1) / block;
     if (blocks > (1ull << 30)) blocks = 1ull << 30;
@@ -1419,7 +1429,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real>
   hipLaunchKernelGGL((k_trace_persistent<Rec, false>), dim3((uint32_t)blocks), dim3(256), lds64, stream, bvh, rays,
                      hits, n, cfg.flags | FLAG_DEFERRED_ONLY, first, is_tri, sp.scene_extent,
                      grp + COUNTER_DEFERRED_POOL, refill_default<Rec, false>(), cfg.order, ring64,
-                     grp + COUNTER_DEFERRED, POOL_BIG);
+                     grp + COUNTER_DEFERRED, POOL_CHUNK);
   return hipGetLastError();
 }
 
@@ -1436,9 +1446,9 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
   hipError_t e = persistent_blocks(k_trace_batch<Rec, SPLIT>, block, lds, total, blocks);
   if (e != hipSuccess) return e;
   const uint32_t refill = cfg.refill ? cfg.refill : refill_default<Rec, SPLIT>();
-  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, stream, grp);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, grp);
   hipLaunchKernelGGL((k_trace_batch<Rec, SPLIT>), dim3((uint32_t)blocks), dim3(block), lds, stream, jobs, njobs, total,
-                     cfg.flags, grp + COUNTER_POOL, refill, ring, grp + COUNTER_DEFERRED, cfg.pool ? cfg.pool : POOL_BIG);
+                     cfg.flags, grp + COUNTER_POOL, refill, ring, grp + COUNTER_DEFERRED, cfg.pool ? cfg.pool : POOL_CHUNK);
   e = hipGetLastError();
   if (e != hipSuccess || !SPLIT) return e;
   const uint32_t ring64 = ring_default<false>();
@@ -1447,7 +1457,7 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_trace_batch<Rec, false>), dim3((uint32_t)blocks), dim3(256), lds64, stream, jobs, njobs, total,
                      cfg.flags | FLAG_DEFERRED_ONLY, grp + COUNTER_DEFERRED_POOL, refill_default<Rec, false>(),
-                     ring64, grp + COUNTER_DEFERRED, POOL_BIG);
+                     ring64, grp + COUNTER_DEFERRED, POOL_CHUNK);
   return hipGetLastError();
 }
 

@@ -863,8 +863,8 @@ int check_launch(const brt_launch &L) {
     return fail(BRT_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
   if (L.stack_ring && L.stack_ring != 4 && L.stack_ring != 8 && L.stack_ring != 16)
     return fail(BRT_ERR_INVALID_ARGUMENT, "stack_ring must be 0, 4, 8 or 16");
-  if (L.pool_size && L.pool_size != 64 && L.pool_size != 128)
-    return fail(BRT_ERR_INVALID_ARGUMENT, "pool_size must be 0, 64 or 128");
+  if (L.pool_size && L.pool_size != 32 && L.pool_size != 64 && L.pool_size != 128)
+    return fail(BRT_ERR_INVALID_ARGUMENT, "pool_size must be 0, 32, 64 or 128");
   if (L.flags & ~(BRT_FLAG_HITS_INOUT | BRT_FLAG_ANY_HIT)) return fail(BRT_ERR_INVALID_ARGUMENT, "unknown launch flags");
   return BRT_OK;
 }
@@ -1610,7 +1610,8 @@ static int intersect_impl(brt_scene *s, const brt_rays *rays, const brt_hits *hi
     SortWorkspace *ws = &fixed;
     hipError_t e = hipSuccess;
     if (noalloc) {
-      e = sort_workspace_in(fixed, scratch + 256, NOALLOC_SCRATCH_BYTES - 256, n);
+      constexpr size_t cbytes = COUNTER_GROUP * sizeof(unsigned long long);  // the launch counters come first
+      e = sort_workspace_in(fixed, scratch + cbytes, NOALLOC_SCRATCH_BYTES - cbytes, n);
       if (e == hipErrorInvalidValue) return fail(BRT_ERR_UNSUPPORTED, "sort workspace does not fit the scratch");
     } else {
       ws = s->dev->sort_workspace(st);

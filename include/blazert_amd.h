@@ -118,9 +118,9 @@ typedef struct brt_launch {
   uint32_t sort;       /* BRT_SORT_*: trace rays in a sorted order (results are written in input order) */
   uint32_t stack_ring; /* traversal-stack entries per lane kept in LDS: 4, 8 or 16; 0 = 8 (FP64) or 16 (split).
                           Deeper entries spill to private memory; a deeper ring spills less but takes more LDS */
-  uint32_t pool_size;  /* persistent: ray indices a wave claims at once away from the end of the batch: 64 or 128;
-                          0 = 128. Smaller claims balance image-order (coherent) rays better, larger ones cost fewer
-                          atomics on the launch's counter; results are the same */
+  uint32_t pool_size;  /* persistent: ray indices a wave claims at once: 32, 64 or 128; 0 = 64. Smaller claims
+                          balance image-order (coherent) rays better, larger ones cost fewer atomics on the launch's
+                          counters; results are the same */
 } brt_launch;
 
 /* One ray / hit in host memory, for the single-ray drop-in (RayHit<T>, blazert/ray.h:72-85). */

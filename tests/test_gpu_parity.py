@@ -183,6 +183,26 @@ def test_launch_configurations_identical(dev, ico8, schedule, refill, block, rin
     assert_bit_exact(g32, base)  # the split path is deterministic per ray, whatever the launch shape
 
 
+@pytest.mark.parametrize("pool", [0, 32, 128])
+def test_ragged_ray_counts_claim_every_ray_once(dev, ico8, pool):
+    """Ray counts around the claim boundaries: fewer rays than one claim per counter (so some of the eight claim
+    counters never hand out a chunk), one past and one short of chunk multiples, and a last chunk of one ray.
+    Every ray must be traced exactly once: same bits as the oracle on every ray."""
+    p, o = ico8
+    org, dirs = W.icosphere_incoherent_rays(8 * 128 * 3 + 1, seed=23)
+    ref = o.intersect_batch(org, dirs, trace=True)
+    ok = ref["trace"]["overflow"] == 0
+    for n in (1, 31, 33, 65, 257, 511, 513, 8 * 32 + 1, 8 * 64 - 1, 8 * 128 + 1, 8 * 128 * 3 + 1):
+        g = gpu_intersect(p, org[:n], dirs[:n], dev, pool_size=pool, normals=False)
+        assert_bit_exact(g, {k: ref[k][:n] for k in ("t", "u", "v", "prim_id", "geom_id")}, ok[:n], normals=False)
+        g = gpu_intersect(p, org[:n], dirs[:n], dev, precision=1, pool_size=pool, normals=False)
+        h = gpu_intersect(p, org[:n], dirs[:n], dev, precision=1, schedule=1, normals=False)
+        assert_bit_exact(g, h, normals=False)
+    from blazert_amd.capi import BlazertError
+    with pytest.raises(BlazertError, match="pool_size"):
+        gpu_intersect(p, org[:64], dirs[:64], dev, pool_size=96)
+
+
 def test_icosphere8_coherent_subset(dev, ico8):
     p, o = ico8
     org, dirs = W.icosphere_coherent_rays(640, 360)
@@ -193,8 +213,8 @@ def test_icosphere8_coherent_subset(dev, ico8):
 
 
 def test_icosphere8_full_persistent_equals_per_thread(dev, ico8):
-    """All 4,194,304 incoherent rays, every field: the persistent schedule (wave-level ray pools of 128 and 64
-    indices, lane refill) against one ray per thread, at both precisions. A ray skipped or traced twice by the pool
+    """All 4,194,304 incoherent rays, every field: the persistent schedule (wave-level ray pools claimed from eight
+    counters, lane refill) against one ray per thread, at both precisions. A ray skipped or traced twice by the pool
     claims would show here."""
     p, _ = ico8
     org, dirs = W.icosphere_incoherent_rays()
@@ -202,9 +222,10 @@ def test_icosphere8_full_persistent_equals_per_thread(dev, ico8):
         a = gpu_intersect(p, org, dirs, dev, precision=precision)
         b = gpu_intersect(p, org, dirs, dev, precision=precision, schedule=1)
         assert_bit_exact(a, b)
-        # 64-index claims (the pool_size launch option) hand out the same rays
-        c = gpu_intersect(p, org, dirs, dev, precision=precision, pool_size=64)
-        assert_bit_exact(a, c)
+        # 32- and 128-index claims (the pool_size launch option) hand out the same rays
+        for pool in (32, 128):
+            c = gpu_intersect(p, org, dirs, dev, precision=precision, pool_size=pool)
+            assert_bit_exact(a, c)
 
 
 def test_icosphere8_full_incoherent_properties(dev, ico8):
