@@ -244,7 +244,9 @@ int brt_traverse(const brt_scene *scene, uint32_t geom_type, const brt_rays *ray
  * one persistent launch over the concatenated rays of all items, with each hit merged into its own item exactly as
  * brt_intersect would. The batch keeps the device pointers: they must stay valid while it is launched.
  * brt_batch_capture records the launch sequence on `stream` (a non-default stream) into a hipGraph owned by the
- * batch; brt_batch_replay launches that graph. No allocation happens after brt_batch_create. */
+ * batch; brt_batch_replay launches that graph. No allocation happens after brt_batch_create. A batch owns one set
+ * of launch counters and writes the same hits buffers every time, so its launches and replays must not overlap:
+ * issue them on one stream, or order them with events. */
 typedef struct brt_batch brt_batch;
 
 typedef struct brt_batch_item {
