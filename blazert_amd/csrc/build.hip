@@ -809,7 +809,7 @@ __global__ void k_tri_recs(const uint32_t *__restrict__ order, const double *__r
     r.e1[k] = v0[k] - v1[k];
     r.e2[k] = v2[k] - v0[k];
   }
-  r.pad = 0.0;
+  r.prim = p;
   out[i] = r;
 }
 

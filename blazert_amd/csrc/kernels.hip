@@ -249,8 +249,14 @@ template <class R>
 struct HitT {
   R h;
   R u, v;
-  uint32_t leaf_idx;  // leaf-order index of the current hit
+  uint32_t leaf_idx;  // leaf-order index of the current hit; the user primitive id for records that carry it
 };
+
+// FP64 triangle records carry their user primitive id (TriRec::prim).
+template <class Rec>
+__device__ constexpr bool id_in_record() {
+  return std::is_same<Rec, TriRec>::value;
+}
 using Hit = HitT<double>;
 
 // Every intersector runs in the scene type of its record (Rec::real): the same operation sequence for double and
@@ -291,7 +297,8 @@ __device__ __forceinline__ bool isect_tri(const Rec &T, const RayT<typename Rec:
       s.h = t * inv_det;
       s.u = u * inv_det;
       s.v = v * inv_det;
-      s.leaf_idx = idx;
+      if constexpr (id_in_record<Rec>()) s.leaf_idx = (uint32_t)T.prim;
+      else s.leaf_idx = idx;
       return true;
     }
   }
@@ -1003,7 +1010,7 @@ __device__ __forceinline__ void finish_ray(const BvhView &bvh, const HitsViewT<t
     else write = s.h < ((first && !(flags & FLAG_INOUT)) ? Real<R>::max : gmem(hits.t)[i]);
   }
   if (write) {
-    const uint32_t pid = gmem(bvh.prim_ids)[s.leaf_idx];
+    const uint32_t pid = id_in_record<Rec>() ? s.leaf_idx : gmem(bvh.prim_ids)[s.leaf_idx];
     if (hits.merged) st_stream(hits.merged, i, 1u);
     st_stream(hits.t, i, s.h);
     if (hits.u) st_stream(hits.u, i, s.u);

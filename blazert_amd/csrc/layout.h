@@ -38,9 +38,10 @@ struct alignas(128) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 
-// Triangle record in leaf order: v0, e1 = v0 - v1, e2 = v2 - v0 (blazert/primitives/trimesh.h:26-27). The FP64
-// record does not carry n = cross(e1, e2): the 16 more bytes per record cost more time (measured) than forming n
-// in every test, as the reference does.
+// Triangle record in leaf order: v0, e1 = v0 - v1, e2 = v2 - v0 (blazert/primitives/trimesh.h:26-27) and the user
+// primitive id (prim_ids at this leaf position), which a hit takes from the record it has just read instead of a
+// dependent lookup when the ray finishes. The FP64 record does not carry n = cross(e1, e2): the 16 more bytes per
+// record cost more time (measured) than forming n in every test, as the reference does.
 struct alignas(16) TriRec {
   using real = double;
   static constexpr uint32_t kind = TRIANGLES;
@@ -48,7 +49,7 @@ struct alignas(16) TriRec {
   double v0[3];
   double e1[3];
   double e2[3];
-  double pad;
+  uint64_t prim;
 };
 static_assert(sizeof(TriRec) == 80, "TriRec size");
 

@@ -609,6 +609,7 @@ void flatten(Geometry &g, GeomType type) {
         r[i].e1[k] = v0[k] - v1[k];  // Triangle::c = a - b
         r[i].e2[k] = v2[k] - v0[k];  // Triangle::b = c - a
       }
+      r[i].prim = p;
     }
     break;
   }
