@@ -1299,19 +1299,31 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
         bool got;
         uint64_t g = 0;
         pool_refill(pool, counter, total, chunk, idle, nidle, lane, got, g, exhausted);
-        if (got) {
-          j = find_job(jobs, njobs, g);
-          const BatchJob &J = jobs[j];
-          const uint64_t local = g - J.offset;
-          idx = J.order ? (uint64_t)J.order[local] : local;
-          bv = J.bvh;
-          start_ray<SPLIT>(bv, J.rays, idx, J.extent, r, q, s);
-          if (SPLIT && deferred_to_exact(r)) {
-            __atomic_store_n(deferred, 1ull, __ATOMIC_RELAXED);
-          } else if (!deferred_only || deferred_to_exact(r)) {
-            has_ray = true;
-            trav_begin<SPLIT>(bv, r, q, s, t);
-            st.reset();
+        // A claim is a run of consecutive indices and usually lies in one job: find the jobs of its first and last
+        // index once per wave (wave-uniform, so scalar loads), and search per lane only when they differ.
+        const unsigned long long gm = __ballot(got);
+        if (gm) {
+          const int lo = __ffsll((long long)gm) - 1, hi = 63 - __clzll((long long)gm);
+          const uint64_t g_lo = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(g >> 32), lo) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)g, lo);
+          const uint64_t g_hi = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(g >> 32), hi) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)g, hi);
+          const uint32_t j_lo = find_job(jobs, njobs, g_lo);
+          const bool one_job = j_lo + 1u >= njobs || jobs[j_lo + 1u].offset > g_hi;
+          if (got) {
+            j = one_job ? j_lo : find_job(jobs, njobs, g);
+            const BatchJob &J = jobs[j];
+            const uint64_t local = g - J.offset;
+            idx = J.order ? (uint64_t)J.order[local] : local;
+            bv = J.bvh;
+            start_ray<SPLIT>(bv, J.rays, idx, J.extent, r, q, s);
+            if (SPLIT && deferred_to_exact(r)) {
+              __atomic_store_n(deferred, 1ull, __ATOMIC_RELAXED);
+            } else if (!deferred_only || deferred_to_exact(r)) {
+              has_ray = true;
+              trav_begin<SPLIT>(bv, r, q, s, t);
+              st.reset();
+            }
           }
         }
       }
