@@ -1257,6 +1257,8 @@ k_trace_persistent(BvhView bvh, RaysViewT<typename Rec::real> rays, HitsViewT<ty
 // BVH of this geometry type + its own rays and hits). A lane finds its job by binary search over the prefix offsets
 // and reads that job's BVH view from the table (a few cached loads per step), so waves freely mix scenes and the
 // uneven batch drains with no per-scene tail.
+constexpr uint32_t BATCH_LDS_JOBS = 32;
+
 __device__ __forceinline__ uint32_t find_job(const BatchJob *jobs, uint32_t njobs, uint64_t g) {
   uint32_t lo = 0, hi = njobs - 1;
   while (lo < hi) {
@@ -1276,6 +1278,18 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
   const bool any_hit = (flags & FLAG_ANY_HIT) != 0;
   const bool deferred_only = !SPLIT && (flags & FLAG_DEFERRED_ONLY);
   if (deferred_only && __atomic_load_n(deferred, __ATOMIC_RELAXED) == 0ull) return;
+  // Finishing a ray needs its job's hit columns and flags: for batches of up to BATCH_LDS_JOBS jobs every block
+  // keeps them in LDS, so that a finish does not wait on a load from the job table.
+  constexpr int JW = (int)(sizeof(HitsView) / 8) + 1;  // hit column pointers, then first | is_tri << 32
+  __shared__ uint64_t job_fin[BATCH_LDS_JOBS * JW];
+  const bool fin_lds = njobs <= BATCH_LDS_JOBS;
+  if (fin_lds) {
+    for (uint32_t k = threadIdx.x; k < njobs; k += blockDim.x) {
+      lds_store(job_fin + k * JW, jobs[k].hits);
+      job_fin[k * JW + JW - 1] = (uint64_t)jobs[k].first | ((uint64_t)jobs[k].is_tri << 32);
+    }
+    __syncthreads();
+  }
   const uint32_t lane = __lane_id();
   SpillArea<SPLIT> spill;
   RingStack<SPLIT> st;
@@ -1332,8 +1346,20 @@ k_trace_batch(const BatchJob *__restrict__ jobs, uint32_t njobs, uint64_t total,
     if (has_ray) {
       if (t.active) trav_step<Rec, SPLIT>(bv, r, q, any_hit, s, t, st);
       if (!t.active) {
-        const BatchJob &J = jobs[j];
-        finish_ray<Rec>(bv, J.hits, idx, r, s, flags, J.first, J.is_tri);
+        HitsView hv;
+        uint32_t jfirst, jtri;
+        if (fin_lds) {
+          __builtin_memcpy(&hv, job_fin + j * JW, sizeof(HitsView));
+          const uint64_t w = job_fin[j * JW + JW - 1];
+          jfirst = (uint32_t)w;
+          jtri = (uint32_t)(w >> 32);
+        } else {
+          const BatchJob &J = jobs[j];
+          hv = J.hits;
+          jfirst = J.first;
+          jtri = J.is_tri;
+        }
+        finish_ray<Rec>(bv, hv, idx, r, s, flags, jfirst, jtri);
         has_ray = false;
       }
     }

@@ -514,6 +514,34 @@ def test_batch_graph_replays(dev, batch_set):
         _batch_check(b, batch_set, 1)
 
 
+@pytest.mark.parametrize("njobs", [32, 33])
+def test_batch_many_jobs_equals_single_launches(dev, njobs):
+    """Batches of 32 jobs (the kernel keeps each job's hit columns in LDS) and 33 (it reads them from the job
+    table): every field of every ray equals the scene's own intersect, at both precisions, and rays of the
+    spheres that follow the triangles merge into the triangle hits."""
+    from blazert_amd import Batch, Scene
+    items = []
+    for i in range(njobs):
+        v, f = W.icosphere(2, seed=i)
+        s = Scene()
+        s.add_mesh(v * (1.0 + 0.1 * (i % 5)), f)
+        if i % 3 == 0:
+            c, r = W.sphere_field(50, seed=i)
+            s.add_spheres(c * 0.1, r * 0.1)
+        s.commit()
+        org, dirs = W.incoherent_rays(300 + 97 * i, seed=200 + i, box=3.0)
+        items.append((s, torch.from_numpy(org).to(dev), torch.from_numpy(dirs).to(dev)))
+    for precision in (0, 1):
+        b = Batch(items, precision=precision)
+        b.launch()
+        torch.cuda.synchronize()
+        for (s, O, D), h in zip(items, b.hits):
+            ref = s.intersect(O, D, precision=precision)
+            torch.cuda.synchronize()
+            for k in ("t", "u", "v", "prim_id", "geom_id"):
+                assert torch.equal(h[k], ref[k]), (precision, k)
+
+
 def test_low_level_traverse_per_type(dev, mixed, oracle_mod):
     """brt_traverse (blazert/bvh/accel.h:60-120) on each geometry type: hit rays get the type's post_traversal
     record bit for bit, missed rays keep the record passed in."""
