@@ -198,8 +198,8 @@ def adjoint_leg(args, dev) -> dict:
     """Differentiable 3-bounce mirror tracing: the 1,310,720-triangle icosphere inside a 12-triangle box (one merged
     mesh), 4,194,304 rays, loss = sum over bounces of w * t with w ~ U(0,1). Times the forward (3 traced bounces) and
     the FP64 adjoint (dL/dvertex, dL/dorigin, dL/ddirection), and checks the adjoint against the oracle's analytic
-    gradient on a ray sample and against central differences (h = 1e-6) on 64 vertices (face ids fixed) and on
-    64 rays (re-traced)."""
+    gradient on a ray sample and against central differences on 64 vertices (face ids fixed, h = 3e-8) and on
+    64 rays (re-traced, h = 1e-6), relative to each gradient entry."""
     import torch
 
     from blazert_amd import Scene, workloads as W
@@ -268,18 +268,15 @@ def adjoint_leg(args, dev) -> dict:
     an_err = max(float(np.abs(a.cpu().numpy() - b).max() / np.abs(b).max())
                  for a, b in ((gvs, rv), (gos, ro), (gds, rd)))
 
-    # central differences on 64 vertices with the face ids fixed (only rays touching the vertex change). Plain
-    # central differences at h = 1e-6 carry a truncation error of order h^2 times the third derivative, visible on
-    # rays with a grazing bounce; the Richardson combination of h and h/2 removes that term and is reported beside it.
-    h = 1e-6
+    # central differences on 64 vertices with the face ids fixed (only rays touching the vertex change), relative
+    # to each entry. The step is h = 3e-8: at 1e-6 the truncation error (h^2 times the third derivative) reaches
+    # 1e-4 relative on vertices whose gradient nearly cancels, and below 1e-8 the rounding of the losses dominates.
+    h_v = 3e-8
     rng = np.random.default_rng(5)
     valid = pid != OR.INVALID
     touched = np.unique(F[pid[valid]].reshape(-1))
     verts = np.concatenate([rng.choice(touched[touched < len(V) - 8], 60, replace=False), len(V) - 1 - np.arange(4)])
-    scale = 1e-3 * np.abs(gv_h).max()
-    fd_err, fd_rich_err, fd_over, fd_rich_over = 0.0, 0.0, 0, 0
-    fd_unexplained = 0  # above the bar by more than the estimated h^2 truncation error
-    trunc_above = []
+    fd_err, fd_big_err, fd_over = 0.0, 0.0, 0
 
     def central(vi, k, hh, idx):
         Vp, Vm = V.copy(), V.copy()
@@ -295,29 +292,22 @@ def adjoint_leg(args, dev) -> dict:
         fmask = np.zeros(len(F) + 1, bool)
         fmask[faces_v] = True
         idx = np.nonzero(fmask[np.where(valid, pid, len(F))].any(axis=0))[0]
-        d1 = central(vi, k, h, idx)
-        d2 = central(vi, k, h / 2, idx)
-        den = max(abs(gv_h[vi, k]), scale)
-        e1 = abs(d1 - gv_h[vi, k]) / den
-        # central differences err by c h^2 + O(h^4): d(h) - d(h/2) = (3/4) c h^2 estimates the truncation at h
-        trunc = (4.0 / 3.0) * abs(d1 - d2) / den
-        er = abs((4 * d2 - d1) / 3 - gv_h[vi, k]) / den
-        if e1 > 1e-6:
-            fd_over += 1
-            trunc_above.append(trunc)
-            fd_unexplained += int(e1 - trunc > 1e-6)
-        fd_err = max(fd_err, e1)
-        fd_rich_err = max(fd_rich_err, er)
-        fd_rich_over += int(er > 1e-6)
+        g = gv_h[vi, k]
+        den = abs(g) if g != 0 else 1.0
+        e = abs(central(vi, k, h_v, idx) - g) / den
+        fd_err = max(fd_err, e)
+        fd_over += int(e > 1e-6)
+        fd_big_err = max(fd_big_err, abs(central(vi, k, 1e-6, idx) - g) / den)
 
     # central differences on 64 rays, every perturbed ray re-traced on the GPU
+    h_r = 1e-6
     ridx = np.arange(0, n, n // 64)[:64]
     pert_o, pert_d = [], []
     for which in (0, 1):
         for k in range(3):
             for sgn in (1.0, -1.0):
                 a, b = org[ridx].copy(), dirs[ridx].copy()
-                (a if which == 0 else b)[:, k] += sgn * h
+                (a if which == 0 else b)[:, k] += sgn * h_r
                 pert_o.append(a)
                 pert_d.append(b)
     po = torch.from_numpy(np.concatenate(pert_o)).to(dev)
@@ -332,10 +322,10 @@ def adjoint_leg(args, dev) -> dict:
         which, k = divmod(j, 3)
         same = (pp[:, 2 * j] == pid[:, ridx]).all(0) & (pp[:, 2 * j + 1] == pid[:, ridx]).all(0)
         flipped += int((~same).sum())
-        fd = (L[2 * j] - L[2 * j + 1]) / (2 * h)
+        fd = (L[2 * j] - L[2 * j + 1]) / (2 * h_r)
         g = (go_h if which == 0 else gd_h)[ridx, k]
         for r in np.nonzero(same)[0]:
-            ray_err = max(ray_err, abs(fd[r] - g[r]) / max(abs(g[r]), 1e-3))
+            ray_err = max(ray_err, abs(fd[r] - g[r]) / (abs(g[r]) if g[r] != 0 else 1.0))
 
     hits = int(valid.sum())
     # algorithmic bytes: per ray 48 B of rays, 12 B of face ids, 24 B of dL/dt and 48 B of ray gradients; per bounce
@@ -362,14 +352,11 @@ def adjoint_leg(args, dev) -> dict:
             "commit_s": round(build_s, 3),
             "forward_bit_exact_vs_oracle": fwd_exact, "forward_sample_rays": ns,
             "max_rel_err_vs_analytic_cpu": an_err, "analytic_sample_rays": na,
-            "fd_vertices_max_rel_err": fd_err, "fd_vertices": 64, "fd_vertices_above_1e-6": fd_over,
-            "fd_bar": {"h": h, "bar": 1e-6, "relative_to": "max(|g|, 1e-3 * max|dL/dvertex|)",
-                       "central_met": fd_over == 0,
-                       "central_misses_truncation_estimates": [float(x) for x in trunc_above],
-                       "central_misses_beyond_truncation": fd_unexplained,
-                       "richardson_h_h2_max_rel_err": fd_rich_err, "richardson_met": fd_rich_over == 0},
-            "fd_vertices_richardson_max_rel_err": fd_rich_err,
-            "fd_rays_max_rel_err": ray_err, "fd_rays": 64, "fd_ray_perturbations_flipped": flipped}
+            "fd_vertices_max_rel_err": float(fd_err), "fd_vertices": 64, "fd_vertices_above_1e-6": fd_over,
+            "fd_bar": {"bar": 1e-6, "relative_to": "|g| of each entry", "h_vertices": h_v, "h_rays": h_r,
+                       "met": bool(fd_over == 0 and ray_err <= 1e-6),
+                       "vertices_max_rel_err_at_h_1e-6": float(fd_big_err)},
+            "fd_rays_max_rel_err": float(ray_err), "fd_rays": 64, "fd_ray_perturbations_flipped": flipped}
 
 
 def device_commit_leg(v, f, host_scene, host_s) -> dict:
