@@ -1090,10 +1090,10 @@ __global__ void __launch_bounds__(256) k_trace(BvhView bvh, RaysViewT<typename R
 // (POOL_CHUNK unless the launch asks for 32 or 128) claimed with one atomic add. Claims on one address serialise
 // across the GPU (about 14 ns each), which bounded short launches and image-order rays, so they are spread over
 // CLAIM_COUNTERS counters on separate 128-byte lines: counter k hands out chunks k, k + CLAIM_COUNTERS,
-// k + 2 CLAIM_COUNTERS, ... A wave starts at the counter of its block (block index mod CLAIM_COUNTERS, which spreads
-// the blocks of every XCD over all counters) and moves on to the next counter once one has passed n. When all have,
-// the pool is exhausted; every lane leaves the loop once its pool is exhausted and its own ray is done, so the grid
-// always drains.
+// k + 2 CLAIM_COUNTERS, ... A wave starts at the counter of its block (block index mod CLAIM_COUNTERS: blocks are
+// dealt to the eight XCDs in turn, so each XCD's waves start at a counter of their own) and moves on to the next
+// counter once one has passed n. When all have, the pool is exhausted; every lane leaves the loop once its pool is
+// exhausted and its own ray is done, so the grid always drains.
 constexpr uint32_t POOL_CHUNK = 64;
 
 // Wave-uniform pool of claimed, not yet started ray indices [next, end).
