@@ -111,7 +111,7 @@ typedef struct brt_hits {
 typedef struct brt_launch {
   uint32_t precision;  /* BRT_PRECISION_* */
   uint32_t flags;      /* BRT_FLAG_* */
-  uint32_t block_size; /* threads per block: 64, 128, 192 or 256; 0 = library default (256) */
+  uint32_t block_size; /* threads per block: 64, 128, 192 or 256; 0 = library default (256; 128 for FP64 spheres) */
   uint32_t schedule;   /* BRT_SCHEDULE_* */
   uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default (32 for
                           triangles on the exact paths, 16 otherwise) */
