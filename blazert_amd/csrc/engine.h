@@ -122,6 +122,8 @@ struct SplitParams {
 hipError_t launch_tri_filter_records(const void *recs, uint32_t n, void *out, hipStream_t stream);
 // Split precision: writes the FP32 pre-test records (SphFilt, n + SPH_FILTER_PAD of them) of n leaf-order SphRecs.
 hipError_t launch_sph_filter_records(const void *recs, uint32_t n, void *out, hipStream_t stream);
+// Split precision: writes the bounding-sphere pre-test records (SphFilt, n + SPH_FILTER_PAD) of n leaf-order CylRecs.
+hipError_t launch_cyl_filter_records(const void *recs, uint32_t n, void *out, hipStream_t stream);
 
 // Enqueues the traversal of one geometry type over n rays on `stream` (asynchronous).
 hipError_t launch_trace(GeomType type, const BvhView &bvh, const RaysView &rays, const HitsView &hits, uint64_t n,

@@ -803,8 +803,8 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
   // records have in common are loaded from a per-lane address in one set of instructions; the rest of the longer
   // ones follow in short branches (from the same lines, already on their way), and the compute branches below run
   // without waiting in between.
-  // split precision, triangles and spheres: FP32 pre-test records, FG of them per step
-  constexpr bool FILTER = SPLIT && (Rec::kind == TRIANGLES || Rec::kind == SPHERES);
+  // split precision, triangles, spheres and cylinders: FP32 pre-test records, FG of them per step
+  constexpr bool FILTER = SPLIT && (Rec::kind == TRIANGLES || Rec::kind == SPHERES || Rec::kind == CYLINDERS);
   using Filt = typename std::conditional<Rec::kind == TRIANGLES, TriFilt, SphFilt>::type;
   constexpr uint32_t FG = Rec::kind == TRIANGLES ? FILTER_GROUP : SPH_FILTER_GROUP;
   constexpr int NODE_Q = SPLIT ? 8 : (sizeof(R) == 8 ? 7 : 4);  // 16-byte chunks of the record used (Node64: 112 B)
@@ -941,7 +941,8 @@ __device__ __forceinline__ void trav_step(const BvhView &bvh, const RayT<typenam
       for (uint32_t j = 0; j < FG; j++) {
         bool pass;
         if constexpr (Rec::kind == TRIANGLES) pass = tri_filter(g[j], D, O, q.om, q.tmin, t.h32);
-        else pass = sph_filter(g[j], D, O, q.om, q.tmin, t.h32);
+        else if constexpr (Rec::kind == SPHERES) pass = sph_filter(g[j], D, O, q.om, q.tmin, t.h32);
+        else pass = sph_filter(g[j], D, O, q.om, 0.0f, __builtin_inff());  // cylinders: any hit at t >= 0 counts
         if (j < left && pass) mask |= 1u << j;
       }
       t.cand = mask;
@@ -1417,6 +1418,37 @@ __global__ void k_sph_filter_records(const SphRec *__restrict__ recs, uint32_t n
   }
 }
 
+// Split-precision pre-test records of the cylinders: each one's bounding sphere, for sph_filter with the window
+// [0, inf). The cylinder test (cylinder.h:181-495) accepts any hit at t >= 0, with no distance check, at a point of
+// the local frame org + t dir = R^T (o + t d - c) with |z| <= h/2 and, up to the rounding of the test, on or inside
+// the elliptic shell of semi-axes a, b; for an orthonormal R that point lies within sqrt(max(a,b)^2 + (h/2)^2) of c.
+// The radius is enlarged by 2^-10 of itself and 2^-20 of the coordinate magnitude, far above any rounding of the
+// FP64 test, and is infinite (never rejects) when R R^T differs from the identity by more than 1e-6 or an input is
+// not finite.
+__global__ void k_cyl_filter_records(const CylRec *__restrict__ recs, uint32_t n, SphFilt *out, uint32_t total) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    SphFilt f{};
+    if (i < n) {
+      const CylRec &C = recs[i];
+      double dev = 0.0;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          const double d = C.R[3 * a] * C.R[3 * b] + C.R[3 * a + 1] * C.R[3 * b + 1] + C.R[3 * a + 2] * C.R[3 * b + 2];
+          dev = fmax(dev, fabs(d - (a == b ? 1.0 : 0.0)));
+        }
+      const double ab = fmax(fabs(C.a), fabs(C.b)), hh = 0.5 * fabs(C.h);
+      const double rb = sqrt(ab * ab + hh * hh);
+      const double cm = fmax(fabs(C.c[0]), fmax(fabs(C.c[1]), fabs(C.c[2])));
+      double rad = rb * (1.0 + 0x1p-10) + 0x1p-20 * (cm + rb);
+      if (!(dev <= 1e-6) || !(rad < 1e300)) rad = __builtin_inf();
+      for (int k = 0; k < 3; k++) f.c[k] = (float)C.c[k];
+      f.r = f32_up(rad);
+    }
+    out[i] = f;
+  }
+}
+
 // Clears a launch's counter group (a fill kernel, not hipMemsetAsync, so that graph captures replay correctly).
 __global__ void k_zero_counters(unsigned long long *c) {
   for (uint32_t k = threadIdx.x; k < (uint32_t)COUNTER_GROUP; k += blockDim.x) c[k] = 0ull;
@@ -1580,6 +1612,15 @@ hipError_t launch_sph_filter_records(const void *recs, uint32_t n, void *out, hi
   uint32_t blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_sph_filter_records, dim3(blocks), dim3(256), 0, stream, static_cast<const SphRec *>(recs), n,
+                     static_cast<SphFilt *>(out), total);
+  return hipGetLastError();
+}
+
+hipError_t launch_cyl_filter_records(const void *recs, uint32_t n, void *out, hipStream_t stream) {
+  const uint32_t total = n + SPH_FILTER_PAD;
+  uint32_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_cyl_filter_records, dim3(blocks), dim3(256), 0, stream, static_cast<const CylRec *>(recs), n,
                      static_cast<SphFilt *>(out), total);
   return hipGetLastError();
 }

@@ -73,8 +73,9 @@ constexpr uint32_t FILTER_GROUP = BRT_FILTER_GROUP;  // triangles pre-tested per
 constexpr uint32_t FILTER_PAD = FILTER_GROUP - 1;
 
 // Split precision: FP32 pre-test record of a sphere, in leaf order beside the SphRec array: the center rounded to
-// nearest and |r| rounded up. Built on the device from the SphRec array; SPH_FILTER_PAD zero records follow the last
-// one, so a group of SPH_FILTER_GROUP records read from any leaf start stays in bounds.
+// nearest and |r| rounded up; for a cylinder, its bounding sphere, enlarged (k_cyl_filter_records). Built on the
+// device from the SphRec / CylRec array; SPH_FILTER_PAD zero records follow the last one, so a group of
+// SPH_FILTER_GROUP records read from any leaf start stays in bounds.
 struct alignas(16) SphFilt {
   float c[3];
   float r;
@@ -173,7 +174,7 @@ struct BvhView {
   const double *vertices;     // triangles only: user vertices (3 per vertex), for mirror tracing and its adjoint
   const Node32 *nodes32;      // Scene<float>: FP32 nodes (nodes64 / nodes4 unused)
   const TriFilt *tri_filter;  // split precision, triangles: FP32 pre-test records (NULL: every test in FP64)
-  const SphFilt *sph_filter;  // split precision, spheres: FP32 pre-test records (NULL: every test in FP64)
+  const SphFilt *sph_filter;  // split precision, spheres and cylinders: FP32 (bounding) spheres (NULL: all in FP64)
   const float *vnormals32;    // Scene<float> triangles: vertex normals
   double root_min[3], root_max[3];
   float root_min32[3], root_max32[3];

@@ -955,7 +955,7 @@ hipError_t make_prim_filter(DeviceScene &d, BvhView &v, uint32_t type, double ex
                             uint64_t *device_bytes) {
   v.tri_filter = nullptr;
   v.sph_filter = nullptr;
-  if (type != TRIANGLES && type != SPHERES) return hipSuccess;
+  if (type != TRIANGLES && type != SPHERES && type != CYLINDERS) return hipSuccess;
   if (!(extent <= FILTER_MAX_EXTENT) || v.n_prims == 0 || !v.prims) return hipSuccess;
   const char *env = std::getenv("BLAZERT_AMD_SPLIT_PRETEST");
   if (env && env[0] == '0') return hipSuccess;
@@ -966,7 +966,8 @@ hipError_t make_prim_filter(DeviceScene &d, BvhView &v, uint32_t type, double ex
   if (e != hipSuccess) return e;
   d.allocs.push_back(p);
   e = type == TRIANGLES ? launch_tri_filter_records(v.prims, v.n_prims, p, st)
-                        : launch_sph_filter_records(v.prims, v.n_prims, p, st);
+      : type == SPHERES  ? launch_sph_filter_records(v.prims, v.n_prims, p, st)
+                         : launch_cyl_filter_records(v.prims, v.n_prims, p, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (type == TRIANGLES) v.tri_filter = static_cast<const TriFilt *>(p);
   else v.sph_filter = static_cast<const SphFilt *>(p);

@@ -174,3 +174,75 @@ def test_sphere_pretest_never_changes_a_hit(dev, scale, shift):
                               b2[k].view(np.uint64 if k == "t" else np.uint32)), k
     with_pt.close()
     without.close()
+
+
+def cylinder_scene(c, a, b, h, rot, pretest):
+    old = os.environ.get("BLAZERT_AMD_SPLIT_PRETEST")
+    os.environ["BLAZERT_AMD_SPLIT_PRETEST"] = "1" if pretest else "0"
+    try:
+        s = Scene()
+        s.add_cylinders(c, a, b, h, rot)
+        s.commit()
+        s.upload()
+    finally:
+        if old is None:
+            del os.environ["BLAZERT_AMD_SPLIT_PRETEST"]
+        else:
+            os.environ["BLAZERT_AMD_SPLIT_PRETEST"] = old
+    return s
+
+
+def cylinder_rays(c, a, b, h, rot, n, seed, scale):
+    """Rays that graze a random cylinder's shell (through a surface point, along a tangent of its ellipse), run
+    parallel to its axis through a point of the rim, cross a cap's rim, or start inside it. Local points are mapped
+    to the world as c + R p, the frame of blazert's cylinder test (cylinder.h:181-495)."""
+    g = np.random.default_rng(seed)
+    k = g.integers(len(c), size=n)
+    th = g.uniform(0, 2 * np.pi, n)
+    z = g.uniform(-0.5, 0.5, n) * h[k]
+    rim = np.stack([a[k] * np.cos(th), b[k] * np.sin(th), z], 1)
+    tangent = np.stack([-a[k] * np.sin(th), b[k] * np.cos(th), g.normal(size=n) * 1e-3], 1)
+    kind = g.integers(4, size=n)
+    p = rim.copy()
+    p[kind == 2, 2] = np.where(g.random((kind == 2).sum()) < 0.5, 0.5, -0.5) * h[k][kind == 2]
+    p[kind == 3] *= g.uniform(0, 0.9, ((kind == 3).sum(), 1))
+    d = np.select([kind[:, None] == 0, kind[:, None] == 1], [tangent, np.array([0.0, 0.0, 1.0])],
+                  g.normal(size=(n, 3)))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    R = rot.reshape(-1, 3, 3)[k]
+    pw = c[k] + np.einsum("nij,nj->ni", R, p)
+    dw = np.einsum("nij,nj->ni", R, d)
+    dw /= np.linalg.norm(dw, axis=1, keepdims=True)
+    back = np.where(kind == 3, 0.0, g.uniform(0.5, 20.0, n) * scale)
+    return pw - dw * back[:, None], dw
+
+
+@pytest.mark.parametrize("scale,shift", [(1.0, 0.0), (1e-5, 0.0), (1e3, 1e8)])
+def test_cylinder_pretest_never_changes_a_hit(dev, scale, shift):
+    """The split path's bounding-sphere pre-test of cylinders only skips FP64 cylinder tests that report no hit: split
+    hits with and without the pre-test records are identical bit for bit, on rays grazing the shell, parallel to the
+    axis through the rim, crossing cap rims and starting inside, for orthonormal rotations and for rotations that
+    are not (whose cylinders the pre-test never rejects)."""
+    g = np.random.default_rng(int(np.log10(scale) + 40) + int(shift > 0))
+    m = 5000
+    c = (g.uniform(-10, 10, size=(m, 3)) * scale + shift).astype(np.float64)
+    a = 10.0 ** g.uniform(-2, 0, m) * scale
+    b = 10.0 ** g.uniform(-2, 0, m) * scale
+    h = 10.0 ** g.uniform(-1, 0.5, m) * scale
+    q, _ = np.linalg.qr(g.normal(size=(m, 3, 3)))
+    q[m // 10:m // 5] *= 1.01  # not orthonormal
+    rot = np.ascontiguousarray(q.reshape(m, 9))
+    with_pt, without = cylinder_scene(c, a, b, h, rot, True), cylinder_scene(c, a, b, h, rot, False)
+    n = 1 << 17
+    org, dirs = cylinder_rays(c, a, b, h, rot, n, 11, scale)
+    inc_o, inc_d = W.incoherent_rays(n, 13, 12.0)
+    org = np.concatenate([org, inc_o * scale + shift])
+    dirs = np.concatenate([dirs, inc_d])
+    x, y = trace(with_pt, org, dirs, dev), trace(without, org, dirs, dev)
+    for k in ("t", "u", "v", "normal"):
+        assert np.array_equal(x[k].view(np.uint64), y[k].view(np.uint64)), k
+    for k in ("prim_id", "geom_id"):
+        assert np.array_equal(x[k], y[k]), k
+    assert (x["prim_id"] != -1).sum() > n // 4
+    with_pt.close()
+    without.close()
