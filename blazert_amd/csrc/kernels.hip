@@ -639,10 +639,12 @@ struct Trav {
 // spread over all banks); older entries spill to private memory only when a ray goes deeper than the ring, and come
 // back when the ring drains. Invariant: entries [lo, sp) are in LDS, [0, lo) in the private spill area.
 // The ring depth is a launch parameter (a power of two, 4..16): deeper rings spill less but take more LDS per block.
-// Defaults (measured on the 1.31M-triangle incoherent set): 8 for FP64 keys; 16 for the FP32 path, whose 4-wide
-// steps push up to three entries at a time.
-template <bool SPLIT>
-constexpr uint32_t ring_default() { return SPLIT ? 16u : 8u; }
+// Defaults: 8 for FP64 keys; 16 for the FP32 path over triangles, whose 4-wide steps push up to three entries at a
+// time (1.31M-triangle incoherent set); 8 for the FP32 path over the other types, where the fourth wave per SIMD
+// that a shallower ring leaves room for pays more than the spills it causes (mixed scene, 8M rays: spheres 3.87 ->
+// 3.35 ms, cylinders 4.97 -> 4.32, planes 1.66 -> 1.47 with 8 instead of 16).
+template <class Rec, bool SPLIT>
+constexpr uint32_t ring_default() { return (SPLIT && Rec::kind == TRIANGLES) ? 16u : 8u; }
 // Default refill threshold (idle lanes of a wave before it takes new rays). Exact path: 32 for triangles (1.31M-
 // triangle incoherent set: 16 is 3% slower, 40 5% slower), 16 for the other types, whose longer primitive tests
 // favour earlier refills (mixed scene: spheres 3% and cylinders 5% faster than at 32). 4-wide split path: 16 for
@@ -1525,7 +1527,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real>
   using R = typename Rec::real;
   const uint32_t block = cfg.block ? cfg.block : block_default<Rec, SPLIT>();
   const uint32_t first = cfg.first ? 1u : 0u;
-  const uint32_t ring = cfg.ring ? cfg.ring : ring_default<SPLIT>();
+  const uint32_t ring = cfg.ring ? cfg.ring : ring_default<Rec, SPLIT>();
   const size_t lds = ring_lds_bytes<SPLIT, R>(block, ring);
   unsigned long long *grp = cfg.counter;
   if (!grp) return hipErrorInvalidValue;
@@ -1548,7 +1550,7 @@ hipError_t launch_kernel(const BvhView &bvh, const RaysViewT<typename Rec::real>
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !SPLIT) return e;
   // the FP64 launch over the deferred rays (default launch shape of the exact path)
-  const uint32_t ring64 = ring_default<false>();
+  const uint32_t ring64 = ring_default<Rec, false>();
   const size_t lds64 = ring_lds_bytes<false, R>(256, ring64);
   uint64_t blocks = 0;
   e = persistent_blocks(k_trace_persistent<Rec, false>, 256, lds64, n, blocks);
@@ -1564,7 +1566,7 @@ template <class Rec, bool SPLIT>
 hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t total, const TraceLaunch &cfg,
                                hipStream_t stream) {
   const uint32_t block = cfg.block ? cfg.block : 256;
-  const uint32_t ring = cfg.ring ? cfg.ring : ring_default<SPLIT>();
+  const uint32_t ring = cfg.ring ? cfg.ring : ring_default<Rec, SPLIT>();
   const size_t lds = ring_lds_bytes<SPLIT>(block, ring);
   unsigned long long *grp = cfg.counter;
   if (!grp || njobs == 0) return hipErrorInvalidValue;
@@ -1578,7 +1580,7 @@ hipError_t launch_batch_kernel(const BatchJob *jobs, uint32_t njobs, uint64_t to
                      cfg.flags, grp + COUNTER_POOL, refill, ring, grp + COUNTER_DEFERRED, cfg.pool ? cfg.pool : POOL_CHUNK);
   e = hipGetLastError();
   if (e != hipSuccess || !SPLIT) return e;
-  const uint32_t ring64 = ring_default<false>();
+  const uint32_t ring64 = ring_default<Rec, false>();
   const size_t lds64 = ring_lds_bytes<false>(256, ring64);
   e = persistent_blocks(k_trace_batch<Rec, false>, 256, lds64, total, blocks);
   if (e != hipSuccess) return e;

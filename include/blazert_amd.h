@@ -116,7 +116,8 @@ typedef struct brt_launch {
   uint32_t refill;     /* persistent: refill a wave once this many of its 64 lanes are idle; 0 = default (32 for
                           triangles on the exact paths, 16 otherwise) */
   uint32_t sort;       /* BRT_SORT_*: trace rays in a sorted order (results are written in input order) */
-  uint32_t stack_ring; /* traversal-stack entries per lane kept in LDS: 4, 8 or 16; 0 = 8 (FP64) or 16 (split).
+  uint32_t stack_ring; /* traversal-stack entries per lane kept in LDS: 4, 8 or 16; 0 = 16 for split-precision
+                          triangles, 8 otherwise.
                           Deeper entries spill to private memory; a deeper ring spills less but takes more LDS */
   uint32_t pool_size;  /* persistent: ray indices a wave claims at once: 32, 64 or 128; 0 = 64. Smaller claims
                           balance image-order (coherent) rays better, larger ones cost fewer atomics on the launch's
